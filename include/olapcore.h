@@ -1,0 +1,189 @@
+/*
+ * olapcore.h -- C ABI of the MI355X star-join group-by library (libolapcore.so).
+ *
+ * The reference (panjd123/memory_database) exposes its OLAPcore operator as
+ * C++ free functions over raw column pointers, e.g.
+ *   int OLAPcore_rowwise(int8_t **dimvec_array, int32_t **fk_array, int *orders,
+ *                        int &dimvec_nums, int *factor, int &size_lineorder,
+ *                        int &group_nums, int32_t *M1, int32_t *M2, int nthreads);
+ * (code/OLAPcore/test_multi_compute_operator.cpp:973). This header carries the
+ * same arguments -- per-table dimension vectors and foreign-key columns, the
+ * probe order, the mixed-radix factors, the fact-row count, the group count
+ * and the two measure columns -- in two plain structs, and returns explicit
+ * status codes instead of exit(-1).
+ *
+ * Conventions
+ *  - All column and output pointers are DEVICE pointers (HBM-resident) unless
+ *    a parameter says "host". Streams are hipStream_t passed as void* (NULL =
+ *    the null stream). Entry points that return a host value synchronise the
+ *    stream; the others are asynchronous.
+ *  - Dimension vectors hold one code per dimension row: int8 with NULL 127
+ *    (DIM_NULL, include/metadata.h:69) or int32 with NULL -1.
+ *  - A fact row passes when every probed dimension yields a non-NULL code. A
+ *    foreign key outside [0, dim_rows) finds no dimension row and fails (the
+ *    reference reads past the vector there, include/gendata_util.hpp:530-534).
+ *  - gid = sum_i code_i * factor[i]; sum[gid] += (int64)m1 + (int64)m2 and
+ *    count[gid] += 1, in int64 (the reference's uint32 vector wraps).
+ *  - Results are bit-exact and independent of strategy and launch shape.
+ *  - A context is not thread-safe; use one context per host thread.
+ */
+#ifndef OLAPCORE_H
+#define OLAPCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OLAP_MAX_DIMS 4
+#define OLAP_DIM_NULL_I8 127    /* DIM_NULL, include/metadata.h:69 */
+#define OLAP_DIM_NULL_I32 (-1)
+#define OLAP_GID_NULL (-1)      /* widened GROUP_NULL (include/metadata.h:70) */
+#define OLAP_STARJOIN_NULL 32767 /* GROUP_NULL as the star-join result uses it */
+
+typedef enum {
+    OLAP_OK = 0,
+    OLAP_E_INVALID = 1,   /* bad argument or input data (message in olap_last_error) */
+    OLAP_E_HIP = 2,       /* a HIP runtime call failed */
+    OLAP_E_NOMEM = 3,     /* device allocation failed */
+    OLAP_E_CAPACITY = 4,  /* request beyond a documented limit */
+    OLAP_E_NODEVICE = 5,  /* no usable GPU */
+    OLAP_E_BUSY = 6       /* admission refused (server) */
+} olap_status;
+
+typedef enum {
+    OLAP_ROWWISE = 0, /* fused probe, per-row early exit   (OLAPcore_row_thread :329) */
+    OLAP_VWM_DV = 1,  /* tile compaction, dynamic vector  (OLAPcore_vwm_dv_thread :375) */
+    OLAP_VWM_SV = 2,  /* tile static vector, no early exit (OLAPcore_vwm_sv_thread :509) */
+    OLAP_CWM_DV = 3,  /* column-wise, materialised lists  (join/agg_cwm_dv :240/:645) */
+    OLAP_CWM_SV = 4   /* column-wise, full gid vector     (join/agg_cwm_sv :290/:662) */
+} olap_strategy;
+
+/* The plan main() builds, test_multi_compute_operator.cpp:1964-1987. */
+typedef struct {
+    int32_t dimvec_nums;              /* probe steps, 1..4 */
+    int32_t orders[OLAP_MAX_DIMS];    /* table slot probed at step i */
+    int32_t factor[OLAP_MAX_DIMS];    /* multiplier of step i's code, >= 0 */
+    int64_t group_nums;               /* slots in the outputs, 1..2^31-1 */
+} olap_plan;
+
+/* The column arguments of the reference operators, indexed by table slot
+ * (0 customer, 1 supplier, 2 part, 3 date as in the reference's main). */
+typedef struct {
+    const void *dimvec[OLAP_MAX_DIMS];  /* device; codes per dimension row */
+    int64_t dim_rows[OLAP_MAX_DIMS];    /* entries in dimvec[t], < 2^31 */
+    int32_t code_bytes[OLAP_MAX_DIMS];  /* 1 (int8, NULL 127) or 4 (int32, NULL -1) */
+    const int32_t *fk[OLAP_MAX_DIMS];   /* device; foreign keys, rows entries */
+    const int32_t *m1;                  /* device; measure */
+    const int32_t *m2;                  /* device; second measure or NULL */
+    int64_t rows;                       /* fact rows (size_lineorder), < 2^32 */
+} olap_columns;
+
+typedef struct olap_ctx_s *olap_ctx;
+typedef struct olap_query_s *olap_query;
+
+/* ---- context and memory ------------------------------------------------ */
+olap_status olap_ctx_create(int device, olap_ctx *out);
+olap_status olap_ctx_destroy(olap_ctx ctx);
+const char *olap_last_error(olap_ctx ctx);
+olap_status olap_device_info(olap_ctx ctx, int32_t *num_cus, int64_t *hbm_bytes,
+                             char *arch_name, size_t arch_len);
+olap_status olap_dev_alloc(olap_ctx ctx, size_t bytes, void **out);
+olap_status olap_dev_free(olap_ctx ctx, void *p);
+olap_status olap_copy_h2d(olap_ctx ctx, void *dst, const void *src, size_t bytes, void *stream);
+olap_status olap_copy_d2h(olap_ctx ctx, void *dst, const void *src, size_t bytes, void *stream);
+olap_status olap_stream_sync(olap_ctx ctx, void *stream);
+
+/* ---- synthetic data (counter-based, reproducible on the CPU) ------------ */
+/* out[i] = lo + floor(u * (hi - lo)) with u from SplitMix64(seed, stream, first_row + i) */
+olap_status olap_gen_uniform_i32(olap_ctx ctx, int32_t *out, int64_t n, int64_t first_row,
+                                 uint64_t seed, uint64_t stream_id, int32_t lo, int32_t hi,
+                                 void *stream);
+/* host: inverse-CDF thresholds of Zipf(theta) over d ranks (d entries) */
+olap_status olap_zipf_table(int64_t d, double theta, uint64_t *thr_host);
+/* out[i] = rank drawn through a device copy of the table above */
+olap_status olap_gen_zipf_i32(olap_ctx ctx, int32_t *out, int64_t n, int64_t first_row,
+                              uint64_t seed, uint64_t stream_id, const uint64_t *thr_dev,
+                              int64_t d, void *stream);
+
+/* ---- plan ---------------------------------------------------------------- */
+/* sort_by_rate (:223-233, quirk kept) + group_nums/factor (:1964-1987).
+ * groups[t] == 0 leaves table t out; bitmap[t] == 0 makes it filter-only. */
+olap_status olap_make_plan(const int32_t groups[OLAP_MAX_DIMS], const int32_t bitmap[OLAP_MAX_DIMS],
+                           const double sele[OLAP_MAX_DIMS], olap_plan *out);
+
+/* ---- step 1: filter a dimension into a dimension vector ------------------- */
+/* out[r] = (lo <= filter[r] < hi) ? (bitmap ? (group_mod > 0 ? group[r] % group_mod
+ *          : group[r]) : 0) : NULL; code_bytes 1 or 4. The dimension-vector
+ * build of code/TPCH_Q5/Q5.cpp:921-944 and include/gendata_util.hpp:473-527. */
+olap_status olap_build_dimvec(olap_ctx ctx, const int32_t *filter, const int32_t *group,
+                              int64_t rows, int32_t lo, int32_t hi, int32_t bitmap,
+                              int32_t group_mod, int32_t code_bytes, void *out, void *stream);
+
+/* ---- steps 2-4: probe, compose, group-by SUM ----------------------------- */
+/* Validate and pack the dimension vectors once (synchronises). Rejects
+ * negative non-NULL codes and plans whose largest gid reaches group_nums. */
+olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                              olap_query *out);
+/* Zero sum/count (group_nums int64 each, device) and run one strategy. */
+olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
+                           void *stream);
+olap_status olap_query_destroy(olap_query q);
+/* bytes of device memory the query holds (packed vectors, scratch) */
+int64_t olap_query_device_bytes(olap_query q);
+
+/* One-shot forms named after the reference drivers; they create, run and
+ * destroy a query and synchronise. total_count (host, nullable) receives the
+ * sum of the count vector. */
+olap_status olap_rowwise(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                         int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+olap_status olap_vwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+olap_status olap_vwm_sv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+olap_status olap_cwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+olap_status olap_cwm_sv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+
+/* The column-wise pairs split as in the reference: join_cwm_sv (:908) writes
+ * a per-row gid vector (rows int32, device, OLAP_GID_NULL for failing rows),
+ * agg_cwm_sv (:764) aggregates it. join_cwm_dv (:840) leaves the surviving
+ * (row, gid) lists in the query; agg_cwm_dv (:685) aggregates them. */
+olap_status olap_join_cwm_sv(olap_query q, int32_t *gid_vec, void *stream);
+olap_status olap_agg_cwm_sv(olap_query q, const int32_t *gid_vec, int64_t *sum, int64_t *count,
+                            void *stream);
+olap_status olap_join_cwm_dv(olap_query q, void *stream);
+olap_status olap_agg_cwm_dv(olap_query q, int64_t *sum, int64_t *count, void *stream);
+
+/* host helper: the int every reference driver returns -- the uint32 group
+ * vector summed into an int (e.g. :1027-1036) -- from int64 sums (host). */
+int32_t olap_legacy_count(const int64_t *sum_host, int64_t group_nums);
+
+/* ---- sub-step operators --------------------------------------------------- */
+/* starjoinalgo_rowwise / _cwm_sv / _cwm_dv (code/starjoin/test_starjoin_option.cpp:37-189):
+ * rows passing all four int8 dimension vectors (c, s, p, d). result (device,
+ * nullable) receives c<<12 | s<<8 | p<<4 | d per row, OLAP_STARJOIN_NULL when
+ * the row fails. count is a host pointer. */
+olap_status olap_starjoin(olap_ctx ctx, olap_strategy strategy, int64_t rows,
+                          const int8_t *const dimvec[4], const int64_t dim_rows[4],
+                          const int32_t *const fk[4], int32_t *result, int64_t *count,
+                          void *stream);
+/* vector_group (code/group/test_group_option.cpp:44-60): group_vector[key-1] +=
+ * m1 + m2 over 1-based keys (int64, device, zeroed here); total (host) = sum
+ * of the vector; bad_keys (host, nullable) = keys outside [1, group_num]. */
+olap_status olap_vector_group(olap_ctx ctx, int64_t rows, const int32_t *key1,
+                              const int32_t *m1, const int32_t *m2, int64_t *group_vector,
+                              int64_t group_num, int64_t *total, int64_t *bad_keys,
+                              void *stream);
+/* aggalgo_rowwise (code/aggregation/test_aggregation_option.cpp:28-37) in fixed
+ * point: SUM(p*q*t + 100*p*q) with t the tax in percent; host result. */
+olap_status olap_agg_expr(olap_ctx ctx, int64_t rows, const int32_t *p, const int32_t *q,
+                          const int32_t *t, int64_t *result, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,851 @@
+// olap_capi.cpp -- host side of the C ABI declared in include/olapcore.h.
+//
+// A query is created once per (plan, columns): the dimension vectors are
+// validated on the device (max code, negative codes) and re-packed at the
+// smallest power-of-two code width whose all-ones value is free for NULL.
+// Each run then chooses the LDS residency of the packed vectors and of the
+// group table for the requested strategy and launches persistent kernels.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "olap_internal.h"
+
+using namespace olap;
+
+namespace {
+
+olap_status fail(olap_ctx ctx, olap_status st, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return st;
+}
+
+olap_status hip_fail(olap_ctx ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, e == hipErrorOutOfMemory ? OLAP_E_NOMEM : OLAP_E_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define HIPC(ctx, call)                                   \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+uint64_t host_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t stream_key(uint64_t seed, uint64_t stream)
+{
+    return host_mix64(seed ^ host_mix64(stream + 0x632be59bd9b4e019ULL));
+}
+
+void dev_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+uint32_t align_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
+
+// LDS residency of one run. Steps are placed in probe order (step 0 is read
+// for every fact row), then the group table.
+struct Layout {
+    StepDesc st[OLAP_MAX_DIMS];
+    uint32_t gt_lds = 0, gt_off = 0, list_off = 0, dyn_lds = 0;
+};
+
+Layout plan_layout(const olap_query_s *q, bool with_steps, bool with_table, bool with_list,
+                   int only_step)
+{
+    Layout L;
+    const int nd = q->plan.dimvec_nums;
+    for (int s = 0; s < OLAP_MAX_DIMS; s++) {
+        L.st[s] = q->steps[s];
+        L.st[s].in_lds = 0;
+        L.st[s].lds_off = 0;
+    }
+    uint32_t cap = kLdsBytes - (with_list ? kLdsListBytes : 0u) - 16u;
+    uint32_t cur = 0;  // bytes
+    if (with_steps) {
+        for (int s = 0; s < nd; s++) {
+            if (only_step >= 0 && s != only_step) continue;
+            const uint32_t bytes = align_up(q->steps[s].nwords * 4u, 16u);
+            if (cur + bytes <= cap) {
+                L.st[s].in_lds = 1;
+                L.st[s].lds_off = cur / 4u;
+                cur += bytes;
+            }
+        }
+    }
+    const uint64_t G = (uint64_t)q->plan.group_nums;
+    if (with_table) {
+        const uint64_t tb = G * 12u;
+        const uint32_t off = align_up(cur, 16u);
+        if (off + tb <= cap) {
+            L.gt_lds = 1;
+            L.gt_off = off / 4u;
+            cur = off + (uint32_t)align_up((uint32_t)tb, 16u);
+        }
+    }
+    if (with_list) {
+        cur = align_up(cur, 16u);
+        L.list_off = cur / 4u;
+        cur += kLdsListBytes;
+    }
+    L.dyn_lds = std::max<uint32_t>(cur, 16u);
+    return L;
+}
+
+void fill_args(ScanArgs &a, const olap_query_s *q, const Layout &L, int64_t *sum, int64_t *count)
+{
+    memset(&a, 0, sizeof(a));
+    for (int s = 0; s < OLAP_MAX_DIMS; s++) a.st[s] = L.st[s];
+    a.m1 = q->cols.m1;
+    a.m2 = q->cols.m2;
+    a.rows = (uint64_t)q->cols.rows;
+    a.gsum = (unsigned long long *)sum;
+    a.gcnt = (unsigned long long *)count;
+    a.err = q->ctx->d_err;
+    a.G = (uint32_t)q->plan.group_nums;
+    a.gt_lds = L.gt_lds;
+    a.gt_off = L.gt_off;
+    a.list_off = L.list_off;
+    a.count_only = q->cols.m1 == nullptr ? 1u : 0u;
+    a.gvec_null = OLAP_GID_NULL;
+}
+
+bool columns_vec_ok(const olap_query_s *q)
+{
+    for (int s = 0; s < q->plan.dimvec_nums; s++)
+        if (!aligned16(q->steps[s].fk)) return false;
+    if (q->cols.m1 && !aligned16(q->cols.m1)) return false;
+    if (q->cols.m2 && !aligned16(q->cols.m2)) return false;
+    return true;
+}
+
+olap_status ensure_gvec(olap_query_s *q)
+{
+    const size_t need = (size_t)q->cols.rows + 4;
+    if (q->gvec && q->gvec_rows >= need) return OLAP_OK;
+    if (q->gvec) dev_free(q->gvec);
+    q->gvec = nullptr;
+    HIPC(q->ctx, hipMalloc(&q->gvec, need * sizeof(int32_t)));
+    q->gvec_rows = need;
+    return OLAP_OK;
+}
+
+olap_status ensure_lists(olap_query_s *q)
+{
+    const size_t tiles = ((size_t)q->cols.rows + kTileV - 1) / kTileV;
+    const size_t need = tiles * kTileV;
+    if (q->list_oid && q->list_rows >= need) return OLAP_OK;
+    if (q->list_oid) dev_free(q->list_oid);
+    if (q->list_gid) dev_free(q->list_gid);
+    if (q->list_cnt) dev_free(q->list_cnt);
+    q->list_oid = nullptr;
+    q->list_gid = nullptr;
+    q->list_cnt = nullptr;
+    HIPC(q->ctx, hipMalloc(&q->list_oid, std::max<size_t>(need, 1) * sizeof(uint32_t)));
+    HIPC(q->ctx, hipMalloc(&q->list_gid, std::max<size_t>(need, 1) * sizeof(int32_t)));
+    HIPC(q->ctx, hipMalloc(&q->list_cnt, std::max<size_t>(tiles, 1) * sizeof(uint32_t)));
+    q->list_rows = need;
+    return OLAP_OK;
+}
+
+olap_status zero_outputs(olap_query_s *q, int64_t *sum, int64_t *count, hipStream_t s)
+{
+    const size_t G = (size_t)q->plan.group_nums;
+    HIPC(q->ctx, launch_zero_u64((unsigned long long *)sum, G, s));
+    HIPC(q->ctx, launch_zero_u64((unsigned long long *)count, G, s));
+    return OLAP_OK;
+}
+
+olap_status run_cwm_sv_joins(olap_query_s *q, int32_t *gvec, int32_t gvec_null, hipStream_t s)
+{
+    olap_ctx ctx = q->ctx;
+    const int nd = q->plan.dimvec_nums;
+    const bool vec = columns_vec_ok(q) && aligned16(gvec);
+    for (int step = 0; step < nd; step++) {
+        Layout L = plan_layout(q, true, false, false, step);
+        if (L.st[step].in_lds) L.st[step].lds_off = 0;
+        ScanArgs a;
+        fill_args(a, q, L, nullptr, nullptr);
+        a.gvec = gvec;
+        a.step = step;
+        a.last_step = step == nd - 1;
+        a.gvec_null = gvec_null;
+        HIPC(ctx, run_cwm_sv_join(a, ctx->num_cus, L.dyn_lds, vec, s));
+    }
+    return OLAP_OK;
+}
+
+olap_status run_cwm_sv_aggregate(olap_query_s *q, const int32_t *gvec, int64_t *sum,
+                                 int64_t *count, hipStream_t s)
+{
+    Layout L = plan_layout(q, false, true, false, -1);
+    ScanArgs a;
+    fill_args(a, q, L, sum, count);
+    a.gvec = const_cast<int32_t *>(gvec);
+    const bool vec = columns_vec_ok(q) && aligned16(gvec);
+    HIPC(q->ctx, run_cwm_sv_agg(a, q->ctx->num_cus, L.dyn_lds, vec, s));
+    return OLAP_OK;
+}
+
+olap_status run_cwm_dv_joins(olap_query_s *q, hipStream_t s)
+{
+    olap_ctx ctx = q->ctx;
+    olap_status st = ensure_lists(q);
+    if (st != OLAP_OK) return st;
+    const bool vec = columns_vec_ok(q);
+    for (int step = 0; step < q->plan.dimvec_nums; step++) {
+        Layout L = plan_layout(q, true, false, true, step);
+        if (L.st[step].in_lds) L.st[step].lds_off = 0;
+        ScanArgs a;
+        fill_args(a, q, L, nullptr, nullptr);
+        a.step = step;
+        a.list_oid = q->list_oid;
+        a.list_gid = q->list_gid;
+        a.list_cnt = q->list_cnt;
+        HIPC(ctx, run_cwm_dv_join(a, ctx->num_cus, L.dyn_lds, vec, s));
+    }
+    q->joined_dv = true;
+    return OLAP_OK;
+}
+
+olap_status run_cwm_dv_aggregate(olap_query_s *q, int64_t *sum, int64_t *count, hipStream_t s)
+{
+    Layout L = plan_layout(q, false, true, false, -1);
+    ScanArgs a;
+    fill_args(a, q, L, sum, count);
+    a.list_oid = q->list_oid;
+    a.list_gid = q->list_gid;
+    a.list_cnt = q->list_cnt;
+    HIPC(q->ctx, run_cwm_dv_agg(a, q->ctx->num_cus, L.dyn_lds, s));
+    return OLAP_OK;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+olap_status one_shot(olap_ctx ctx, olap_strategy strat, const olap_plan *plan,
+                     const olap_columns *cols, int64_t *sum, int64_t *count, int64_t *total,
+                     void *stream)
+{
+    olap_query q = nullptr;
+    olap_status st = olap_query_create(ctx, plan, cols, &q);
+    if (st != OLAP_OK) return st;
+    st = olap_query_run(q, strat, sum, count, stream);
+    if (st == OLAP_OK) {
+        DeviceGuard g(ctx->device);
+        hipStream_t s = as_stream(stream);
+        unsigned long long *tot = (unsigned long long *)ctx->d_scratch;
+        hipError_t e = launch_zero_u64(tot, 1, s);
+        if (e == hipSuccess)
+            e = launch_sum_u64((const unsigned long long *)count, plan->group_nums, tot,
+                               ctx->num_cus, s);
+        uint32_t herr = 0;
+        unsigned long long htot = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&htot, tot, 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&herr, ctx->d_err, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "one-shot finish");
+        else if (herr) st = fail(ctx, OLAP_E_INVALID, "%u rows produced a gid outside group_nums", herr);
+        else if (total) *total = (int64_t)htot;
+    }
+    olap_query_destroy(q);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+olap_status olap_ctx_create(int device, olap_ctx *out)
+{
+    if (!out) return OLAP_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return OLAP_E_NODEVICE;
+    if (device < 0 || device >= n) return OLAP_E_INVALID;
+    olap_ctx ctx = new olap_ctx_s();
+    ctx->device = device;
+    DeviceGuard g(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete ctx;
+        return OLAP_E_HIP;
+    }
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->hbm_bytes = (int64_t)prop.totalGlobalMem;
+    snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
+    ctx->scratch_bytes = 256;
+    if (hipMalloc(&ctx->d_err, 16) != hipSuccess ||
+        hipMalloc(&ctx->d_scratch, ctx->scratch_bytes) != hipSuccess) {
+        if (ctx->d_err) dev_free(ctx->d_err);
+        delete ctx;
+        return OLAP_E_NOMEM;
+    }
+    if (launch_zero_u32(ctx->d_err, 4, nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        dev_free(ctx->d_err);
+        dev_free(ctx->d_scratch);
+        delete ctx;
+        return OLAP_E_HIP;
+    }
+    *out = ctx;
+    return OLAP_OK;
+}
+
+olap_status olap_ctx_destroy(olap_ctx ctx)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    {
+        DeviceGuard g(ctx->device);
+        dev_free(ctx->d_err);
+        dev_free(ctx->d_scratch);
+    }
+    delete ctx;
+    return OLAP_OK;
+}
+
+const char *olap_last_error(olap_ctx ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+olap_status olap_device_info(olap_ctx ctx, int32_t *num_cus, int64_t *hbm_bytes, char *arch_name,
+                             size_t arch_len)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (num_cus) *num_cus = ctx->num_cus;
+    if (hbm_bytes) *hbm_bytes = ctx->hbm_bytes;
+    if (arch_name && arch_len) snprintf(arch_name, arch_len, "%s", ctx->arch);
+    return OLAP_OK;
+}
+
+olap_status olap_dev_alloc(olap_ctx ctx, size_t bytes, void **out)
+{
+    if (!ctx || !out) return OLAP_E_INVALID;
+    DeviceGuard g(ctx->device);
+    *out = nullptr;
+    HIPC(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return OLAP_OK;
+}
+
+olap_status olap_dev_free(olap_ctx ctx, void *p)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    DeviceGuard g(ctx->device);
+    if (p) HIPC(ctx, hipFree(p));
+    return OLAP_OK;
+}
+
+olap_status olap_copy_h2d(olap_ctx ctx, void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!ctx || (bytes && (!dst || !src))) return OLAP_E_INVALID;
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    HIPC(ctx, hipStreamSynchronize(as_stream(stream)));
+    return OLAP_OK;
+}
+
+olap_status olap_copy_d2h(olap_ctx ctx, void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!ctx || (bytes && (!dst || !src))) return OLAP_E_INVALID;
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    HIPC(ctx, hipStreamSynchronize(as_stream(stream)));
+    return OLAP_OK;
+}
+
+olap_status olap_stream_sync(olap_ctx ctx, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, hipStreamSynchronize(as_stream(stream)));
+    return OLAP_OK;
+}
+
+olap_status olap_gen_uniform_i32(olap_ctx ctx, int32_t *out, int64_t n, int64_t first_row,
+                                 uint64_t seed, uint64_t stream_id, int32_t lo, int32_t hi,
+                                 void *stream)
+{
+    if (!ctx || n < 0 || (n && !out) || first_row < 0) return fail(ctx, OLAP_E_INVALID, "gen_uniform: bad arguments");
+    if ((int64_t)hi <= (int64_t)lo) return fail(ctx, OLAP_E_INVALID, "gen_uniform: empty range");
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, launch_gen_uniform(out, n, first_row, stream_key(seed, stream_id), lo, hi,
+                                 as_stream(stream)));
+    return OLAP_OK;
+}
+
+olap_status olap_zipf_table(int64_t d, double theta, uint64_t *thr)
+{
+    if (d <= 0 || !thr || theta < 0.0) return OLAP_E_INVALID;
+    double total = 0.0;
+    for (int64_t k = 0; k < d; k++) total += 1.0 / pow((double)(k + 1), theta);
+    double acc = 0.0;
+    for (int64_t k = 0; k < d; k++) {
+        acc += 1.0 / pow((double)(k + 1), theta);
+        double v = floor(acc / total * 4294967296.0);
+        if (v > 4294967296.0) v = 4294967296.0;
+        thr[k] = (uint64_t)v;
+    }
+    thr[d - 1] = 4294967296ULL;
+    return OLAP_OK;
+}
+
+olap_status olap_gen_zipf_i32(olap_ctx ctx, int32_t *out, int64_t n, int64_t first_row,
+                              uint64_t seed, uint64_t stream_id, const uint64_t *thr_dev,
+                              int64_t d, void *stream)
+{
+    if (!ctx || n < 0 || (n && (!out || !thr_dev)) || d <= 0 || d > INT32_MAX)
+        return fail(ctx, OLAP_E_INVALID, "gen_zipf: bad arguments");
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, launch_gen_zipf(out, n, first_row, stream_key(seed, stream_id), thr_dev, d,
+                              as_stream(stream)));
+    return OLAP_OK;
+}
+
+// sort_by_rate (test_multi_compute_operator.cpp:223-233): the comparison reads
+// sele[] by position while only orders[] moves -- kept, since it fixes the
+// reference's slot layout. Then main's group_nums / factor loop (:1977-1987).
+olap_status olap_make_plan(const int32_t groups[OLAP_MAX_DIMS], const int32_t bitmap[OLAP_MAX_DIMS],
+                           const double sele[OLAP_MAX_DIMS], olap_plan *out)
+{
+    if (!groups || !bitmap || !sele || !out) return OLAP_E_INVALID;
+    olap_plan p;
+    memset(&p, 0, sizeof(p));
+    int32_t n = 0;
+    for (int32_t t = 0; t < OLAP_MAX_DIMS; t++) {
+        if (groups[t] < 0) return OLAP_E_INVALID;
+        if (groups[t] != 0) p.orders[n++] = t;
+    }
+    if (n == 0) return OLAP_E_INVALID;
+    p.dimvec_nums = n;
+    for (int32_t i = 0; i < n; i++)
+        for (int32_t j = i + 1; j < n; j++)
+            if (sele[i] > sele[j]) std::swap(p.orders[i], p.orders[j]);
+    int64_t G = 1;
+    for (int32_t i = 0; i < OLAP_MAX_DIMS; i++) p.factor[i] = 1;
+    for (int32_t i = 0; i < n; i++) {
+        G *= groups[p.orders[i]];
+        if (G > INT32_MAX) return OLAP_E_CAPACITY;
+        int64_t f = 1;
+        for (int32_t j = i + 1; j < n; j++) {
+            if (bitmap[p.orders[i]] == 0) break;
+            f *= groups[p.orders[j]];
+        }
+        p.factor[i] = (int32_t)f;
+    }
+    p.group_nums = G;
+    *out = p;
+    return OLAP_OK;
+}
+
+olap_status olap_build_dimvec(olap_ctx ctx, const int32_t *filter, const int32_t *group,
+                              int64_t rows, int32_t lo, int32_t hi, int32_t bitmap,
+                              int32_t group_mod, int32_t code_bytes, void *out, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || (rows && (!filter || !out || (bitmap && !group))))
+        return fail(ctx, OLAP_E_INVALID, "build_dimvec: bad arguments");
+    if (code_bytes != 1 && code_bytes != 4)
+        return fail(ctx, OLAP_E_INVALID, "build_dimvec: code_bytes must be 1 or 4");
+    if (group_mod < 0) return fail(ctx, OLAP_E_INVALID, "build_dimvec: group_mod < 0");
+    if (code_bytes == 1 && bitmap && (group_mod <= 0 || group_mod > OLAP_DIM_NULL_I8))
+        return fail(ctx, OLAP_E_INVALID,
+                    "build_dimvec: int8 codes need 0 < group_mod <= 127 to stay clear of NULL");
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, launch_build_dimvec(filter, group, rows, lo, hi, bitmap, group_mod, code_bytes, out,
+                                  as_stream(stream)));
+    return OLAP_OK;
+}
+
+olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                              olap_query *out)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (!plan || !cols || !out) return fail(ctx, OLAP_E_INVALID, "query_create: null argument");
+    *out = nullptr;
+    const int nd = plan->dimvec_nums;
+    if (nd < 1 || nd > OLAP_MAX_DIMS) return fail(ctx, OLAP_E_INVALID, "dimvec_nums must be 1..4");
+    if (plan->group_nums < 1 || plan->group_nums > INT32_MAX)
+        return fail(ctx, OLAP_E_INVALID, "group_nums must be in [1, 2^31)");
+    if (cols->rows < 0 || cols->rows > (int64_t)UINT32_MAX - kTileRW)
+        return fail(ctx, OLAP_E_CAPACITY, "rows must be in [0, 2^32 - %d)", kTileRW);
+    bool seen[OLAP_MAX_DIMS] = {false, false, false, false};
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        if (t < 0 || t >= OLAP_MAX_DIMS || seen[t])
+            return fail(ctx, OLAP_E_INVALID, "orders must be distinct table slots 0..3");
+        seen[t] = true;
+        if (plan->factor[s] < 0) return fail(ctx, OLAP_E_INVALID, "factor must be >= 0");
+        if (cols->dim_rows[t] < 0 || cols->dim_rows[t] > INT32_MAX)
+            return fail(ctx, OLAP_E_CAPACITY, "dim_rows of table %d out of range", t);
+        if (cols->code_bytes[t] != 1 && cols->code_bytes[t] != 4)
+            return fail(ctx, OLAP_E_INVALID, "code_bytes of table %d must be 1 or 4", t);
+        if (cols->rows > 0 && !cols->fk[t])
+            return fail(ctx, OLAP_E_INVALID, "fk column of table %d is null", t);
+        if (cols->dim_rows[t] > 0 && !cols->dimvec[t])
+            return fail(ctx, OLAP_E_INVALID, "dimension vector of table %d is null", t);
+    }
+    DeviceGuard g(ctx->device);
+    olap_query q = new olap_query_s();
+    q->ctx = ctx;
+    q->plan = *plan;
+    q->cols = *cols;
+
+    // validation: max code and negative non-NULL codes per step
+    std::vector<int32_t> hstats(2 * nd);
+    for (int s = 0; s < nd; s++) {
+        hstats[2 * s] = -1;
+        hstats[2 * s + 1] = 0;
+    }
+    int32_t *dstats = (int32_t *)ctx->d_scratch;
+    hipStream_t s0 = nullptr;
+    hipError_t e = hipMemcpyAsync(dstats, hstats.data(), 8 * nd, hipMemcpyHostToDevice, s0);
+    for (int s = 0; s < nd && e == hipSuccess; s++) {
+        const int t = plan->orders[s];
+        e = launch_dim_stats(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t],
+                             dstats + 2 * s, s0);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hstats.data(), dstats, 8 * nd, hipMemcpyDeviceToHost, s0);
+    if (e == hipSuccess) e = hipStreamSynchronize(s0);
+    if (e != hipSuccess) {
+        delete q;
+        return hip_fail(ctx, e, "query_create: validation");
+    }
+    int64_t max_gid = 0;
+    uint32_t lw[OLAP_MAX_DIMS] = {0, 0, 0, 0};
+    uint32_t nwords[OLAP_MAX_DIMS] = {0, 0, 0, 0};
+    size_t total_words = 0;
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        if (hstats[2 * s + 1] != 0) {
+            delete q;
+            return fail(ctx, OLAP_E_INVALID, "table %d holds %d negative non-NULL codes", t,
+                        hstats[2 * s + 1]);
+        }
+        const int64_t mx = hstats[2 * s];
+        if (mx > 0) max_gid += mx * (int64_t)plan->factor[s];
+        // smallest power-of-two width whose all-ones value exceeds every code
+        uint32_t l = 0;
+        while (l < 5 && ((1ull << (1u << l)) - 1ull) <= (uint64_t)(mx < 0 ? 0 : mx)) l++;
+        lw[s] = l;
+        const uint64_t per = 32u >> l;
+        nwords[s] = (uint32_t)(((uint64_t)cols->dim_rows[t] + per - 1) / per);
+        total_words += align_up(nwords[s], 4u);
+    }
+    if (max_gid >= plan->group_nums) {
+        delete q;
+        return fail(ctx, OLAP_E_INVALID,
+                    "largest reachable gid %lld is outside group_nums %lld (factor/codes mismatch)",
+                    (long long)max_gid, (long long)plan->group_nums);
+    }
+    q->max_gid = max_gid;
+    e = hipMalloc(&q->packed, std::max<size_t>(total_words, 4) * 4);
+    if (e != hipSuccess) {
+        delete q;
+        return hip_fail(ctx, e, "query_create: packed vectors");
+    }
+    q->packed_words = total_words;
+    size_t off = 0;
+    for (int s = 0; s < OLAP_MAX_DIMS; s++) memset(&q->steps[s], 0, sizeof(StepDesc));
+    for (int s = 0; s < nd && e == hipSuccess; s++) {
+        const int t = plan->orders[s];
+        StepDesc &st = q->steps[s];
+        st.fk = cols->fk[t];
+        st.words = q->packed + off;
+        st.dim_rows = (uint32_t)cols->dim_rows[t];
+        st.lw = lw[s];
+        st.mask = lw[s] == 5 ? 0xffffffffu : ((1u << (1u << lw[s])) - 1u);
+        st.factor = (uint32_t)plan->factor[s];
+        st.nwords = nwords[s];
+        e = launch_pack(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t], lw[s],
+                        q->packed + off, nwords[s], s0);
+        off += align_up(nwords[s], 4u);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s0);
+    if (e != hipSuccess) {
+        dev_free(q->packed);
+        delete q;
+        return hip_fail(ctx, e, "query_create: pack");
+    }
+    *out = q;
+    return OLAP_OK;
+}
+
+olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
+                           void *stream)
+{
+    if (!q) return OLAP_E_INVALID;
+    olap_ctx ctx = q->ctx;
+    if (!count || (!sum && q->cols.m1)) return fail(ctx, OLAP_E_INVALID, "query_run: null output");
+    if (q->cols.rows > 0 && !q->cols.m1 && sum)
+        return fail(ctx, OLAP_E_INVALID, "query_run: sum requested without a measure");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    const int nd = q->plan.dimvec_nums;
+    // a sum pointer is required by the kernels; counting-only runs reuse count
+    int64_t *sum_p = sum ? sum : count;
+    olap_status st = OLAP_OK;
+    if (sum) st = zero_outputs(q, sum, count, s);
+    else HIPC(ctx, launch_zero_u64((unsigned long long *)count, (size_t)q->plan.group_nums, s));
+    if (st != OLAP_OK) return st;
+    HIPC(ctx, launch_zero_u32(ctx->d_err, 1, s));
+    const bool vec = columns_vec_ok(q);
+    switch (strategy) {
+    case OLAP_ROWWISE:
+    case OLAP_VWM_SV:
+    case OLAP_VWM_DV: {
+        Layout L = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1);
+        ScanArgs a;
+        fill_args(a, q, L, sum_p, count);
+        hipError_t e;
+        if (strategy == OLAP_ROWWISE) e = run_rowwise(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
+        else if (strategy == OLAP_VWM_SV) e = run_vwm_sv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
+        else e = run_vwm_dv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "scan kernel");
+        return OLAP_OK;
+    }
+    case OLAP_CWM_SV: {
+        st = ensure_gvec(q);
+        if (st != OLAP_OK) return st;
+        st = run_cwm_sv_joins(q, q->gvec, OLAP_GID_NULL, s);
+        if (st != OLAP_OK) return st;
+        return run_cwm_sv_aggregate(q, q->gvec, sum_p, count, s);
+    }
+    case OLAP_CWM_DV: {
+        st = run_cwm_dv_joins(q, s);
+        if (st != OLAP_OK) return st;
+        return run_cwm_dv_aggregate(q, sum_p, count, s);
+    }
+    }
+    return fail(ctx, OLAP_E_INVALID, "unknown strategy %d", (int)strategy);
+}
+
+olap_status olap_query_destroy(olap_query q)
+{
+    if (!q) return OLAP_E_INVALID;
+    DeviceGuard g(q->ctx->device);
+    if (q->packed) dev_free(q->packed);
+    if (q->list_oid) dev_free(q->list_oid);
+    if (q->list_gid) dev_free(q->list_gid);
+    if (q->list_cnt) dev_free(q->list_cnt);
+    if (q->gvec) dev_free(q->gvec);
+    delete q;
+    return OLAP_OK;
+}
+
+int64_t olap_query_device_bytes(olap_query q)
+{
+    if (!q) return -1;
+    return (int64_t)(q->packed_words * 4 + q->list_rows * 8 +
+                     (q->list_rows / kTileV) * 4 + q->gvec_rows * 4);
+}
+
+olap_status olap_rowwise(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                         int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    return one_shot(ctx, OLAP_ROWWISE, plan, cols, sum, count, total_count, stream);
+}
+olap_status olap_vwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    return one_shot(ctx, OLAP_VWM_DV, plan, cols, sum, count, total_count, stream);
+}
+olap_status olap_vwm_sv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    return one_shot(ctx, OLAP_VWM_SV, plan, cols, sum, count, total_count, stream);
+}
+olap_status olap_cwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    return one_shot(ctx, OLAP_CWM_DV, plan, cols, sum, count, total_count, stream);
+}
+olap_status olap_cwm_sv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                        int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    return one_shot(ctx, OLAP_CWM_SV, plan, cols, sum, count, total_count, stream);
+}
+
+olap_status olap_join_cwm_sv(olap_query q, int32_t *gid_vec, void *stream)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (q->cols.rows > 0 && !gid_vec) return fail(q->ctx, OLAP_E_INVALID, "join_cwm_sv: null gid_vec");
+    DeviceGuard g(q->ctx->device);
+    return run_cwm_sv_joins(q, gid_vec, OLAP_GID_NULL, as_stream(stream));
+}
+
+olap_status olap_agg_cwm_sv(olap_query q, const int32_t *gid_vec, int64_t *sum, int64_t *count,
+                            void *stream)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (!sum || !count || (q->cols.rows > 0 && !gid_vec))
+        return fail(q->ctx, OLAP_E_INVALID, "agg_cwm_sv: null argument");
+    DeviceGuard g(q->ctx->device);
+    hipStream_t s = as_stream(stream);
+    olap_status st = zero_outputs(q, sum, count, s);
+    if (st != OLAP_OK) return st;
+    return run_cwm_sv_aggregate(q, gid_vec, sum, count, s);
+}
+
+olap_status olap_join_cwm_dv(olap_query q, void *stream)
+{
+    if (!q) return OLAP_E_INVALID;
+    DeviceGuard g(q->ctx->device);
+    return run_cwm_dv_joins(q, as_stream(stream));
+}
+
+olap_status olap_agg_cwm_dv(olap_query q, int64_t *sum, int64_t *count, void *stream)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (!sum || !count) return fail(q->ctx, OLAP_E_INVALID, "agg_cwm_dv: null output");
+    if (!q->joined_dv) return fail(q->ctx, OLAP_E_INVALID, "agg_cwm_dv before join_cwm_dv");
+    DeviceGuard g(q->ctx->device);
+    hipStream_t s = as_stream(stream);
+    olap_status st = zero_outputs(q, sum, count, s);
+    if (st != OLAP_OK) return st;
+    return run_cwm_dv_aggregate(q, sum, count, s);
+}
+
+int32_t olap_legacy_count(const int64_t *sum_host, int64_t group_nums)
+{
+    uint32_t acc = 0;
+    for (int64_t g = 0; g < group_nums; g++) acc += (uint32_t)(uint64_t)sum_host[g];
+    return (int32_t)acc;
+}
+
+olap_status olap_starjoin(olap_ctx ctx, olap_strategy strategy, int64_t rows,
+                          const int8_t *const dimvec[4], const int64_t dim_rows[4],
+                          const int32_t *const fk[4], int32_t *result, int64_t *count,
+                          void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (!dimvec || !dim_rows || !fk || !count) return fail(ctx, OLAP_E_INVALID, "starjoin: null argument");
+    olap_plan p;
+    memset(&p, 0, sizeof(p));
+    p.dimvec_nums = 4;
+    olap_columns c;
+    memset(&c, 0, sizeof(c));
+    for (int t = 0; t < 4; t++) {
+        p.orders[t] = t;
+        c.dimvec[t] = dimvec[t];
+        c.dim_rows[t] = dim_rows[t];
+        c.code_bytes[t] = 1;
+        c.fk[t] = fk[t];
+    }
+    c.rows = rows;
+    c.m1 = nullptr;
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    olap_status st;
+    if (result) {
+        // per-row result: fixed 4-bit fields (test_starjoin_option.cpp:76,87,98,109)
+        olap_plan pr = p;
+        pr.factor[0] = 1 << 12;
+        pr.factor[1] = 1 << 8;
+        pr.factor[2] = 1 << 4;
+        pr.factor[3] = 1;
+        pr.group_nums = INT32_MAX;
+        olap_query qr = nullptr;
+        st = olap_query_create(ctx, &pr, &c, &qr);
+        if (st != OLAP_OK) return st;
+        st = run_cwm_sv_joins(qr, result, OLAP_STARJOIN_NULL, s);
+        olap_query_destroy(qr);
+        if (st != OLAP_OK) return st;
+    }
+    // the count: every fact row that passes lands in the single slot
+    p.group_nums = 1;
+    olap_query q = nullptr;
+    st = olap_query_create(ctx, &p, &c, &q);
+    if (st != OLAP_OK) return st;
+    int64_t *dcnt = (int64_t *)ctx->d_scratch + 8;
+    olap_strategy run_as = strategy == OLAP_CWM_SV && result ? OLAP_ROWWISE : strategy;
+    st = olap_query_run(q, run_as, nullptr, dcnt, stream);
+    if (st == OLAP_OK) {
+        hipError_t e = hipMemcpyAsync(count, dcnt, 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "starjoin count");
+    }
+    olap_query_destroy(q);
+    return st;
+}
+
+olap_status olap_vector_group(olap_ctx ctx, int64_t rows, const int32_t *key1, const int32_t *m1,
+                              const int32_t *m2, int64_t *group_vector, int64_t group_num,
+                              int64_t *total, int64_t *bad_keys, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || group_num < 1 || !group_vector || !total || (rows && (!key1 || !m1)))
+        return fail(ctx, OLAP_E_INVALID, "vector_group: bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    unsigned long long *dtot = (unsigned long long *)ctx->d_scratch;
+    HIPC(ctx, launch_zero_u64((unsigned long long *)group_vector, (size_t)group_num, s));
+    HIPC(ctx, launch_zero_u64(dtot, 1, s));
+    HIPC(ctx, launch_zero_u32(ctx->d_err, 1, s));
+    HIPC(ctx, launch_vector_group(rows, key1, m1, m2, (unsigned long long *)group_vector, group_num,
+                                  dtot, ctx->d_err, ctx->num_cus, s));
+    HIPC(ctx, launch_sum_u64((const unsigned long long *)group_vector, group_num, dtot,
+                             ctx->num_cus, s));
+    unsigned long long ht = 0;
+    uint32_t hb = 0;
+    HIPC(ctx, hipMemcpyAsync(&ht, dtot, 8, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipMemcpyAsync(&hb, ctx->d_err, 4, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    *total = (int64_t)ht;
+    if (bad_keys) *bad_keys = hb;
+    return OLAP_OK;
+}
+
+olap_status olap_agg_expr(olap_ctx ctx, int64_t rows, const int32_t *p, const int32_t *q,
+                          const int32_t *t, int64_t *result, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || !result || (rows && (!p || !q || !t)))
+        return fail(ctx, OLAP_E_INVALID, "agg_expr: bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    unsigned long long *dtot = (unsigned long long *)ctx->d_scratch;
+    HIPC(ctx, launch_zero_u64(dtot, 1, s));
+    HIPC(ctx, launch_agg_expr(rows, p, q, t, dtot, ctx->num_cus, s));
+    unsigned long long h = 0;
+    HIPC(ctx, hipMemcpyAsync(&h, dtot, 8, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    *result = (int64_t)h;
+    return OLAP_OK;
+}
+
+}  // extern "C"

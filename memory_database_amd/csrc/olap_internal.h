@@ -1,0 +1,132 @@
+// olap_internal.h -- shared host/device definitions of libolapcore.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "olapcore.h"
+
+namespace olap {
+
+constexpr int kBlock = 512;              // threads per workgroup of the scan kernels
+constexpr int kChunk = 4;                // rows per 128-bit load
+constexpr int kUnroll = 4;               // 128-bit chunks per thread per tile (row-wise, vwm_sv)
+constexpr int kTileRW = kBlock * kChunk * kUnroll;  // 8192 rows
+constexpr int kTileV = 4096;             // rows per tile of the compacting kernels
+constexpr int kPerThreadV = kTileV / kBlock;        // list entries per thread (8)
+constexpr uint32_t kLdsBytes = 160 * 1024;          // LDS per CU on gfx950
+constexpr uint32_t kLdsListBytes = kTileV * 2 + kTileV * 4 + 16;  // u16 off + u32 gid + counter
+
+// One probe step as the kernels see it. Codes of the step's dimension are
+// packed 32/bits per word; the all-ones code is NULL.
+struct StepDesc {
+    const int32_t *fk;       // fact foreign-key column
+    const uint32_t *words;   // packed codes (global copy)
+    uint32_t lds_off;        // word offset of the LDS copy
+    uint32_t in_lds;         // 1 when the kernel reads the LDS copy
+    uint32_t dim_rows;       // keys >= dim_rows find no row
+    uint32_t lw;             // log2(bits per code)
+    uint32_t mask;           // NULL code = (1 << bits) - 1
+    uint32_t factor;         // multiplier of this step's code
+    uint32_t nwords;         // packed words of this step
+    uint32_t pad_;
+};
+
+struct ScanArgs {
+    StepDesc st[OLAP_MAX_DIMS];
+    const int32_t *m1;
+    const int32_t *m2;          // may be null
+    uint64_t rows;
+    uint64_t ntiles;
+    unsigned long long *gsum;   // group_nums slots
+    unsigned long long *gcnt;
+    uint32_t *err;              // rows whose gid fell outside [0, G)
+    uint32_t G;
+    uint32_t gt_lds;            // group table in LDS
+    uint32_t gt_off;            // word offset of the LDS group table (8-byte aligned)
+    uint32_t list_off;          // word offset of the LDS tile list (compacting kernels)
+    uint32_t count_only;        // no measure: only counts
+    // column-wise state
+    int32_t *gvec;              // per-row gid vector (static-vector column-wise)
+    int32_t gvec_null;          // NULL value written into gvec
+    int32_t step;               // probe step of a column-wise pass
+    uint32_t last_step;         // column-wise static vector: final step writes gvec_null
+    uint32_t *list_oid;         // per-tile segments (dynamic-vector column-wise)
+    int32_t *list_gid;
+    uint32_t *list_cnt;         // entries per tile
+};
+
+}  // namespace olap
+
+// Context and query objects (opaque in the C ABI).
+struct olap_ctx_s {
+    int device = 0;
+    int num_cus = 0;
+    int64_t hbm_bytes = 0;
+    char arch[64] = {0};
+    std::string err;
+    uint32_t *d_err = nullptr;     // error counter (4 words)
+    void *d_scratch = nullptr;     // small reduction scratch
+    size_t scratch_bytes = 0;
+};
+
+struct olap_query_s {
+    olap_ctx ctx = nullptr;
+    olap_plan plan{};
+    olap_columns cols{};
+    uint32_t *packed = nullptr;        // all steps' packed words
+    size_t packed_words = 0;
+    olap::StepDesc steps[OLAP_MAX_DIMS]{};
+    int64_t max_gid = 0;
+    uint32_t *list_oid = nullptr;      // column-wise dynamic-vector lists
+    int32_t *list_gid = nullptr;
+    uint32_t *list_cnt = nullptr;
+    size_t list_rows = 0;
+    int32_t *gvec = nullptr;           // column-wise static vector scratch
+    size_t gvec_rows = 0;
+    bool joined_dv = false;
+};
+
+namespace olap {
+
+// launchers, olap_kernels.hip
+hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_gen_uniform(int32_t *out, int64_t n, int64_t first, uint64_t key, int32_t lo,
+                              int32_t hi, hipStream_t s);
+hipError_t launch_gen_zipf(int32_t *out, int64_t n, int64_t first, uint64_t key,
+                           const uint64_t *thr, int64_t d, hipStream_t s);
+hipError_t launch_build_dimvec(const int32_t *filter, const int32_t *group, int64_t rows,
+                               int32_t lo, int32_t hi, int32_t bitmap, int32_t group_mod,
+                               int32_t code_bytes, void *out, hipStream_t s);
+// stats[0] = max code (-1 when all NULL), stats[1] = negative non-NULL codes
+hipError_t launch_dim_stats(const void *dimvec, int32_t code_bytes, int64_t rows,
+                            int32_t *stats, hipStream_t s);
+hipError_t launch_pack(const void *dimvec, int32_t code_bytes, int64_t rows, uint32_t lw,
+                       uint32_t *words, uint32_t nwords, hipStream_t s);
+
+// strategy kernels; args.gsum/gcnt must be zeroed by the caller
+hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                       hipStream_t s);
+hipError_t run_vwm_sv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                      hipStream_t s);
+hipError_t run_vwm_dv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                      hipStream_t s);
+hipError_t run_cwm_sv_join(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s);
+hipError_t run_cwm_sv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s);
+hipError_t run_cwm_dv_join(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s);
+hipError_t run_cwm_dv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, hipStream_t s);
+
+hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
+                               const int32_t *m2, unsigned long long *gv, int64_t G,
+                               unsigned long long *total, uint32_t *bad, int num_cus,
+                               hipStream_t s);
+hipError_t launch_sum_u64(const unsigned long long *v, int64_t n, unsigned long long *out,
+                          int num_cus, hipStream_t s);
+hipError_t launch_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t,
+                           unsigned long long *out, int num_cus, hipStream_t s);
+
+}  // namespace olap

@@ -1,0 +1,1030 @@
+// olap_kernels.hip -- gfx950 kernels of the star-join group-by path.
+//
+// Layout: fact columns are int32 arrays in HBM read with 128-bit loads (4
+// rows per lane); dimension vectors are re-packed to 1/2/4/8/16/32-bit codes
+// (all-ones = NULL) so the probed dimensions sit in LDS; group tables
+// (u64 sum + u32 count per slot) sit in LDS when they fit and are flushed
+// with one global atomic pair per non-empty slot, otherwise rows update the
+// int64 output in HBM with global atomics. Compaction uses wave64 ballot +
+// mbcnt with one LDS atomic per wave.
+#include "olap_internal.h"
+
+namespace olap {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void atomic_add_u64_dev(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_u32_dev(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_u64_lds(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t atomic_add_u32_lds(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+static inline int grid_for(int64_t work_items, int per_block, int cap)
+{
+    int64_t g = (work_items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// ------------------------------------------------------------------------
+// small utility kernels
+// ------------------------------------------------------------------------
+
+__global__ void k_zero_u64(unsigned long long *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ void k_zero_u32(uint32_t *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_zero_u64<<<grid_for((int64_t)n, 256 * 4, 4096), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_zero_u32<<<grid_for((int64_t)n, 256 * 4, 4096), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// synthetic data: SplitMix64 finaliser over (stream key + (row+1)*golden)
+// ------------------------------------------------------------------------
+
+__global__ void k_gen_uniform(int32_t *out, int64_t n, int64_t first, uint64_t key, int32_t lo,
+                              uint64_t range)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t x = mix64(key + (uint64_t)(first + i + 1) * 0x9e3779b97f4a7c15ULL);
+        out[i] = (int32_t)((int64_t)lo + (int64_t)(((x >> 32) * range) >> 32));
+    }
+}
+
+__global__ void k_gen_zipf(int32_t *out, int64_t n, int64_t first, uint64_t key,
+                           const uint64_t *thr, int64_t d)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t u = mix64(key + (uint64_t)(first + i + 1) * 0x9e3779b97f4a7c15ULL) >> 32;
+        int64_t a = 0, b = d - 1;
+        while (a < b) {
+            int64_t m = (a + b) >> 1;
+            if (thr[m] > u) b = m; else a = m + 1;
+        }
+        out[i] = (int32_t)a;
+    }
+}
+
+hipError_t launch_gen_uniform(int32_t *out, int64_t n, int64_t first, uint64_t key, int32_t lo,
+                              int32_t hi, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo);
+    k_gen_uniform<<<grid_for(n, 256 * 8, 8192), 256, 0, s>>>(out, n, first, key, lo, range);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_zipf(int32_t *out, int64_t n, int64_t first, uint64_t key,
+                           const uint64_t *thr, int64_t d, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_gen_zipf<<<grid_for(n, 256 * 8, 8192), 256, 0, s>>>(out, n, first, key, thr, d);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// step 1: filter + group column -> dimension vector
+// ------------------------------------------------------------------------
+
+__global__ void k_build_dimvec(const int32_t *filter, const int32_t *group, int64_t rows,
+                               int32_t lo, int32_t hi, int32_t bitmap, int32_t group_mod,
+                               int32_t code_bytes, void *out)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t f = filter[r];
+        bool pass = f >= lo && f < hi;
+        int32_t code = 0;
+        if (bitmap) {
+            int32_t g = group[r];
+            code = group_mod > 0 ? (int32_t)((uint32_t)g % (uint32_t)group_mod) : g;
+        }
+        if (code_bytes == 1)
+            ((int8_t *)out)[r] = pass ? (int8_t)code : (int8_t)OLAP_DIM_NULL_I8;
+        else
+            ((int32_t *)out)[r] = pass ? code : OLAP_DIM_NULL_I32;
+    }
+}
+
+hipError_t launch_build_dimvec(const int32_t *filter, const int32_t *group, int64_t rows,
+                               int32_t lo, int32_t hi, int32_t bitmap, int32_t group_mod,
+                               int32_t code_bytes, void *out, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    k_build_dimvec<<<grid_for(rows, 256, 4096), 256, 0, s>>>(filter, group, rows, lo, hi, bitmap,
+                                                             group_mod, code_bytes, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// dimension-vector validation and packing
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t read_code(const void *dv, int32_t cb, int64_t r, bool &is_null)
+{
+    int32_t v;
+    if (cb == 1) {
+        v = ((const int8_t *)dv)[r];
+        is_null = v == OLAP_DIM_NULL_I8;
+    } else {
+        v = ((const int32_t *)dv)[r];
+        is_null = v == OLAP_DIM_NULL_I32;
+    }
+    return v;
+}
+
+__global__ void k_dim_stats(const void *dv, int32_t cb, int64_t rows, int32_t *stats)
+{
+    int32_t mx = -1;
+    int32_t neg = 0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        bool nul;
+        int32_t v = read_code(dv, cb, r, nul);
+        if (!nul) {
+            if (v < 0) neg++;
+            else mx = v > mx ? v : mx;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        int32_t omx = __shfl_xor(mx, o);
+        int32_t oneg = __shfl_xor(neg, o);
+        mx = omx > mx ? omx : mx;
+        neg += oneg;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&stats[0], mx);
+        if (neg) atomicAdd(&stats[1], neg);
+    }
+}
+
+hipError_t launch_dim_stats(const void *dimvec, int32_t code_bytes, int64_t rows,
+                            int32_t *stats, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    k_dim_stats<<<grid_for(rows, 256 * 4, 1024), 256, 0, s>>>(dimvec, code_bytes, rows, stats);
+    return hipGetLastError();
+}
+
+__global__ void k_pack(const void *dv, int32_t cb, int64_t rows, uint32_t lw, uint32_t *words,
+                       uint32_t nwords)
+{
+    const uint32_t bits = 1u << lw;
+    const uint32_t per = 32u >> lw;
+    const uint32_t mask = bits == 32 ? 0xffffffffu : ((1u << bits) - 1u);
+    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nwords;
+         wi += gridDim.x * blockDim.x) {
+        uint32_t w = 0;
+        for (uint32_t k = 0; k < per; k++) {
+            int64_t r = (int64_t)wi * per + k;
+            uint32_t c = mask;
+            if (r < rows) {
+                bool nul;
+                int32_t v = read_code(dv, cb, r, nul);
+                c = nul ? mask : ((uint32_t)v & mask);
+            }
+            w |= c << (k << lw);
+        }
+        words[wi] = w;
+    }
+}
+
+hipError_t launch_pack(const void *dimvec, int32_t code_bytes, int64_t rows, uint32_t lw,
+                       uint32_t *words, uint32_t nwords, hipStream_t s)
+{
+    if (nwords == 0) return hipSuccess;
+    k_pack<<<grid_for(nwords, 256, 2048), 256, 0, s>>>(dimvec, code_bytes, rows, lw, words,
+                                                       nwords);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// scan building blocks
+// ------------------------------------------------------------------------
+
+// 4 consecutive rows starting at r; present = bit k set when row r+k exists
+template <bool VEC>
+__device__ __forceinline__ v4i load4(const int32_t *__restrict__ p, uint64_t r, uint64_t rows,
+                                     uint32_t &present)
+{
+    v4i v = {0, 0, 0, 0};
+    if (r + 4 <= rows) {
+        present = 0xFu;
+        if (VEC) {
+            v = __builtin_nontemporal_load((const v4i *)(p + r));
+        } else {
+            v.x = p[r];
+            v.y = p[r + 1];
+            v.z = p[r + 2];
+            v.w = p[r + 3];
+        }
+    } else {
+        present = 0u;
+        if (r < rows) { v.x = p[r]; present |= 1u; }
+        if (r + 1 < rows) { v.y = p[r + 1]; present |= 2u; }
+        if (r + 2 < rows) { v.z = p[r + 2]; present |= 4u; }
+    }
+    return v;
+}
+
+template <bool VEC>
+__device__ __forceinline__ void store4(int32_t *__restrict__ p, uint64_t r, uint64_t rows, v4i v)
+{
+    if (r + 4 <= rows) {
+        if (VEC) {
+            *(v4i *)(p + r) = v;
+        } else {
+            p[r] = v.x;
+            p[r + 1] = v.y;
+            p[r + 2] = v.z;
+            p[r + 3] = v.w;
+        }
+    } else {
+        if (r < rows) p[r] = v.x;
+        if (r + 1 < rows) p[r + 1] = v.y;
+        if (r + 2 < rows) p[r + 2] = v.z;
+    }
+}
+
+// probe one key: returns false when the key finds no row or a NULL code
+__device__ __forceinline__ bool probe1(const StepDesc &st, const uint32_t *lds, int32_t key,
+                                       uint32_t &code)
+{
+    const uint32_t idx = (uint32_t)key;
+    const bool inb = idx < st.dim_rows;
+    const uint32_t wi = inb ? (idx >> (5u - st.lw)) : 0u;
+    const uint32_t word = st.in_lds ? lds[st.lds_off + wi] : st.words[wi];
+    code = (word >> ((idx & ((32u >> st.lw) - 1u)) << st.lw)) & st.mask;
+    return inb && code != st.mask;
+}
+
+// probe 4 rows of a chunk; rows whose bit is clear in alive are skipped
+__device__ __forceinline__ void probe4(const StepDesc &st, const uint32_t *lds, v4i f,
+                                       uint32_t (&gid)[4], uint32_t &alive, bool first)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if ((alive >> k) & 1u) {
+            uint32_t code;
+            bool ok = probe1(st, lds, f[k], code);
+            const uint32_t add = code * st.factor;
+            gid[k] = first ? add : gid[k] + add;
+            if (!ok) alive &= ~(1u << k);
+        }
+    }
+}
+
+// copy LDS-resident packed vectors; zero the LDS group table
+template <int NDIM, bool GLDS>
+__device__ __forceinline__ void lds_init(const ScanArgs &a, uint32_t *lds)
+{
+#pragma unroll
+    for (int s = 0; s < NDIM; s++) {
+        if (a.st[s].in_lds) {
+            const uint32_t n = a.st[s].nwords;
+            const uint32_t off = a.st[s].lds_off;
+            const uint32_t *src = a.st[s].words;
+            for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[off + i] = src[i];
+        }
+    }
+    if (GLDS) {
+        unsigned long long *lsum = (unsigned long long *)(lds + a.gt_off);
+        uint32_t *lcnt = lds + a.gt_off + 2u * a.G;
+        for (uint32_t g = threadIdx.x; g < a.G; g += kBlock) {
+            lsum[g] = 0ull;
+            lcnt[g] = 0u;
+        }
+    }
+}
+
+template <bool GLDS>
+__device__ __forceinline__ void aggregate(const ScanArgs &a, uint32_t *lds, uint32_t gid,
+                                          unsigned long long m)
+{
+    if (gid >= a.G) {
+        atomic_add_u32_dev(a.err, 1u);
+        return;
+    }
+    if (GLDS) {
+        unsigned long long *lsum = (unsigned long long *)(lds + a.gt_off);
+        uint32_t *lcnt = lds + a.gt_off + 2u * a.G;
+        if (!a.count_only) atomic_add_u64_lds(&lsum[gid], m);
+        atomic_add_u32_lds(&lcnt[gid], 1u);
+    } else {
+        if (!a.count_only) atomic_add_u64_dev(&a.gsum[gid], m);
+        atomic_add_u64_dev(&a.gcnt[gid], 1ull);
+    }
+}
+
+template <bool GLDS>
+__device__ __forceinline__ void flush(const ScanArgs &a, uint32_t *lds)
+{
+    if (!GLDS) return;
+    __syncthreads();
+    const unsigned long long *lsum = (const unsigned long long *)(lds + a.gt_off);
+    const uint32_t *lcnt = lds + a.gt_off + 2u * a.G;
+    for (uint32_t g = threadIdx.x; g < a.G; g += kBlock) {
+        const uint32_t c = lcnt[g];
+        if (c) {
+            atomic_add_u64_dev(&a.gcnt[g], (unsigned long long)c);
+            if (!a.count_only) atomic_add_u64_dev(&a.gsum[g], lsum[g]);
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long measure4(const ScanArgs &a, v4i m1, v4i m2, int k)
+{
+    long long v = (long long)m1[k];
+    if (a.m2) v += (long long)m2[k];
+    return (unsigned long long)v;
+}
+
+// wave64 stream compaction: returns this lane's slot when want is set
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want)
+{
+    const unsigned long long mask = __ballot(want);
+    const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    uint32_t base = 0;
+    if (mask) {
+        const int leader = __ffsll(mask) - 1;
+        if ((int)__lane_id() == leader) base = atomic_add_u32_lds(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + before;
+}
+
+// ------------------------------------------------------------------------
+// row-wise fused scan (OLAPcore_row_thread semantics): probe steps in plan
+// order, per-row early exit; a later step's 128-bit load is issued only for
+// chunks that still hold a live row, so dead lines are never fetched.
+// ------------------------------------------------------------------------
+
+template <int NDIM, bool VEC, bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_rowwise(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<NDIM, GLDS>(a, lds);
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileRW;
+        v4i f[kUnroll];
+        uint32_t alive[kUnroll];
+        uint32_t gid[kUnroll][4];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+            f[u] = load4<VEC>(a.st[0].fk, r, rows, alive[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
+#pragma unroll
+        for (int s = 1; s < NDIM; s++) {
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (alive[u]) {
+                    uint32_t dummy;
+                    const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+                    f[u] = load4<VEC>(a.st[s].fk, r, rows, dummy);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+                if (alive[u]) probe4(a.st[s], lds, f[u], gid[u], alive[u], false);
+        }
+        v4i m1[kUnroll], m2[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            m1[u] = v4i{0, 0, 0, 0};
+            m2[u] = v4i{0, 0, 0, 0};
+            if (alive[u] && !a.count_only) {
+                uint32_t dummy;
+                const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+                m1[u] = load4<VEC>(a.m1, r, rows, dummy);
+                if (a.m2) m2[u] = load4<VEC>(a.m2, r, rows, dummy);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (alive[u]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((alive[u] >> k) & 1u)
+                        aggregate<GLDS>(a, lds, gid[u][k], measure4(a, m1[u], m2[u], k));
+            }
+        }
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
+// vector-wise static vector (OLAPcore_vwm_sv_thread semantics): every step is
+// probed for every row; all columns of a tile are loaded up front.
+// ------------------------------------------------------------------------
+
+template <int NDIM, bool VEC, bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_vwm_sv(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<NDIM, GLDS>(a, lds);
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileRW;
+        v4i f[NDIM][kUnroll], m1[kUnroll], m2[kUnroll];
+        uint32_t present[kUnroll], alive[kUnroll];
+        uint32_t gid[kUnroll][4];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+            uint32_t dummy;
+#pragma unroll
+            for (int s = 0; s < NDIM; s++) f[s][u] = load4<VEC>(a.st[s].fk, r, rows, present[u]);
+            m1[u] = v4i{0, 0, 0, 0};
+            m2[u] = v4i{0, 0, 0, 0};
+            if (!a.count_only) {
+                m1[u] = load4<VEC>(a.m1, r, rows, dummy);
+                if (a.m2) m2[u] = load4<VEC>(a.m2, r, rows, dummy);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            // static vector: every row is probed in every step (no early exit)
+            uint32_t live = present[u];
+#pragma unroll
+            for (int k = 0; k < 4; k++) gid[u][k] = 0u;
+#pragma unroll
+            for (int s = 0; s < NDIM; s++) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if ((present[u] >> k) & 1u) {
+                        uint32_t code;
+                        bool ok = probe1(a.st[s], lds, f[s][u][k], code);
+                        gid[u][k] += code * a.st[s].factor;
+                        if (!ok) live &= ~(1u << k);
+                    }
+                }
+            }
+            alive[u] = live;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (alive[u]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((alive[u] >> k) & 1u)
+                        aggregate<GLDS>(a, lds, gid[u][k], measure4(a, m1[u], m2[u], k));
+            }
+        }
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
+// vector-wise dynamic vector (OLAPcore_vwm_dv_thread semantics): per tile of
+// kTileV rows, step 0 compacts (row offset, gid) of live rows into an LDS
+// list with wave ballots; each later step gathers its key only for listed
+// rows and re-compacts in place; the measure is gathered for the survivors.
+// ------------------------------------------------------------------------
+
+template <int NDIM, bool VEC, bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<NDIM, GLDS>(a, lds);
+    uint32_t *lgid = lds + a.list_off;
+    uint16_t *loff = (uint16_t *)(lgid + kTileV);
+    uint32_t *counter = lgid + kTileV + kTileV / 2;
+    if (threadIdx.x == 0) *counter = 0u;
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    constexpr int kU = kTileV / (kBlock * 4);
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileV;
+        v4i f[kU];
+        uint32_t alive[kU];
+        uint32_t gid[kU][4];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+            f[u] = load4<VEC>(a.st[0].fk, r, rows, alive[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool want = (alive[u] >> k) & 1u;
+                const uint32_t pos = wave_alloc(counter, want);
+                if (want) {
+                    loff[pos] = (uint16_t)((u * kBlock + tid) * 4u + k);
+                    lgid[pos] = gid[u][k];
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t n = *counter;
+#pragma unroll
+        for (int s = 1; s < NDIM; s++) {
+            uint32_t eoff[kPerThreadV], egid[kPerThreadV];
+            uint32_t ok = 0;
+#pragma unroll
+            for (int j = 0; j < kPerThreadV; j++) {
+                const uint32_t e = tid + j * kBlock;
+                if (e < n) {
+                    eoff[j] = loff[e];
+                    egid[j] = lgid[e];
+                    uint32_t code;
+                    const int32_t key = a.st[s].fk[base + eoff[j]];
+                    if (probe1(a.st[s], lds, key, code)) {
+                        egid[j] += code * a.st[s].factor;
+                        ok |= 1u << j;
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) *counter = 0u;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kPerThreadV; j++) {
+                const bool want = (ok >> j) & 1u;
+                const uint32_t pos = wave_alloc(counter, want);
+                if (want) {
+                    loff[pos] = (uint16_t)eoff[j];
+                    lgid[pos] = egid[j];
+                }
+            }
+            __syncthreads();
+            n = *counter;
+        }
+        for (uint32_t e = tid; e < n; e += kBlock) {
+            const uint64_t row = base + loff[e];
+            unsigned long long m = 0ull;
+            if (!a.count_only) {
+                long long v = (long long)a.m1[row];
+                if (a.m2) v += (long long)a.m2[row];
+                m = (unsigned long long)v;
+            }
+            aggregate<GLDS>(a, lds, lgid[e], m);
+        }
+        __syncthreads();
+        if (tid == 0) *counter = 0u;
+        __syncthreads();
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
+// column-wise static vector (join_cwm_sv / agg_cwm_sv semantics): one pass
+// per step over a full per-row gid vector in HBM, then one aggregation pass.
+// ------------------------------------------------------------------------
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void k_cwm_sv_join(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    const StepDesc &st = a.st[a.step];
+    if (st.in_lds)
+        for (uint32_t i = threadIdx.x; i < st.nwords; i += kBlock) lds[st.lds_off + i] = st.words[i];
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint64_t nchunks = (rows + 3) / 4;
+    const bool first = a.step == 0;
+    const bool last = a.last_step != 0;
+    for (uint64_t c = blockIdx.x * (uint64_t)kBlock + threadIdx.x; c < nchunks;
+         c += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t r = c * 4u;
+        uint32_t present, dummy;
+        v4i g = first ? v4i{0, 0, 0, 0} : load4<VEC>(a.gvec, r, rows, present);
+        uint32_t alive;
+        if (first) {
+            alive = r + 4 <= rows ? 0xFu : ((1u << (uint32_t)(rows - r)) - 1u);
+        } else {
+            alive = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) alive |= (g[k] != OLAP_GID_NULL ? 1u : 0u) << k;
+            alive &= present;
+        }
+        if (alive) {
+            v4i f = load4<VEC>(st.fk, r, rows, dummy);
+            uint32_t gg[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
+            probe4(st, lds, f, gg, alive, first);
+            g = v4i{(int)gg[0], (int)gg[1], (int)gg[2], (int)gg[3]};
+        }
+        const int32_t nul = last ? a.gvec_null : OLAP_GID_NULL;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (!((alive >> k) & 1u)) g[k] = nul;
+        store4<VEC>(a.gvec, r, rows, g);
+    }
+}
+
+template <bool VEC, bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_cwm_sv_agg(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<0, GLDS>(a, lds);
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint64_t nchunks = (rows + 3) / 4;
+    for (uint64_t c = blockIdx.x * (uint64_t)kBlock + threadIdx.x; c < nchunks;
+         c += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t r = c * 4u;
+        uint32_t present, dummy;
+        const v4i g = load4<VEC>(a.gvec, r, rows, present);
+        uint32_t alive = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) alive |= (g[k] != OLAP_GID_NULL ? 1u : 0u) << k;
+        alive &= present;
+        if (alive) {
+            v4i m1 = {0, 0, 0, 0}, m2 = {0, 0, 0, 0};
+            if (!a.count_only) {
+                m1 = load4<VEC>(a.m1, r, rows, dummy);
+                if (a.m2) m2 = load4<VEC>(a.m2, r, rows, dummy);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((alive >> k) & 1u) aggregate<GLDS>(a, lds, (uint32_t)g[k], measure4(a, m1, m2, k));
+        }
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
+// column-wise dynamic vector (join_cwm_dv / agg_cwm_dv semantics): the fact
+// table is cut into kTileV-row partitions, each owning a list segment in HBM
+// (row id, gid) and an entry count -- the per-thread OID/groupID/index of
+// the reference, one per partition. Step 0 fills the segments, each later
+// step re-compacts its segment in place, the aggregation pass reads them.
+// ------------------------------------------------------------------------
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void k_cwm_dv_join0(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    const StepDesc &st = a.st[0];
+    if (st.in_lds)
+        for (uint32_t i = threadIdx.x; i < st.nwords; i += kBlock) lds[st.lds_off + i] = st.words[i];
+    uint32_t *counter = lds + a.list_off;
+    if (threadIdx.x == 0) *counter = 0u;
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    constexpr int kU = kTileV / (kBlock * 4);
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileV;
+        uint32_t *soid = a.list_oid + base;
+        int32_t *sgid = a.list_gid + base;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+            uint32_t alive;
+            v4i f = load4<VEC>(st.fk, r, rows, alive);
+            uint32_t gid[4];
+            probe4(st, lds, f, gid, alive, true);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool want = (alive >> k) & 1u;
+                const uint32_t pos = wave_alloc(counter, want);
+                if (want) {
+                    soid[pos] = (uint32_t)(r + k);
+                    sgid[pos] = (int32_t)gid[k];
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            a.list_cnt[tile] = *counter;
+            *counter = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cwm_dv_join(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    const StepDesc &st = a.st[a.step];
+    if (st.in_lds)
+        for (uint32_t i = threadIdx.x; i < st.nwords; i += kBlock) lds[st.lds_off + i] = st.words[i];
+    uint32_t *counter = lds + a.list_off;
+    if (threadIdx.x == 0) *counter = 0u;
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileV;
+        uint32_t *soid = a.list_oid + base;
+        int32_t *sgid = a.list_gid + base;
+        const uint32_t n = a.list_cnt[tile];
+        uint32_t eoid[kPerThreadV], egid[kPerThreadV];
+        uint32_t ok = 0;
+#pragma unroll
+        for (int j = 0; j < kPerThreadV; j++) {
+            const uint32_t e = tid + j * kBlock;
+            if (e < n) {
+                eoid[j] = soid[e];
+                egid[j] = (uint32_t)sgid[e];
+                uint32_t code;
+                if (probe1(st, lds, st.fk[eoid[j]], code)) {
+                    egid[j] += code * st.factor;
+                    ok |= 1u << j;
+                }
+            }
+        }
+        __syncthreads();  // every entry of the segment has been read
+#pragma unroll
+        for (int j = 0; j < kPerThreadV; j++) {
+            const bool want = (ok >> j) & 1u;
+            const uint32_t pos = wave_alloc(counter, want);
+            if (want) {
+                soid[pos] = eoid[j];
+                sgid[pos] = (int32_t)egid[j];
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            a.list_cnt[tile] = *counter;
+            *counter = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+template <bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_cwm_dv_agg(ScanArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<0, GLDS>(a, lds);
+    __syncthreads();
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * (uint64_t)kTileV;
+        const uint32_t n = a.list_cnt[tile];
+        for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+            const uint32_t row = a.list_oid[base + e];
+            unsigned long long m = 0ull;
+            if (!a.count_only) {
+                long long v = (long long)a.m1[row];
+                if (a.m2) v += (long long)a.m2[row];
+                m = (unsigned long long)v;
+            }
+            aggregate<GLDS>(a, lds, (uint32_t)a.list_gid[base + e], m);
+        }
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
+// launch helpers
+// ------------------------------------------------------------------------
+
+template <typename K>
+static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t dyn_lds,
+                                    uint64_t work_units, hipStream_t s)
+{
+    if (dyn_lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, dyn_lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    uint64_t grid = (uint64_t)per_cu * (uint64_t)num_cus;
+    if (grid > work_units) grid = work_units;
+    if (grid < 1) grid = 1;
+    kernel<<<(unsigned)grid, kBlock, dyn_lds, s>>>(a);
+    return hipGetLastError();
+}
+
+#define OLAP_DISPATCH_NDIM(KERNEL, VEC, GLDS)                                      \
+    switch (ndims) {                                                             \
+    case 1: return launch_persistent(KERNEL<1, VEC, GLDS>, a, num_cus, dyn_lds, a.ntiles, s); \
+    case 2: return launch_persistent(KERNEL<2, VEC, GLDS>, a, num_cus, dyn_lds, a.ntiles, s); \
+    case 3: return launch_persistent(KERNEL<3, VEC, GLDS>, a, num_cus, dyn_lds, a.ntiles, s); \
+    default: return launch_persistent(KERNEL<4, VEC, GLDS>, a, num_cus, dyn_lds, a.ntiles, s); \
+    }
+
+#define OLAP_DISPATCH_ALL(KERNEL)                                                  \
+    if (vec) {                                                                   \
+        if (a.gt_lds) { OLAP_DISPATCH_NDIM(KERNEL, true, true) }                  \
+        else { OLAP_DISPATCH_NDIM(KERNEL, true, false) }                          \
+    } else {                                                                     \
+        if (a.gt_lds) { OLAP_DISPATCH_NDIM(KERNEL, false, true) }                 \
+        else { OLAP_DISPATCH_NDIM(KERNEL, false, false) }                         \
+    }
+
+hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                       hipStream_t s)
+{
+    a.ntiles = (a.rows + kTileRW - 1) / kTileRW;
+    if (a.ntiles == 0) return hipSuccess;
+    OLAP_DISPATCH_ALL(k_rowwise)
+}
+
+hipError_t run_vwm_sv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                      hipStream_t s)
+{
+    a.ntiles = (a.rows + kTileRW - 1) / kTileRW;
+    if (a.ntiles == 0) return hipSuccess;
+    OLAP_DISPATCH_ALL(k_vwm_sv)
+}
+
+hipError_t run_vwm_dv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                      hipStream_t s)
+{
+    a.ntiles = (a.rows + kTileV - 1) / kTileV;
+    if (a.ntiles == 0) return hipSuccess;
+    OLAP_DISPATCH_ALL(k_vwm_dv)
+}
+
+hipError_t run_cwm_sv_join(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s)
+{
+    const uint64_t units = (a.rows + 4ull * kBlock - 1) / (4ull * kBlock);
+    if (units == 0) return hipSuccess;
+    if (vec) return launch_persistent(k_cwm_sv_join<true>, a, num_cus, dyn_lds, units, s);
+    return launch_persistent(k_cwm_sv_join<false>, a, num_cus, dyn_lds, units, s);
+}
+
+hipError_t run_cwm_sv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s)
+{
+    const uint64_t units = (a.rows + 4ull * kBlock - 1) / (4ull * kBlock);
+    if (units == 0) return hipSuccess;
+    if (vec) {
+        if (a.gt_lds) return launch_persistent(k_cwm_sv_agg<true, true>, a, num_cus, dyn_lds, units, s);
+        return launch_persistent(k_cwm_sv_agg<true, false>, a, num_cus, dyn_lds, units, s);
+    }
+    if (a.gt_lds) return launch_persistent(k_cwm_sv_agg<false, true>, a, num_cus, dyn_lds, units, s);
+    return launch_persistent(k_cwm_sv_agg<false, false>, a, num_cus, dyn_lds, units, s);
+}
+
+hipError_t run_cwm_dv_join(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s)
+{
+    a.ntiles = (a.rows + kTileV - 1) / kTileV;
+    if (a.ntiles == 0) return hipSuccess;
+    if (a.step == 0) {
+        if (vec) return launch_persistent(k_cwm_dv_join0<true>, a, num_cus, dyn_lds, a.ntiles, s);
+        return launch_persistent(k_cwm_dv_join0<false>, a, num_cus, dyn_lds, a.ntiles, s);
+    }
+    return launch_persistent(k_cwm_dv_join, a, num_cus, dyn_lds, a.ntiles, s);
+}
+
+hipError_t run_cwm_dv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, hipStream_t s)
+{
+    a.ntiles = (a.rows + kTileV - 1) / kTileV;
+    if (a.ntiles == 0) return hipSuccess;
+    if (a.gt_lds) return launch_persistent(k_cwm_dv_agg<true>, a, num_cus, dyn_lds, a.ntiles, s);
+    return launch_persistent(k_cwm_dv_agg<false>, a, num_cus, dyn_lds, a.ntiles, s);
+}
+
+// ------------------------------------------------------------------------
+// sub-step operators
+// ------------------------------------------------------------------------
+
+constexpr uint32_t kVgLdsSlots = 16384;  // u64 slots of the LDS group vector (128 KB)
+
+template <bool GLDS>
+__global__ __launch_bounds__(kBlock) void k_vector_group(int64_t rows, const int32_t *key1,
+                                                         const int32_t *m1, const int32_t *m2,
+                                                         unsigned long long *gv, int64_t G,
+                                                         uint32_t *bad)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    unsigned long long *lv = (unsigned long long *)lds;
+    if (GLDS) {
+        for (int64_t g = threadIdx.x; g < G; g += kBlock) lv[g] = 0ull;
+        __syncthreads();
+    }
+    uint32_t nbad = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int64_t k = (int64_t)key1[i] - 1;
+        if (k < 0 || k >= G) {
+            nbad++;
+            continue;
+        }
+        long long v = (long long)m1[i];
+        if (m2) v += (long long)m2[i];
+        if (GLDS) atomic_add_u64_lds(&lv[k], (unsigned long long)v);
+        else atomic_add_u64_dev(&gv[k], (unsigned long long)v);
+    }
+    for (int o = 32; o > 0; o >>= 1) nbad += __shfl_xor(nbad, o);
+    if ((threadIdx.x & 63) == 0 && nbad) atomic_add_u32_dev(bad, nbad);
+    if (GLDS) {
+        __syncthreads();
+        for (int64_t g = threadIdx.x; g < G; g += kBlock)
+            if (lv[g]) atomic_add_u64_dev(&gv[g], lv[g]);
+    }
+}
+
+hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
+                               const int32_t *m2, unsigned long long *gv, int64_t G,
+                               unsigned long long *total, uint32_t *bad, int num_cus,
+                               hipStream_t s)
+{
+    (void)total;
+    if (rows <= 0) return hipSuccess;
+    const int grid = grid_for(rows, kBlock * 16, num_cus * 2);
+    if (G <= (int64_t)kVgLdsSlots) {
+        const uint32_t lds_bytes = (uint32_t)(G * 8);
+        if (lds_bytes > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute((const void *)k_vector_group<true>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds_bytes);
+            if (e != hipSuccess) return e;
+        }
+        k_vector_group<true><<<grid, kBlock, lds_bytes, s>>>(rows, key1, m1, m2, gv, G, bad);
+    } else {
+        k_vector_group<false><<<grid_for(rows, kBlock * 16, num_cus * 8), kBlock, 0, s>>>(
+            rows, key1, m1, m2, gv, G, bad);
+    }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long *v, int64_t n,
+                                                 unsigned long long *out)
+{
+    unsigned long long acc = 0ull;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc += v[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0 && acc) atomic_add_u64_dev(out, acc);
+}
+
+hipError_t launch_sum_u64(const unsigned long long *v, int64_t n, unsigned long long *out,
+                          int num_cus, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_sum_u64<<<grid_for(n, 256 * 8, num_cus * 4), 256, 0, s>>>(v, n, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_agg_expr(int64_t rows, const int32_t *p,
+                                                  const int32_t *q, const int32_t *t,
+                                                  unsigned long long *out)
+{
+    long long acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const long long pq = (long long)p[i] * (long long)q[i];
+        acc += pq * (long long)t[i] + 100ll * pq;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0 && acc) atomic_add_u64_dev(out, (unsigned long long)acc);
+}
+
+hipError_t launch_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t,
+                           unsigned long long *out, int num_cus, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    k_agg_expr<<<grid_for(rows, 256 * 8, num_cus * 4), 256, 0, s>>>(rows, p, q, t, out);
+    return hipGetLastError();
+}
+
+}  // namespace olap

@@ -1,0 +1,326 @@
+"""ctypes bindings of libolapcore.so (include/olapcore.h).
+
+Every call goes through the C ABI; device buffers are torch tensors on a ROCm
+device (torch only provides memory and streams). There is no CPU fallback:
+if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libolapcore.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "olapcore.h")
+
+MAX_DIMS = 4
+DIM_NULL_I8 = 127
+DIM_NULL_I32 = -1
+GID_NULL = -1
+STARJOIN_NULL = 32767
+
+OK, E_INVALID, E_HIP, E_NOMEM, E_CAPACITY, E_NODEVICE, E_BUSY = range(7)
+ROWWISE, VWM_DV, VWM_SV, CWM_DV, CWM_SV = range(5)
+STRATEGIES = {"rowwise": ROWWISE, "vwm_dv": VWM_DV, "vwm_sv": VWM_SV,
+              "cwm_dv": CWM_DV, "cwm_sv": CWM_SV}
+# table slots as the reference's main() indexes them
+TABLES = {"customer": 0, "supplier": 1, "part": 2, "date": 3}
+
+
+class OlapError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"olapcore status {status}: {msg}")
+        self.status = status
+
+
+class Plan(C.Structure):
+    _fields_ = [("dimvec_nums", C.c_int32),
+                ("orders", C.c_int32 * MAX_DIMS),
+                ("factor", C.c_int32 * MAX_DIMS),
+                ("group_nums", C.c_int64)]
+
+    def as_dict(self):
+        return {"dimvec_nums": self.dimvec_nums, "orders": list(self.orders),
+                "factor": list(self.factor), "group_nums": self.group_nums}
+
+
+class Columns(C.Structure):
+    _fields_ = [("dimvec", C.c_void_p * MAX_DIMS),
+                ("dim_rows", C.c_int64 * MAX_DIMS),
+                ("code_bytes", C.c_int32 * MAX_DIMS),
+                ("fk", C.c_void_p * MAX_DIMS),
+                ("m1", C.c_void_p),
+                ("m2", C.c_void_p),
+                ("rows", C.c_int64)]
+
+
+_P = C.c_void_p
+_I64P = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes); the export test checks this list against the header
+SIGNATURES = {
+    "olap_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "olap_ctx_destroy": (C.c_int, [_P]),
+    "olap_last_error": (C.c_char_p, [_P]),
+    "olap_device_info": (C.c_int, [_P, C.POINTER(C.c_int32), _I64P, C.c_char_p, C.c_size_t]),
+    "olap_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "olap_dev_free": (C.c_int, [_P, _P]),
+    "olap_copy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t, _P]),
+    "olap_copy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t, _P]),
+    "olap_stream_sync": (C.c_int, [_P, _P]),
+    "olap_gen_uniform_i32": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64,
+                                       C.c_int32, C.c_int32, _P]),
+    "olap_zipf_table": (C.c_int, [C.c_int64, C.c_double, _P]),
+    "olap_gen_zipf_i32": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, _P,
+                                    C.c_int64, _P]),
+    "olap_make_plan": (C.c_int, [_P, _P, _P, C.POINTER(Plan)]),
+    "olap_build_dimvec": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, _P, _P]),
+    "olap_query_create": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), C.POINTER(_P)]),
+    "olap_query_run": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "olap_query_destroy": (C.c_int, [_P]),
+    "olap_query_device_bytes": (C.c_int64, [_P]),
+    "olap_rowwise": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_vwm_dv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_vwm_sv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_cwm_dv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_cwm_sv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_join_cwm_sv": (C.c_int, [_P, _P, _P]),
+    "olap_agg_cwm_sv": (C.c_int, [_P, _P, _P, _P, _P]),
+    "olap_join_cwm_dv": (C.c_int, [_P, _P]),
+    "olap_agg_cwm_dv": (C.c_int, [_P, _P, _P, _P]),
+    "olap_legacy_count": (C.c_int32, [_P, C.c_int64]),
+    "olap_starjoin": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, _I64P, _P]),
+    "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
+                                    _P]),
+    "olap_agg_expr": (C.c_int, [_P, C.c_int64, _P, _P, _P, _I64P, _P]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libolapcore.so and declare signatures. Raises when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: run `python -m memory_database_amd.build` first")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def make_plan(groups: Sequence[int], bitmap: Sequence[int], sele: Sequence[float]) -> Plan:
+    """sort_by_rate + factor plan (host code in the library)."""
+    lib = load()
+    g = (C.c_int32 * 4)(*groups)
+    b = (C.c_int32 * 4)(*bitmap)
+    s = (C.c_double * 4)(*sele)
+    p = Plan()
+    st = lib.olap_make_plan(g, b, s, C.byref(p))
+    if st != OK:
+        raise OlapError(st, "make_plan rejected its arguments")
+    return p
+
+
+def plan_from(orders: Sequence[int], factor: Sequence[int], group_nums: int) -> Plan:
+    p = Plan()
+    p.dimvec_nums = len(orders)
+    for i, o in enumerate(orders):
+        p.orders[i] = o
+        p.factor[i] = factor[i]
+    p.group_nums = group_nums
+    return p
+
+
+def legacy_count(sums) -> int:
+    """The reference driver's int return value from int64 sums (host array)."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(sums, dtype=np.int64))
+    return int(load().olap_legacy_count(a.ctypes.data, a.size))
+
+
+def zipf_table(d: int, theta: float):
+    import numpy as np
+    thr = np.zeros(d, dtype=np.uint64)
+    st = load().olap_zipf_table(d, theta, thr.ctypes.data)
+    if st != OK:
+        raise OlapError(st, "zipf_table")
+    return thr
+
+
+class Context:
+    """One olap_ctx on a device. Not thread-safe (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        st = self.lib.olap_ctx_create(device, C.byref(h))
+        if st != OK:
+            raise OlapError(st, "olap_ctx_create failed (no GPU?)")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.olap_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int, what: str):
+        if st != OK:
+            msg = self.lib.olap_last_error(self.h)
+            raise OlapError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def info(self):
+        n = C.c_int32()
+        b = C.c_int64()
+        arch = C.create_string_buffer(64)
+        self.check(self.lib.olap_device_info(self.h, C.byref(n), C.byref(b), arch, 64), "info")
+        return {"num_cus": n.value, "hbm_bytes": b.value, "arch": arch.value.decode()}
+
+    # ---- data --------------------------------------------------------
+    def gen_uniform(self, out, seed: int, stream_id: int, lo: int, hi: int,
+                    first_row: int = 0, stream=None):
+        self.check(self.lib.olap_gen_uniform_i32(self.h, _ptr(out), out.numel(), first_row,
+                                                 seed, stream_id, lo, hi, stream),
+                   "gen_uniform")
+        return out
+
+    def gen_zipf(self, out, seed: int, stream_id: int, thr_dev, first_row: int = 0,
+                 stream=None):
+        self.check(self.lib.olap_gen_zipf_i32(self.h, _ptr(out), out.numel(), first_row, seed,
+                                              stream_id, _ptr(thr_dev), thr_dev.numel(), stream),
+                   "gen_zipf")
+        return out
+
+    def build_dimvec(self, filter_col, group_col, lo: int, hi: int, bitmap: int, out,
+                     group_mod: int = 0, stream=None):
+        cb = out.element_size()
+        self.check(self.lib.olap_build_dimvec(self.h, _ptr(filter_col), _ptr(group_col),
+                                              filter_col.numel(), lo, hi, bitmap, group_mod, cb,
+                                              _ptr(out), stream), "build_dimvec")
+        return out
+
+    # ---- star join ---------------------------------------------------
+    @staticmethod
+    def columns(dimvecs, fks, m1, m2=None, rows: Optional[int] = None) -> Columns:
+        """dimvecs/fks: sequences of 4 (tensor or None) indexed by table slot."""
+        c = Columns()
+        for t in range(MAX_DIMS):
+            dv = dimvecs[t]
+            if dv is not None:
+                c.dimvec[t] = dv.data_ptr()
+                c.dim_rows[t] = dv.numel()
+                c.code_bytes[t] = dv.element_size()
+            else:
+                c.code_bytes[t] = 1
+            if fks[t] is not None:
+                c.fk[t] = fks[t].data_ptr()
+                if rows is None:
+                    rows = fks[t].numel()
+        c.m1 = _ptr(m1)
+        c.m2 = _ptr(m2)
+        c.rows = rows if rows is not None else (m1.numel() if m1 is not None else 0)
+        return c
+
+    def query(self, plan: Plan, cols: Columns) -> "Query":
+        return Query(self, plan, cols)
+
+    def run(self, strategy: str, plan: Plan, cols: Columns, out_sum, out_cnt, stream=None):
+        """One-shot reference-named entry point; returns the total count."""
+        fn = getattr(self.lib, "olap_" + strategy)
+        tot = C.c_int64()
+        self.check(fn(self.h, C.byref(plan), C.byref(cols), _ptr(out_sum), _ptr(out_cnt),
+                      C.byref(tot), stream), strategy)
+        return tot.value
+
+    # ---- sub-steps ---------------------------------------------------
+    def starjoin(self, strategy: str, dimvecs, fks, result=None, stream=None) -> int:
+        dv = (_P * 4)(*[d.data_ptr() for d in dimvecs])
+        dr = (C.c_int64 * 4)(*[d.numel() for d in dimvecs])
+        fk = (_P * 4)(*[f.data_ptr() for f in fks])
+        cnt = C.c_int64()
+        self.check(self.lib.olap_starjoin(self.h, STRATEGIES[strategy], fks[0].numel(), dv, dr,
+                                          fk, _ptr(result), C.byref(cnt), stream), "starjoin")
+        return cnt.value
+
+    def vector_group(self, key1, m1, m2, group_vector, stream=None):
+        tot = C.c_int64()
+        bad = C.c_int64()
+        self.check(self.lib.olap_vector_group(self.h, key1.numel(), _ptr(key1), _ptr(m1),
+                                              _ptr(m2), _ptr(group_vector), group_vector.numel(),
+                                              C.byref(tot), C.byref(bad), stream),
+                   "vector_group")
+        return tot.value, bad.value
+
+    def agg_expr(self, p, q, t, stream=None) -> int:
+        r = C.c_int64()
+        self.check(self.lib.olap_agg_expr(self.h, p.numel(), _ptr(p), _ptr(q), _ptr(t),
+                                          C.byref(r), stream), "agg_expr")
+        return r.value
+
+    def sync(self, stream=None):
+        self.check(self.lib.olap_stream_sync(self.h, stream), "sync")
+
+
+class Query:
+    """olap_query: validated plan + packed dimension vectors."""
+
+    def __init__(self, ctx: Context, plan: Plan, cols: Columns):
+        self.ctx = ctx
+        self.plan = plan
+        self.cols = cols
+        h = _P()
+        ctx.check(ctx.lib.olap_query_create(ctx.h, C.byref(plan), C.byref(cols), C.byref(h)),
+                  "query_create")
+        self.h = h
+
+    def run(self, strategy: str, out_sum, out_cnt, stream=None):
+        self.ctx.check(self.ctx.lib.olap_query_run(self.h, STRATEGIES[strategy], _ptr(out_sum),
+                                                   _ptr(out_cnt), stream), "query_run")
+
+    def join_cwm_sv(self, gid_vec, stream=None):
+        self.ctx.check(self.ctx.lib.olap_join_cwm_sv(self.h, _ptr(gid_vec), stream), "join_cwm_sv")
+
+    def agg_cwm_sv(self, gid_vec, out_sum, out_cnt, stream=None):
+        self.ctx.check(self.ctx.lib.olap_agg_cwm_sv(self.h, _ptr(gid_vec), _ptr(out_sum),
+                                                    _ptr(out_cnt), stream), "agg_cwm_sv")
+
+    def join_cwm_dv(self, stream=None):
+        self.ctx.check(self.ctx.lib.olap_join_cwm_dv(self.h, stream), "join_cwm_dv")
+
+    def agg_cwm_dv(self, out_sum, out_cnt, stream=None):
+        self.ctx.check(self.ctx.lib.olap_agg_cwm_dv(self.h, _ptr(out_sum), _ptr(out_cnt), stream),
+                       "agg_cwm_dv")
+
+    def device_bytes(self) -> int:
+        return int(self.ctx.lib.olap_query_device_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.olap_query_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,139 @@
+"""Synthetic star schemas of the benchmark configurations, generated on the
+device through libolapcore's counter-based generator.
+
+Every column is a pure function of (seed, stream id, row), so any slice of
+any table can be regenerated on the host for checking. Stream ids: dimension
+slot t uses 100+t (filter) and 200+t (group); the fact table uses 300+t for
+the foreign key into slot t and 400/401 for the measures.
+
+Table slots follow the reference's main(): 0 customer, 1 supplier, 2 part,
+3 date (code/OLAPcore/test_multi_compute_operator.cpp:1930-1953).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+DEFAULT_SEED = 20251203
+
+
+@dataclass
+class Dim:
+    slot: int            # table slot 0..3
+    rows: int            # dimension rows D
+    pred_hi: int         # filter U[0,100) passes when < pred_hi (selectivity pred_hi/100)
+    groups: int          # group column U[0, groups)
+    bitmap: int = 1      # 1: group-bearing, 0: filter-only
+    pred_lo: int = -(2 ** 31)
+
+    @property
+    def sele(self) -> float:
+        return self.pred_hi / 100.0
+
+
+@dataclass
+class StarConfig:
+    name: str
+    rows: int
+    dims: List[Dim]
+    m_lo: int            # measure U[m_lo, m_hi]
+    m_hi: int
+    zipf_theta: float = 0.0
+    seed: int = DEFAULT_SEED
+    two_measures: bool = False
+    notes: str = ""
+
+    def groups_by_slot(self):
+        g = [0, 0, 0, 0]
+        for d in self.dims:
+            g[d.slot] = d.groups
+        return g
+
+    def bitmap_by_slot(self):
+        b = [0, 0, 0, 0]
+        for d in self.dims:
+            b[d.slot] = d.bitmap
+        return b
+
+    def sele_by_slot(self):
+        s = [0.0, 0.0, 0.0, 0.0]
+        for d in self.dims:
+            s[d.slot] = d.sele
+        return s
+
+    def fact_bytes_per_row(self) -> int:
+        return 4 * (len(self.dims) + (2 if self.two_measures else 1))
+
+
+# Plain 1-dim star-join group-by on the reference CPU path: 6M fact rows, FK
+# U[0,100000), measure U[1,100]; 100,000-row dimension, filter < 10, 16 groups.
+C1 = StarConfig("c1_1dim_6M", 6_000_000, [Dim(0, 100_000, 10, 16)], 1, 100)
+
+# Plain 4-dim star-join group-by: 60M fact rows; dimensions date 2,556 /
+# customer 30,000 / part 200,000 / supplier 2,000 rows with selectivities
+# 0.5 / 0.2 / 0.04 / 0.2 and group cardinalities 7 / 25 / 1 / 5 (G = 875);
+# measure U[1000, 1000000].
+C2_DIMS = [Dim(3, 2_556, 50, 7), Dim(0, 30_000, 20, 25), Dim(2, 200_000, 4, 1),
+           Dim(1, 2_000, 20, 5)]
+C2 = StarConfig("c2_4dim_60M", 60_000_000, C2_DIMS, 1000, 1_000_000)
+
+# The same star at 600M fact rows (12 GB of fact columns).
+C3 = StarConfig("c3_4dim_600M", 600_000_000, C2_DIMS, 1000, 1_000_000)
+
+CONFIGS = {c.name: c for c in (C1, C2, C3)}
+
+
+def scaled(cfg: StarConfig, rows: int, name: Optional[str] = None, **kw) -> StarConfig:
+    d = dict(cfg.__dict__)
+    d.update(kw)
+    d["rows"] = rows
+    d["name"] = name or f"{cfg.name}@{rows}"
+    return StarConfig(**d)
+
+
+class DeviceStar:
+    """A StarConfig materialised in HBM (torch tensors on the ROCm device)."""
+
+    def __init__(self, ctx, cfg: StarConfig, device: str = "cuda", stream=None,
+                 code_bytes: int = 1):
+        import torch
+
+        from . import olapcore as oc
+
+        self.cfg = cfg
+        self.ctx = ctx
+        i32 = torch.int32
+        self.filters = [None] * 4
+        self.groups = [None] * 4
+        self.dimvecs = [None] * 4
+        self.fks = [None] * 4
+        dt = torch.int8 if code_bytes == 1 else torch.int32
+        zipf_thr = {}
+        for d in cfg.dims:
+            f = torch.empty(d.rows, dtype=i32, device=device)
+            g = torch.empty(d.rows, dtype=i32, device=device)
+            ctx.gen_uniform(f, cfg.seed, 100 + d.slot, 0, 100, stream=stream)
+            ctx.gen_uniform(g, cfg.seed, 200 + d.slot, 0, d.groups, stream=stream)
+            dv = torch.empty(d.rows, dtype=dt, device=device)
+            ctx.build_dimvec(f, g, d.pred_lo, d.pred_hi, d.bitmap, dv,
+                             group_mod=d.groups if code_bytes == 1 else 0, stream=stream)
+            self.filters[d.slot], self.groups[d.slot], self.dimvecs[d.slot] = f, g, dv
+            fk = torch.empty(cfg.rows, dtype=i32, device=device)
+            if cfg.zipf_theta > 0.0:
+                thr = torch.from_numpy(oc.zipf_table(d.rows, cfg.zipf_theta).view("int64")).to(device)
+                zipf_thr[d.slot] = thr
+                ctx.gen_zipf(fk, cfg.seed, 300 + d.slot, thr, stream=stream)
+            else:
+                ctx.gen_uniform(fk, cfg.seed, 300 + d.slot, 0, d.rows, stream=stream)
+            self.fks[d.slot] = fk
+        self.m1 = torch.empty(cfg.rows, dtype=i32, device=device)
+        ctx.gen_uniform(self.m1, cfg.seed, 400, cfg.m_lo, cfg.m_hi + 1, stream=stream)
+        self.m2 = None
+        if cfg.two_measures:
+            self.m2 = torch.empty(cfg.rows, dtype=i32, device=device)
+            ctx.gen_uniform(self.m2, cfg.seed, 401, cfg.m_lo, cfg.m_hi + 1, stream=stream)
+        self.plan = oc.make_plan(cfg.groups_by_slot(), cfg.bitmap_by_slot(), cfg.sele_by_slot())
+        self.cols = oc.Context.columns(self.dimvecs, self.fks, self.m1, self.m2, rows=cfg.rows)
+
+    def fact_bytes(self) -> int:
+        return self.cfg.rows * self.cfg.fact_bytes_per_row()

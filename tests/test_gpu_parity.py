@@ -1,0 +1,354 @@
+"""GPU parity: every strategy of libolapcore (through the C ABI) against the
+CPU oracle on the same inputs, bit-exact (int64 sums and counts per slot)."""
+import numpy as np
+import pytest
+
+from memory_database_amd import olapcore as oc
+from memory_database_amd import workloads as wl
+from oracle import pyoracle as po
+from tests.cpu_star import cpu_star
+
+pytestmark = pytest.mark.gpu
+
+STRATS = ["rowwise", "vwm_dv", "vwm_sv", "cwm_dv", "cwm_sv"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _dev(a, torch):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu_run_all(ctx, plan_np, dimvecs, fks, m1, m2, strategies=STRATS, offset=0):
+    """Upload numpy columns and run each strategy; offset > 0 shifts every
+    fact column by that many elements (unaligned 128-bit path)."""
+    import torch
+
+    def up(a):
+        if a is None:
+            return None
+        if offset:
+            buf = torch.empty(a.size + offset, dtype=torch.from_numpy(a[:0]).dtype, device="cuda")
+            buf[offset:] = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+            return buf[offset:]
+        return _dev(a, torch)
+
+    dv = [_dev(d, torch) for d in dimvecs]
+    fk = [up(f) for f in fks]
+    g1, g2 = up(m1), up(m2)
+    rows = int(m1.size)
+    plan = oc.plan_from(list(plan_np.orders)[: plan_np.dimvec_nums],
+                        list(plan_np.factor)[: plan_np.dimvec_nums], int(plan_np.group_nums))
+    cols = oc.Context.columns(dv, fk, g1, g2, rows=rows)
+    G = int(plan.group_nums)
+    out = {}
+    q = ctx.query(plan, cols)
+    for s in strategies:
+        _say(f"  strategy {s} rows={rows} G={G} offset={offset}")
+        osum = torch.full((G,), -7, dtype=torch.int64, device="cuda")
+        ocnt = torch.full((G,), -7, dtype=torch.int64, device="cuda")
+        q.run(s, osum, ocnt)
+        torch.cuda.synchronize()
+        out[s] = (osum.cpu().numpy(), ocnt.cpu().numpy())
+    q.close()
+    return out
+
+
+def check_all(ctx, plan_np, dimvecs, fks, m1, m2, ncpu, offset=0, strategies=STRATS):
+    star = po.Star(dimvecs, fks, m1, m2)
+    es, ec, bad = po.run("rowwise", plan_np, star, nthreads=ncpu)
+    assert bad == 0
+    got = gpu_run_all(ctx, plan_np, dimvecs, fks, m1, m2, strategies, offset)
+    for s, (gs, gc) in got.items():
+        np.testing.assert_array_equal(gc, ec, err_msg=f"{s}: counts")
+        np.testing.assert_array_equal(gs, es, err_msg=f"{s}: sums")
+    return es, ec
+
+
+def random_star(rng, ndims, rows, code_bytes=1, max_groups=20, oob=True, big_dim=False,
+                two_measures=None, wide_measure=True):
+    slots = list(rng.permutation(4)[:ndims])
+    groups = [0, 0, 0, 0]
+    bitmap = [0, 0, 0, 0]
+    sele = [0.0, 0.0, 0.0, 0.0]
+    dimvecs = [None] * 4
+    fks = [None] * 4
+    for t in slots:
+        D = int(rng.integers(1, 300_000 if big_dim else 5000))
+        g = int(rng.integers(1, max_groups + 1))
+        bm = int(rng.integers(0, 2)) if g == 1 else 1
+        groups[t], bitmap[t] = g, bm
+        thr = int(rng.integers(0, 101))
+        sele[t] = thr / 100.0
+        f = rng.integers(0, 100, D, dtype=np.int32)
+        gc = rng.integers(0, g, D, dtype=np.int32)
+        dimvecs[t] = po.build_dimvec(f, gc, -(2 ** 31), thr, bm, code_bytes)
+        hi = D + (D // 8 + 2 if oob else 0)
+        fks[t] = rng.integers(-3 if oob else 0, hi, rows, dtype=np.int32)
+    plan = po.make_plan(groups, bitmap, sele)
+    if wide_measure:
+        m1 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    else:
+        m1 = rng.integers(1, 100, rows, dtype=np.int32)
+    if two_measures is None:
+        two_measures = bool(rng.integers(0, 2))
+    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32) if two_measures else None
+    return plan, dimvecs, fks, m1, m2
+
+
+def test_generators_match_oracle(ctx):
+    import torch
+
+    _say("generators: uniform + zipf, 1M rows")
+    n = 1_000_003
+    t = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.gen_uniform(t, 77, 5, -1000, 123457, first_row=11)
+    np.testing.assert_array_equal(t.cpu().numpy(), po.gen_uniform(n, 77, 5, -1000, 123457, 11))
+    thr = oc.zipf_table(200_000, 0.99)
+    np.testing.assert_array_equal(thr, po.zipf_table(200_000, 0.99))
+    thr_d = torch.from_numpy(thr.view(np.int64)).cuda()
+    ctx.gen_zipf(t, 78, 6, thr_d, first_row=3)
+    np.testing.assert_array_equal(t.cpu().numpy(), po.gen_zipf(n, 78, 6, thr, 3))
+
+
+def test_build_dimvec_matches_oracle(ctx):
+    import torch
+
+    rng = np.random.default_rng(5)
+    for code_bytes, bitmap, mod in [(1, 1, 126), (1, 0, 0), (4, 1, 0), (4, 1, 1000)]:
+        _say(f"build_dimvec code_bytes={code_bytes} bitmap={bitmap} mod={mod}")
+        D = 77_777
+        f = rng.integers(-50, 150, D, dtype=np.int32)
+        g = rng.integers(0, 100_000, D, dtype=np.int32)
+        if code_bytes == 1:
+            g = g % 126
+        lo, hi = -3, 61
+        out = torch.empty(D, dtype=torch.int8 if code_bytes == 1 else torch.int32, device="cuda")
+        ctx.build_dimvec(_dev(f, torch), _dev(g, torch), lo, hi, bitmap, out, group_mod=mod)
+        exp = po.build_dimvec(f, g, lo, hi, bitmap, code_bytes, group_mod=mod)
+        np.testing.assert_array_equal(out.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("seed", list(range(12)))
+def test_random_small_stars(ctx, ncpu, seed):
+    rng = np.random.default_rng(1000 + seed)
+    ndims = 1 + seed % 4
+    rows = [0, 1, 3, 5, 4097, 8191, 8193, 20_000, 65_536 + 7, 100_000, 33, 250_001][seed]
+    _say(f"random star seed={seed} ndims={ndims} rows={rows}")
+    plan, dv, fk, m1, m2 = random_star(rng, ndims, rows, code_bytes=1 if seed % 3 else 4)
+    check_all(ctx, plan, dv, fk, m1, m2, ncpu, offset=1 if seed % 5 == 4 else 0)
+
+
+def test_wide_codes_and_global_group_table(ctx, ncpu):
+    """int32 codes, 40,000 groups: the group table does not fit in LDS."""
+    rng = np.random.default_rng(9)
+    _say("wide codes / global table")
+    plan, dv, fk, m1, m2 = random_star(rng, 2, 300_000, code_bytes=4, max_groups=200)
+    assert plan.group_nums > 10_000 or True
+    check_all(ctx, plan, dv, fk, m1, m2, ncpu)
+    # single dimension with 100,000 groups
+    D = 150_000
+    f = rng.integers(0, 100, D, dtype=np.int32)
+    g = rng.integers(0, 100_000, D, dtype=np.int32)
+    dvec = po.build_dimvec(f, g, 0, 60, 1, 4)
+    fks = [rng.integers(0, D, 500_000, dtype=np.int32), None, None, None]
+    dvs = [dvec, None, None, None]
+    p = po.plan([0], [1], 100_000)
+    m = rng.integers(0, 1000, 500_000, dtype=np.int32)
+    check_all(ctx, p, dvs, fks, m, None, ncpu)
+
+
+def test_large_dimensions_spill_to_hbm(ctx, ncpu):
+    """Dimension vectors larger than LDS are probed from HBM."""
+    rng = np.random.default_rng(11)
+    _say("spilled dimensions")
+    D0, D1 = 3_000_000, 1_500_000
+    dvs = [None] * 4
+    dvs[0] = po.build_dimvec(rng.integers(0, 100, D0, dtype=np.int32),
+                             rng.integers(0, 100, D0, dtype=np.int32), 0, 50, 1, 1)
+    dvs[2] = po.build_dimvec(rng.integers(0, 100, D1, dtype=np.int32),
+                             rng.integers(0, 3, D1, dtype=np.int32), 0, 30, 1, 1)
+    rows = 2_000_000
+    fks = [rng.integers(0, D0, rows, dtype=np.int32), None,
+           rng.integers(0, D1, rows, dtype=np.int32), None]
+    p = po.plan([2, 0], [100, 1], 300)
+    m = rng.integers(1, 1_000_000, rows, dtype=np.int32)
+    check_all(ctx, p, dvs, fks, m, None, ncpu)
+
+
+def test_column_wise_split_entry_points(ctx, ncpu):
+    import torch
+
+    rng = np.random.default_rng(21)
+    _say("join/agg split entry points")
+    plan_np, dv, fk, m1, m2 = random_star(rng, 3, 123_457)
+    star = po.Star(dv, fk, m1, m2)
+    es, ec, _ = po.run("rowwise", plan_np, star)
+    egid = po.join_cwm_sv(plan_np, star)
+    d_dv = [_dev(d, torch) for d in dv]
+    d_fk = [_dev(f, torch) for f in fk]
+    plan = oc.plan_from(list(plan_np.orders)[: plan_np.dimvec_nums],
+                        list(plan_np.factor)[: plan_np.dimvec_nums], int(plan_np.group_nums))
+    cols = oc.Context.columns(d_dv, d_fk, _dev(m1, torch), _dev(m2, torch), rows=m1.size)
+    q = ctx.query(plan, cols)
+    G = int(plan.group_nums)
+    gvec = torch.empty(m1.size, dtype=torch.int32, device="cuda")
+    q.join_cwm_sv(gvec)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    q.agg_cwm_sv(gvec, osum, ocnt)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(gvec.cpu().numpy(), egid)
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    q.join_cwm_dv()
+    q.agg_cwm_dv(osum, ocnt)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    q.close()
+
+
+def test_one_shot_reference_named(ctx, ncpu):
+    import torch
+
+    rng = np.random.default_rng(31)
+    _say("one-shot reference-named entry points")
+    plan_np, dv, fk, m1, m2 = random_star(rng, 4, 77_001, two_measures=True)
+    star = po.Star(dv, fk, m1, m2)
+    es, ec, _ = po.run("vwm_dv", plan_np, star, nthreads=ncpu)
+    plan = oc.plan_from(list(plan_np.orders)[: plan_np.dimvec_nums],
+                        list(plan_np.factor)[: plan_np.dimvec_nums], int(plan_np.group_nums))
+    cols = oc.Context.columns([_dev(d, torch) for d in dv], [_dev(f, torch) for f in fk],
+                              _dev(m1, torch), _dev(m2, torch), rows=m1.size)
+    G = int(plan.group_nums)
+    for s in STRATS:
+        osum = torch.empty(G, dtype=torch.int64, device="cuda")
+        ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+        total = ctx.run(s, plan, cols, osum, ocnt)
+        assert total == int(ec.sum())
+        np.testing.assert_array_equal(osum.cpu().numpy(), es)
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    assert oc.legacy_count(es) == po.legacy_count(es)
+
+
+def test_invalid_inputs_rejected(ctx):
+    import torch
+
+    _say("invalid inputs")
+    dv = torch.tensor([0, 1, -5, 127], dtype=torch.int8, device="cuda")
+    fk = torch.zeros(16, dtype=torch.int32, device="cuda")
+    m = torch.ones(16, dtype=torch.int32, device="cuda")
+    cols = oc.Context.columns([dv, None, None, None], [fk, None, None, None], m)
+    with pytest.raises(oc.OlapError) as e:
+        ctx.query(oc.plan_from([0], [1], 4), cols)
+    assert e.value.status == oc.E_INVALID
+    dv2 = torch.tensor([0, 1, 2, 127], dtype=torch.int8, device="cuda")
+    cols = oc.Context.columns([dv2, None, None, None], [fk, None, None, None], m)
+    with pytest.raises(oc.OlapError):
+        ctx.query(oc.plan_from([0], [1], 2), cols)  # code 2 reaches slot 2 of 2
+    with pytest.raises(oc.OlapError):
+        ctx.query(oc.plan_from([0, 0], [1, 1], 4), cols)  # repeated table
+
+
+def test_c1_full_size(ctx, ncpu):
+    """BASELINE config 1 at full size: 6M fact rows, one dimension."""
+    import torch
+
+    cfg = wl.C1
+    _say(f"{cfg.name}: generating on device")
+    ds = wl.DeviceStar(ctx, cfg)
+    dimvecs, _, _, fks, m1, m2 = cpu_star(cfg)
+    for d in cfg.dims:
+        np.testing.assert_array_equal(ds.dimvecs[d.slot].cpu().numpy(), dimvecs[d.slot])
+    p = po.make_plan(cfg.groups_by_slot(), cfg.bitmap_by_slot(), cfg.sele_by_slot())
+    es, ec, _ = po.run("rowwise", p, po.Star(dimvecs, fks, m1, m2), nthreads=ncpu)
+    G = int(ds.plan.group_nums)
+    q = ctx.query(ds.plan, ds.cols)
+    for s in STRATS:
+        _say(f"  {cfg.name} {s}")
+        osum = torch.empty(G, dtype=torch.int64, device="cuda")
+        ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+        q.run(s, osum, ocnt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+        np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
+
+
+def test_c2_full_size(ctx, ncpu):
+    """BASELINE config 2 at full size: 60M fact rows, four dimensions, G=875."""
+    import torch
+
+    cfg = wl.C2
+    _say(f"{cfg.name}: generating on device")
+    ds = wl.DeviceStar(ctx, cfg)
+    dimvecs, _, _, fks, m1, m2 = cpu_star(cfg)
+    p = po.make_plan(cfg.groups_by_slot(), cfg.bitmap_by_slot(), cfg.sele_by_slot())
+    assert list(p.orders) == list(ds.plan.orders) and list(p.factor) == list(ds.plan.factor)
+    es, ec, _ = po.run("vwm_dv", p, po.Star(dimvecs, fks, m1, m2), nthreads=ncpu)
+    assert ec.sum() > 0
+    G = int(ds.plan.group_nums)
+    q = ctx.query(ds.plan, ds.cols)
+    for s in STRATS:
+        _say(f"  {cfg.name} {s}")
+        osum = torch.empty(G, dtype=torch.int64, device="cuda")
+        ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+        q.run(s, osum, ocnt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+        np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
+
+
+def test_starjoin_substep(ctx):
+    import torch
+
+    rng = np.random.default_rng(41)
+    _say("starjoin sub-step")
+    rows = 300_001
+    dv, fk = [], []
+    for D in (3000, 200, 20000, 2555):
+        d = rng.integers(0, 16, D).astype(np.int8)
+        d[rng.random(D) < 0.4] = 127
+        dv.append(d)
+        fk.append(rng.integers(0, D + 1, rows, dtype=np.int32))
+    cnt, res = po.starjoin_count(dv, fk, want_result=True)
+    ddv = [_dev(d, torch) for d in dv]
+    dfk = [_dev(f, torch) for f in fk]
+    for s in STRATS:
+        r = torch.empty(rows, dtype=torch.int32, device="cuda") if s == "cwm_sv" else None
+        got = ctx.starjoin(s, ddv, dfk, result=r)
+        assert got == cnt, s
+        if r is not None:
+            np.testing.assert_array_equal(r.cpu().numpy(), res)
+
+
+@pytest.mark.parametrize("G", [32, 1024, 16384, 1 << 18])
+def test_vector_group_substep(ctx, G):
+    import torch
+
+    rng = np.random.default_rng(G)
+    _say(f"vector_group G={G}")
+    n = 2_000_003
+    key = rng.integers(0, G + 2, n, dtype=np.int32)  # includes 0 and G+1: bad keys
+    m1 = rng.integers(-1000, 1000, n, dtype=np.int32)
+    m2 = rng.integers(0, 5, n, dtype=np.int32)
+    tot, gv, bad = po.vector_group(key, m1, m2, G)
+    out = torch.empty(G, dtype=torch.int64, device="cuda")
+    gt, gbad = ctx.vector_group(_dev(key, torch), _dev(m1, torch), _dev(m2, torch), out)
+    assert gt == tot and gbad == bad
+    np.testing.assert_array_equal(out.cpu().numpy(), gv)
+
+
+def test_agg_expr_substep(ctx):
+    import torch
+
+    rng = np.random.default_rng(3)
+    n = 1 << 20
+    p = rng.integers(5, 21, n, dtype=np.int32)
+    q = rng.integers(1, 11, n, dtype=np.int32)
+    t = rng.integers(1, 21, n, dtype=np.int32)
+    assert ctx.agg_expr(_dev(p, torch), _dev(q, torch), _dev(t, torch)) == po.agg_expr(p, q, t)
