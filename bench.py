@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other strategies / configs")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the star-join path is measured on a single "
@@ -183,6 +185,19 @@ def main():
             per[s] = round(statistics.mean(e), 4)
             log(f"  strategy {s}: {per[s]:.3f} ms")
         extra["strategies_ms"] = per
+
+    if args.copy_calib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+        # calibrate the counters read for the scan kernel
+        nbytes = args.copy_calib << 20
+        src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nbytes} bytes read + written")
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

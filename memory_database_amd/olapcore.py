@@ -239,6 +239,8 @@ class Context:
         c.m1 = _ptr(m1)
         c.m2 = _ptr(m2)
         c.rows = rows if rows is not None else (m1.numel() if m1 is not None else 0)
+        # the struct holds raw device pointers: keep the tensors alive with it
+        c._keep = (list(dimvecs), list(fks), m1, m2)
         return c
 
     def query(self, plan: Plan, cols: Columns) -> "Query":
@@ -289,6 +291,7 @@ class Query:
         self.plan = plan
         self.cols = cols
         h = _P()
+        self.h = None
         ctx.check(ctx.lib.olap_query_create(ctx.h, C.byref(plan), C.byref(cols), C.byref(h)),
                   "query_create")
         self.h = h

@@ -1,0 +1,324 @@
+"""Pinning of the CPU oracle (no GPU needed).
+
+The oracle (oracle/olap_oracle.c) is checked against
+  - the golden fixtures written from the reference's own operator workers
+    (tests/golden/*.npz, tests/golden/make_golden.py),
+  - the compiled reference itself (oracle/_ref) when it was built here,
+  - a nested-loop brute-force join straight from filter/group columns,
+  - closed forms: a one-dimension query is a weighted histogram
+    (numpy.bincount); selectivity 0 gives nothing; marginals add up.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.cpu_star import brute_force, dense
+
+STRATS = ["rowwise", "vwm_dv", "vwm_sv", "cwm_dv", "cwm_sv"]
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF = po.Reference()
+needs_ref = pytest.mark.skipif(not REF.ok, reason="oracle/_ref not built (no reference sources)")
+
+
+# --------------------------------------------------------------------- plan
+
+def test_plan_known_case():
+    # BASELINE config 2 in the reference's table slots c, s, p, d
+    p = po.make_plan([25, 5, 1, 7], [1, 1, 1, 1], [0.2, 0.2, 0.04, 0.5])
+    assert list(p.orders) == [2, 0, 1, 3]
+    assert list(p.factor) == [875, 35, 7, 1]
+    assert p.group_nums == 875
+
+
+def test_sort_by_rate_positional_quirk():
+    # sort_by_rate compares sele[] by position while moving orders[]; with
+    # customer left out (groups 0) the comparisons read the wrong entries.
+    # orders starts [1, 2, 3]; sele[0] = 0.9 > sele[1] = 0.1 swaps orders 0/1,
+    # sele[0] > sele[2] = 0.5 swaps orders 0/2, sele[1] < sele[2] keeps.
+    assert po.sort_by_rate([0.9, 0.1, 0.5, 0.2], [1, 2, 3]) == [3, 1, 2]
+    p = po.make_plan([0, 5, 3, 7], [0, 1, 1, 1], [0.9, 0.1, 0.5, 0.2])
+    assert list(p.orders)[:3] == [3, 1, 2]
+    assert list(p.factor)[:3] == [15, 3, 1]
+    # orders [0, 1, 3]; sele[0] > sele[2] (part's unused 0.0) swaps 0/2 ->
+    # [3, 1, 0]; sele[1] > sele[2] swaps 1/2 -> [3, 0, 1]. customer is
+    # filter-only (bitmap 0): its factor stays 1; date's is 2 * 3.
+    p = po.make_plan([2, 3, 0, 4], [0, 1, 0, 1], [0.1, 0.2, 0.0, 0.3])
+    assert list(p.orders)[:3] == [3, 0, 1]
+    assert list(p.factor)[:3] == [6, 1, 1]
+
+
+def test_plan_matches_library_host_code():
+    from memory_database_amd import olapcore as oc
+
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        groups = [int(x) for x in rng.integers(0, 6, 4)]
+        if not any(groups):
+            continue
+        bitmap = [int(x) for x in rng.integers(0, 2, 4)]
+        sele = [float(x) for x in rng.choice([0.01, 0.1, 0.2, 0.5, 1.0], 4)]
+        a = po.make_plan(groups, bitmap, sele)
+        b = oc.make_plan(groups, bitmap, sele)
+        n = a.dimvec_nums
+        assert n == b.dimvec_nums
+        assert list(a.orders)[:n] == list(b.orders)[:n]
+        assert list(a.factor)[:n] == list(b.factor)[:n]
+        assert a.group_nums == b.group_nums
+
+
+@needs_ref
+def test_sort_by_rate_matches_reference():
+    rng = np.random.default_rng(1)
+    for _ in range(500):
+        n = int(rng.integers(1, 5))
+        sele = [float(x) for x in rng.choice([0.01, 0.1, 0.2, 0.5, 1.0], 4)]
+        orders = [int(x) for x in rng.permutation(4)[:n]]
+        assert po.sort_by_rate(sele, orders) == REF.sort_by_rate(sele, orders)
+
+
+# --------------------------------------------------------------------- generators
+
+def test_uniform_generator_properties():
+    a = po.gen_uniform(200_000, 7, 3, -5, 17)
+    assert a.min() == -5 and a.max() == 16
+    counts = np.bincount(a + 5)
+    assert counts.min() > 200_000 / 22 * 0.9
+    # counter-based: any slice regenerates identically
+    b = po.gen_uniform(1000, 7, 3, -5, 17, first_row=12_345)
+    np.testing.assert_array_equal(b, po.gen_uniform(13_345, 7, 3, -5, 17)[12_345:])
+    assert not np.array_equal(a[:1000], po.gen_uniform(1000, 7, 4, -5, 17))
+
+
+def test_zipf_table_and_sampler():
+    from memory_database_amd import olapcore as oc
+
+    for theta in (0.0, 0.5, 1.0):
+        thr = po.zipf_table(1000, theta)
+        assert thr[-1] == 2 ** 32 and np.all(np.diff(thr.astype(np.int64)) >= 0)
+        np.testing.assert_array_equal(thr, oc.zipf_table(1000, theta))
+    thr = po.zipf_table(1000, 1.0)
+    x = po.gen_zipf(400_000, 1, 9, thr)
+    assert x.min() >= 0 and x.max() < 1000
+    freq = np.bincount(x, minlength=1000) / x.size
+    h = (1.0 / np.arange(1, 1001)).sum()
+    assert abs(freq[0] - 1.0 / h) < 0.01       # P(rank 0) = 1/H_1000
+    assert abs(freq[1] - 0.5 / h) < 0.01
+
+
+@needs_ref
+def test_compat_generator_matches_reference_gen_data():
+    args = (0.2, 0.2, 0.04, 0.5, 1.0, [1, 0, 1, 1], [25, 1, 40, 7])
+    a = po.compat_gen_data(*args)
+    b = REF.gen_data(*args)
+    for x, y in zip(a[0], b[0]):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a[1], b[1]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[2], b[2])
+
+
+# --------------------------------------------------------------------- strategies
+
+def tiny_star(rng, ndims, rows):
+    slots = list(rng.permutation(4)[:ndims])
+    groups, bitmap, sele = [0] * 4, [0] * 4, [0.0] * 4
+    filters, gcols, preds, dimvecs, fks = [None] * 4, [None] * 4, [None] * 4, [None] * 4, [None] * 4
+    for t in slots:
+        D = int(rng.integers(1, 40))
+        g = int(rng.integers(1, 6))
+        bm = int(rng.integers(0, 2)) if g == 1 else 1
+        groups[t], bitmap[t] = g, bm
+        lo, hi = int(rng.integers(-5, 50)), int(rng.integers(0, 110))
+        sele[t] = float(rng.random())
+        filters[t] = rng.integers(0, 100, D, dtype=np.int32)
+        gcols[t] = rng.integers(0, g, D, dtype=np.int32)
+        preds[t] = (lo, hi)
+        dimvecs[t] = po.build_dimvec(filters[t], gcols[t], lo, hi, bm, 1)
+        fks[t] = rng.integers(-2, D + 3, rows, dtype=np.int32)  # some keys find no row
+    plan = po.make_plan(groups, bitmap, sele)
+    m1 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32) if rng.random() < 0.5 else None
+    return plan, filters, gcols, preds, bitmap, dimvecs, fks, m1, m2
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_strategies_match_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    ndims = 1 + seed % 4
+    rows = int(rng.integers(0, 400))
+    plan, filters, gcols, preds, bitmap, dv, fk, m1, m2 = tiny_star(rng, ndims, rows)
+    n = plan.dimvec_nums
+    exp, bad = brute_force(filters, gcols, preds, bitmap, fk, m1, m2, list(plan.orders)[:n],
+                           list(plan.factor)[:n], int(plan.group_nums))
+    es, ec = dense(exp, int(plan.group_nums))
+    star = po.Star(dv, fk, m1, m2, rows=rows)
+    for s in STRATS:
+        for nt in (1, 3):
+            gs, gc, gbad = po.run(s, plan, star, nthreads=nt)
+            assert gbad == bad
+            np.testing.assert_array_equal(gc, ec, err_msg=f"{s} nthreads={nt}")
+            np.testing.assert_array_equal(gs, es, err_msg=f"{s} nthreads={nt}")
+
+
+def test_one_dimension_is_a_weighted_histogram():
+    rng = np.random.default_rng(3)
+    D, N, g = 5000, 200_000, 37
+    f = rng.integers(0, 100, D, dtype=np.int32)
+    gc = rng.integers(0, g, D, dtype=np.int32)
+    dv = po.build_dimvec(f, gc, 0, 30, 1, 1)
+    fk = rng.integers(0, D, N, dtype=np.int32)
+    m = rng.integers(-1000, 1000, N, dtype=np.int32)
+    star = po.Star([dv, None, None, None], [fk, None, None, None], m)
+    p = po.plan([0], [1], g)
+    s, c, _ = po.run("vwm_dv", p, star, nthreads=4)
+    code = dv[fk].astype(np.int64)
+    keep = code != 127
+    np.testing.assert_array_equal(c, np.bincount(code[keep], minlength=g))
+    np.testing.assert_array_equal(s, np.bincount(code[keep], weights=m[keep], minlength=g)
+                                  .astype(np.int64))
+
+
+def test_selectivity_edge_cases_and_marginals():
+    rng = np.random.default_rng(4)
+    D0, D1, N = 300, 200, 50_000
+    g0 = rng.integers(0, 6, D0, dtype=np.int32)
+    g1 = rng.integers(0, 4, D1, dtype=np.int32)
+    f0 = rng.integers(0, 100, D0, dtype=np.int32)
+    f1 = rng.integers(0, 100, D1, dtype=np.int32)
+    fk0 = rng.integers(0, D0, N, dtype=np.int32)
+    fk1 = rng.integers(0, D1, N, dtype=np.int32)
+    m = rng.integers(0, 10_000, N, dtype=np.int32)
+    # selectivity 0 on one dimension: nothing survives
+    dv0 = po.build_dimvec(f0, g0, 0, 40, 1, 1)
+    none = po.build_dimvec(f1, g1, 0, 0, 1, 1)
+    star = po.Star([dv0, none, None, None], [fk0, fk1, None, None], m)
+    for s in STRATS:
+        _, c, _ = po.run(s, po.plan([0, 1], [4, 1], 24), star)
+        assert c.sum() == 0
+    # selectivity 1, one group: the sum of every measure
+    all1 = po.build_dimvec(f1, g1, -1, 1000, 0, 1)
+    star = po.Star([None, all1, None, None], [None, fk1, None, None], m)
+    s, c, _ = po.run("rowwise", po.plan([1], [1], 1), star)
+    assert c[0] == N and s[0] == int(m.astype(np.int64).sum())
+    # marginal: the 2-dim query summed over dimension 1's groups (with
+    # dimension 1 at selectivity 1) equals the 1-dim query on dimension 0
+    dv1 = po.build_dimvec(f1, g1, -1, 1000, 1, 1)
+    star2 = po.Star([dv0, dv1, None, None], [fk0, fk1, None, None], m)
+    s2, c2, _ = po.run("cwm_dv", po.plan([0, 1], [4, 1], 24), star2, nthreads=2)
+    star1 = po.Star([dv0, None, None, None], [fk0, None, None, None], m)
+    s1, c1, _ = po.run("vwm_sv", po.plan([0], [1], 6), star1)
+    np.testing.assert_array_equal(s2.reshape(6, 4).sum(1), s1)
+    np.testing.assert_array_equal(c2.reshape(6, 4).sum(1), c1)
+
+
+def test_out_of_range_gid_is_counted_not_written():
+    dv = np.array([0, 1, 2, 3], np.int8)
+    fk = np.array([0, 1, 2, 3, 3, 9], np.int32)
+    m = np.ones(6, np.int32)
+    star = po.Star([dv, None, None, None], [fk, None, None, None], m)
+    for s in STRATS:
+        _, c, bad = po.run(s, po.plan([0], [1], 2), star)
+        assert bad == 3 and list(c) == [1, 1]
+
+
+def test_legacy_count_wraps_like_the_reference():
+    from memory_database_amd import olapcore as oc
+
+    sums = np.array([2 ** 32 + 5, 2 ** 31, 2 ** 31, -1], np.int64)
+    # uint32 adds: 5 + 2^31 + 2^31 + (2^32 - 1) = 4 (mod 2^32)
+    assert po.legacy_count(sums) == 4 == oc.legacy_count(sums)
+
+
+# --------------------------------------------------------------------- reference
+
+def _load_case(path):
+    z = np.load(path)
+    dv = [z[f"dimvec{t}"] for t in range(4)]
+    fk = [z[f"fk{t}"] for t in range(4)]
+    return z, dv, fk
+
+
+GOLDEN_CASES = sorted(glob.glob(os.path.join(GOLDEN, "olapcore_*.npz")))
+
+
+def test_golden_fixtures_present():
+    assert len(GOLDEN_CASES) >= 6
+
+
+@pytest.mark.parametrize("path", GOLDEN_CASES, ids=os.path.basename)
+def test_oracle_matches_golden(path):
+    z, dv, fk = _load_case(path)
+    orders, factor, G = list(z["orders"]), list(z["factor"]), int(z["group_nums"])
+    p = po.plan(orders, factor, G)
+    # the fixture vectors carry the reference's NULL pad at index D
+    star = po.Star(dv, fk, z["m1"], z["m2"])
+    for s in STRATS:
+        gs, gc, bad = po.run(s, p, star, nthreads=2)
+        assert bad == 0
+        np.testing.assert_array_equal(gs.astype(np.uint32), z[f"{s}_sum_u32"], err_msg=s)
+        np.testing.assert_array_equal(gc.astype(np.uint32), z[f"{s}_count_u32"], err_msg=s)
+    gid = po.join_cwm_sv(p, star)
+    exp = z["cwm_sv_gid"].astype(np.int32)
+    np.testing.assert_array_equal(np.where(gid < 0, 32767, gid), exp)
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_compiled_reference(seed):
+    """Fresh random stars through the reference's own workers."""
+    rng = np.random.default_rng(100 + seed)
+    rows = int(rng.integers(1, 30_000))
+    plan, _, _, _, _, dv, fk, m1, m2 = tiny_star(rng, 1 + seed % 4, rows)
+    n = plan.dimvec_nums
+    orders, factor, G = list(plan.orders)[:n], list(plan.factor)[:n], int(plan.group_nums)
+    # reference inputs: FKs inside [0, D] and a NULL pad at D
+    dvr, fkr = [], []
+    for t in range(4):
+        d = dv[t] if dv[t] is not None else np.full(1, 127, np.int8)
+        dvr.append(np.concatenate([d, np.array([127], np.int8)]))
+        f = fk[t] if fk[t] is not None else np.zeros(rows, np.int32)
+        fkr.append(np.clip(f, 0, d.size).astype(np.int32))
+    a = rng.integers(0, 1000, rows, dtype=np.int32)
+    b = rng.integers(0, 1000, rows, dtype=np.int32)
+    star = po.Star(dvr, fkr, a, b)
+    ones, zeros = np.ones(rows, np.int32), np.zeros(rows, np.int32)
+    for s in STRATS:
+        gs, gc, _ = po.run(s, po.plan(orders, factor, G), star, nthreads=3)
+        r = REF.run(s, dvr, fkr, orders, factor, G, a, b)
+        c = REF.run(s, dvr, fkr, orders, factor, G, ones, zeros)
+        np.testing.assert_array_equal(gs.astype(np.uint32), r, err_msg=s)
+        np.testing.assert_array_equal(gc.astype(np.uint32), c, err_msg=s)
+
+
+# --------------------------------------------------------------------- sub-steps
+
+def test_starjoin_matches_golden():
+    z = np.load(os.path.join(GOLDEN, "starjoin_seed21.npz"))
+    dv = [z[f"dimvec{t}"] for t in range(4)]
+    fk = [z[f"fk{t}"] for t in range(4)]
+    cnt, res = po.starjoin_count(dv, fk, want_result=True)
+    assert cnt == int(z["count"]) == int(z["count_cwm_sv"])
+    np.testing.assert_array_equal(res, z["result"])
+
+
+@pytest.mark.parametrize("G", [32, 1024])
+def test_vector_group_matches_golden(G):
+    z = np.load(os.path.join(GOLDEN, f"group_G{G}.npz"))
+    tot, gv, bad = po.vector_group(z["key"], z["m1"], z["m2"], G)
+    assert bad == 0 and tot == int(z["total"])
+    np.testing.assert_array_equal(gv.astype(np.int32), z["group_vector"])
+
+
+def test_agg_expr_closed_form():
+    rng = np.random.default_rng(8)
+    p = rng.integers(5, 21, 10_000, dtype=np.int32)
+    q = rng.integers(1, 11, 10_000, dtype=np.int32)
+    t = rng.integers(1, 21, 10_000, dtype=np.int32)
+    pq = p.astype(np.int64) * q
+    assert po.agg_expr(p, q, t) == int((pq * t + 100 * pq).sum())
+    # 100x the reference's double formula, exactly
+    ref = float(((p * q) * (t / 100.0) + p * q).astype(np.float64).sum())
+    assert abs(po.agg_expr(p, q, t) / 100.0 - ref) < 1e-6 * ref
