@@ -165,6 +165,11 @@ def main():
     osum = torch.empty(G, dtype=torch.int64, device="cuda")
     ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
 
+    # the autotuner times every launch shape once (untimed) and keeps the best
+    shape = q.autotune(args.strategy, osum, ocnt, stream=s_handle)
+    log(f"autotune: shape {shape[0]} = {shape[1]} threads x {shape[2]} rows/thread "
+        f"({shape[3]:.3f} ms)")
+
     def step():
         q.run(args.strategy, osum, ocnt, stream=s_handle)
 
@@ -221,6 +226,7 @@ def main():
                    "dims": [{"slot": d.slot, "rows": d.rows, "sele": d.sele, "groups": d.groups}
                             for d in cfg.dims],
                    "measure": [cfg.m_lo, cfg.m_hi], "group_nums": G, "strategy": args.strategy,
+                   "launch": {"threads": shape[1], "rows_per_thread": shape[2]},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -131,6 +131,19 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
 olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                            void *stream);
 olap_status olap_query_destroy(olap_query q);
+/* Launch shapes (threads per workgroup x rows per thread) of a strategy's
+ * scan kernel, and the autotuner: it runs every shape of `strategy` on the
+ * query (sum/count are overwritten, then hold a run of the winner), keeps the
+ * fastest median-of-3 for this query and caches it in the context for
+ * queries of the same class (strategy, probe steps, group-table residency,
+ * log2 rows). Synchronises. No counterpart in the reference, which has one
+ * fixed launch per strategy. */
+int32_t olap_num_shapes(olap_strategy strategy);
+olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *threads,
+                            int32_t *rows_per_thread);
+olap_status olap_query_set_shape(olap_query q, int32_t shape);
+olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
+                                void *stream, int32_t *shape_out, double *ms_out);
 /* bytes of device memory the query holds (packed vectors, scratch) */
 int64_t olap_query_device_bytes(olap_query q);
 

@@ -246,6 +246,26 @@ olap_status run_cwm_dv_aggregate(olap_query_s *q, int64_t *sum, int64_t *count, 
     return OLAP_OK;
 }
 
+// autotuner cache key: strategy, probe steps, LDS residency of the group
+// table, log2 of the fact-row count
+uint64_t query_class(const olap_query_s *q, olap_strategy strat)
+{
+    Layout L = plan_layout(q, true, true, strat == OLAP_VWM_DV, -1);
+    uint64_t rows_log2 = 0;
+    while ((1ull << rows_log2) < (uint64_t)std::max<int64_t>(q->cols.rows, 1)) rows_log2++;
+    return ((uint64_t)strat << 40) | ((uint64_t)q->plan.dimvec_nums << 32) |
+           ((uint64_t)L.gt_lds << 31) | rows_log2;
+}
+
+int effective_shape(const olap_query_s *q)
+{
+    if (q->row_shape >= 0) return q->row_shape;
+    const uint64_t key = query_class(q, OLAP_ROWWISE);
+    for (const auto &kv : q->ctx->tuned)
+        if (kv.first == key) return kv.second;
+    return 0;
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -632,7 +652,8 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
         ScanArgs a;
         fill_args(a, q, L, sum_p, count);
         hipError_t e;
-        if (strategy == OLAP_ROWWISE) e = run_rowwise(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
+        if (strategy == OLAP_ROWWISE)
+            e = run_rowwise(a, nd, ctx->num_cus, L.dyn_lds, vec, effective_shape(q), s);
         else if (strategy == OLAP_VWM_SV) e = run_vwm_sv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         else e = run_vwm_dv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "scan kernel");
@@ -665,6 +686,87 @@ olap_status olap_query_destroy(olap_query q)
     if (q->gvec) dev_free(q->gvec);
     delete q;
     return OLAP_OK;
+}
+
+int32_t olap_num_shapes(olap_strategy strategy)
+{
+    return strategy == OLAP_ROWWISE ? (int32_t)num_row_shapes() : 1;
+}
+
+olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *threads,
+                            int32_t *rows_per_thread)
+{
+    if (shape < 0 || shape >= olap_num_shapes(strategy)) return OLAP_E_INVALID;
+    ScanShape sh = strategy == OLAP_ROWWISE ? row_shape(shape) : ScanShape{kBlock, kUnroll};
+    if (threads) *threads = sh.bs;
+    if (rows_per_thread) *rows_per_thread = sh.unroll * kChunk;
+    return OLAP_OK;
+}
+
+olap_status olap_query_set_shape(olap_query q, int32_t shape)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (shape < -1 || shape >= (int32_t)num_row_shapes())
+        return fail(q->ctx, OLAP_E_INVALID, "shape %d out of range", shape);
+    q->row_shape = shape;
+    return OLAP_OK;
+}
+
+olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
+                                void *stream, int32_t *shape_out, double *ms_out)
+{
+    if (!q) return OLAP_E_INVALID;
+    olap_ctx ctx = q->ctx;
+    const int n = olap_num_shapes(strategy);
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    hipEvent_t e0, e1;
+    HIPC(ctx, hipEventCreate(&e0));
+    HIPC(ctx, hipEventCreate(&e1));
+    int best = 0;
+    double best_ms = 1e30;
+    const int saved = q->row_shape;
+    olap_status st = OLAP_OK;
+    for (int i = 0; i < n && st == OLAP_OK; i++) {
+        q->row_shape = i;
+        st = olap_query_run(q, strategy, sum, count, stream);  // warm
+        std::vector<double> t;
+        for (int r = 0; r < 3 && st == OLAP_OK; r++) {
+            hipError_t e = hipEventRecord(e0, s);
+            if (e == hipSuccess) st = olap_query_run(q, strategy, sum, count, stream);
+            if (e == hipSuccess && st == OLAP_OK) e = hipEventRecord(e1, s);
+            if (e == hipSuccess && st == OLAP_OK) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess && st == OLAP_OK) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) st = hip_fail(ctx, e, "autotune timing");
+            t.push_back(ms);
+        }
+        if (st != OLAP_OK) break;
+        std::sort(t.begin(), t.end());
+        if (t[1] < best_ms) {
+            best_ms = t[1];
+            best = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != OLAP_OK) {
+        q->row_shape = saved;
+        return st;
+    }
+    q->row_shape = best;
+    const uint64_t key = query_class(q, strategy);
+    bool found = false;
+    for (auto &kv : ctx->tuned)
+        if (kv.first == key) {
+            kv.second = best;
+            found = true;
+        }
+    if (!found) ctx->tuned.push_back({key, best});
+    if (shape_out) *shape_out = best;
+    if (ms_out) *ms_out = best_ms;
+    // leave the outputs holding a complete run of the chosen shape
+    return olap_query_run(q, strategy, sum, count, stream);
 }
 
 int64_t olap_query_device_bytes(olap_query q)

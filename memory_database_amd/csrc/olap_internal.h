@@ -20,6 +20,11 @@ constexpr int kPerThreadV = kTileV / kBlock;        // list entries per thread (
 constexpr uint32_t kLdsBytes = 160 * 1024;          // LDS per CU on gfx950
 constexpr uint32_t kLdsListBytes = kTileV * 2 + kTileV * 4 + 16;  // u16 off + u32 gid + counter
 
+struct ScanShape {
+    int bs;      // threads per workgroup
+    int unroll;  // 128-bit chunks per thread per tile
+};
+
 // One probe step as the kernels see it. Codes of the step's dimension are
 // packed 32/bits per word; the all-ones code is NULL.
 struct StepDesc {
@@ -68,6 +73,8 @@ struct olap_ctx_s {
     int64_t hbm_bytes = 0;
     char arch[64] = {0};
     std::string err;
+    // autotuner cache: query class -> chosen launch shape
+    std::vector<std::pair<uint64_t, int>> tuned;
     uint32_t *d_err = nullptr;     // error counter (4 words)
     void *d_scratch = nullptr;     // small reduction scratch
     size_t scratch_bytes = 0;
@@ -88,6 +95,7 @@ struct olap_query_s {
     int32_t *gvec = nullptr;           // column-wise static vector scratch
     size_t gvec_rows = 0;
     bool joined_dv = false;
+    int row_shape = -1;                // row-wise launch shape (-1: default / cached)
 };
 
 namespace olap {
@@ -110,7 +118,9 @@ hipError_t launch_pack(const void *dimvec, int32_t code_bytes, int64_t rows, uin
 
 // strategy kernels; args.gsum/gcnt must be zeroed by the caller
 hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
-                       hipStream_t s);
+                       int shape, hipStream_t s);
+int num_row_shapes();
+ScanShape row_shape(int i);
 hipError_t run_vwm_sv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
                       hipStream_t s);
 hipError_t run_vwm_dv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,

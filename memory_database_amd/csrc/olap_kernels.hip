@@ -325,13 +325,13 @@ __device__ __forceinline__ void lds_init(const ScanArgs &a, uint32_t *lds)
             const uint32_t n = a.st[s].nwords;
             const uint32_t off = a.st[s].lds_off;
             const uint32_t *src = a.st[s].words;
-            for (uint32_t i = threadIdx.x; i < n; i += kBlock) lds[off + i] = src[i];
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds[off + i] = src[i];
         }
     }
     if (GLDS) {
         unsigned long long *lsum = (unsigned long long *)(lds + a.gt_off);
         uint32_t *lcnt = lds + a.gt_off + 2u * a.G;
-        for (uint32_t g = threadIdx.x; g < a.G; g += kBlock) {
+        for (uint32_t g = threadIdx.x; g < a.G; g += blockDim.x) {
             lsum[g] = 0ull;
             lcnt[g] = 0u;
         }
@@ -364,7 +364,7 @@ __device__ __forceinline__ void flush(const ScanArgs &a, uint32_t *lds)
     __syncthreads();
     const unsigned long long *lsum = (const unsigned long long *)(lds + a.gt_off);
     const uint32_t *lcnt = lds + a.gt_off + 2u * a.G;
-    for (uint32_t g = threadIdx.x; g < a.G; g += kBlock) {
+    for (uint32_t g = threadIdx.x; g < a.G; g += blockDim.x) {
         const uint32_t c = lcnt[g];
         if (c) {
             atomic_add_u64_dev(&a.gcnt[g], (unsigned long long)c);
@@ -401,54 +401,72 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want)
 // chunks that still hold a live row, so dead lines are never fetched.
 // ------------------------------------------------------------------------
 
-template <int NDIM, bool VEC, bool GLDS>
-__global__ __launch_bounds__(kBlock) void k_rowwise(ScanArgs a)
+template <int NDIM, bool VEC, bool GLDS, int BS, int U>
+__global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
 {
+    constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
     extern __shared__ __align__(16) uint32_t lds[];
     lds_init<NDIM, GLDS>(a, lds);
     __syncthreads();
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
-    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-        const uint64_t base = tile * (uint64_t)kTileRW;
-        v4i f[kUnroll];
-        uint32_t alive[kUnroll];
-        uint32_t gid[kUnroll][4];
+    // software pipeline: the first key column of the next tile is in flight
+    // while the dependent probe chain of the current tile runs
+    v4i nf[U];
+    uint32_t np[U];
+    uint64_t tile = blockIdx.x;
+    if (tile < a.ntiles) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
-            f[u] = load4<VEC>(a.st[0].fk, r, rows, alive[u]);
+        for (int u = 0; u < U; u++)
+            nf[u] = load4<VEC>(a.st[0].fk, tile * TILE + (uint64_t)(u * BS + tid) * 4u, rows, np[u]);
+    }
+    for (; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * TILE;
+        v4i f[U];
+        uint32_t alive[U];
+        uint32_t gid[U][4];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            f[u] = nf[u];
+            alive[u] = np[u];
+        }
+        const uint64_t next = tile + gridDim.x;
+        if (next < a.ntiles) {
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                nf[u] = load4<VEC>(a.st[0].fk, next * TILE + (uint64_t)(u * BS + tid) * 4u, rows,
+                                   np[u]);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
+        for (int u = 0; u < U; u++) probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
 #pragma unroll
         for (int s = 1; s < NDIM; s++) {
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
+            for (int u = 0; u < U; u++) {
                 if (alive[u]) {
                     uint32_t dummy;
-                    const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+                    const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
                     f[u] = load4<VEC>(a.st[s].fk, r, rows, dummy);
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++)
+            for (int u = 0; u < U; u++)
                 if (alive[u]) probe4(a.st[s], lds, f[u], gid[u], alive[u], false);
         }
-        v4i m1[kUnroll], m2[kUnroll];
+        v4i m1[U], m2[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < U; u++) {
             m1[u] = v4i{0, 0, 0, 0};
             m2[u] = v4i{0, 0, 0, 0};
             if (alive[u] && !a.count_only) {
                 uint32_t dummy;
-                const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
+                const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
                 m1[u] = load4<VEC>(a.m1, r, rows, dummy);
                 if (a.m2) m2[u] = load4<VEC>(a.m2, r, rows, dummy);
             }
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < U; u++) {
             if (alive[u]) {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
@@ -824,7 +842,8 @@ __global__ __launch_bounds__(kBlock) void k_cwm_dv_agg(ScanArgs a)
 
 template <typename K>
 static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t dyn_lds,
-                                    uint64_t work_units, hipStream_t s)
+                                    uint64_t work_units, hipStream_t s, int bs = kBlock,
+                                    int max_per_cu = 0)
 {
     if (dyn_lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)kernel,
@@ -832,13 +851,14 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
         if (e != hipSuccess) return e;
     }
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, dyn_lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, dyn_lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) return hipErrorInvalidConfiguration;
+    if (max_per_cu > 0 && per_cu > max_per_cu) per_cu = max_per_cu;
     uint64_t grid = (uint64_t)per_cu * (uint64_t)num_cus;
     if (grid > work_units) grid = work_units;
     if (grid < 1) grid = 1;
-    kernel<<<(unsigned)grid, kBlock, dyn_lds, s>>>(a);
+    kernel<<<(unsigned)grid, bs, dyn_lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -859,13 +879,69 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
         else { OLAP_DISPATCH_NDIM(KERNEL, false, false) }                         \
     }
 
-hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
-                       hipStream_t s)
+// Row-wise launch shapes: (threads per workgroup, 128-bit chunks per thread
+// per tile). The autotuner times them on a live query.
+const ScanShape kRowShapes[] = {{512, 4}, {512, 2}, {1024, 2}, {1024, 4}, {256, 8}, {512, 8}};
+const int kNumRowShapes = (int)(sizeof(kRowShapes) / sizeof(kRowShapes[0]));
+
+template <int NDIM, bool VEC, bool GLDS>
+static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int shape,
+                                hipStream_t s)
 {
-    a.ntiles = (a.rows + kTileRW - 1) / kTileRW;
+    const ScanShape sh = kRowShapes[shape];
+    a.ntiles = (a.rows + (uint64_t)sh.bs * 4u * sh.unroll - 1) / ((uint64_t)sh.bs * 4u * sh.unroll);
     if (a.ntiles == 0) return hipSuccess;
-    OLAP_DISPATCH_ALL(k_rowwise)
+    switch (shape) {
+    case 0: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 4>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+    case 1: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 2>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+    case 2: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 1024, 2>, a, num_cus, dyn_lds, a.ntiles, s, 1024);
+    case 3: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 1024, 4>, a, num_cus, dyn_lds, a.ntiles, s, 1024);
+    case 4: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 256, 8>, a, num_cus, dyn_lds, a.ntiles, s, 256);
+    default: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 8>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+    }
 }
+
+hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
+                       int shape, hipStream_t s)
+{
+    if (shape < 0 || shape >= kNumRowShapes) shape = 0;
+    if (!vec) {
+        // unaligned columns: scalar loads, default shape only
+        a.ntiles = (a.rows + kTileRW - 1) / kTileRW;
+        if (a.ntiles == 0) return hipSuccess;
+        if (a.gt_lds) {
+            switch (ndims) {
+            case 1: return launch_persistent(k_rowwise<1, false, true, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+            case 2: return launch_persistent(k_rowwise<2, false, true, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+            case 3: return launch_persistent(k_rowwise<3, false, true, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+            default: return launch_persistent(k_rowwise<4, false, true, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+            }
+        }
+        switch (ndims) {
+        case 1: return launch_persistent(k_rowwise<1, false, false, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+        case 2: return launch_persistent(k_rowwise<2, false, false, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+        case 3: return launch_persistent(k_rowwise<3, false, false, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+        default: return launch_persistent(k_rowwise<4, false, false, kBlock, kUnroll>, a, num_cus, dyn_lds, a.ntiles, s);
+        }
+    }
+    if (a.gt_lds) {
+        switch (ndims) {
+        case 1: return rowwise_shape<1, true, true>(a, num_cus, dyn_lds, shape, s);
+        case 2: return rowwise_shape<2, true, true>(a, num_cus, dyn_lds, shape, s);
+        case 3: return rowwise_shape<3, true, true>(a, num_cus, dyn_lds, shape, s);
+        default: return rowwise_shape<4, true, true>(a, num_cus, dyn_lds, shape, s);
+        }
+    }
+    switch (ndims) {
+    case 1: return rowwise_shape<1, true, false>(a, num_cus, dyn_lds, shape, s);
+    case 2: return rowwise_shape<2, true, false>(a, num_cus, dyn_lds, shape, s);
+    case 3: return rowwise_shape<3, true, false>(a, num_cus, dyn_lds, shape, s);
+    default: return rowwise_shape<4, true, false>(a, num_cus, dyn_lds, shape, s);
+    }
+}
+
+int num_row_shapes() { return kNumRowShapes; }
+ScanShape row_shape(int i) { return kRowShapes[i < 0 || i >= kNumRowShapes ? 0 : i]; }
 
 hipError_t run_vwm_sv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
                       hipStream_t s)

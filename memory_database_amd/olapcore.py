@@ -81,6 +81,11 @@ SIGNATURES = {
     "olap_query_run": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "olap_query_destroy": (C.c_int, [_P]),
     "olap_query_device_bytes": (C.c_int64, [_P]),
+    "olap_num_shapes": (C.c_int32, [C.c_int]),
+    "olap_shape_info": (C.c_int, [C.c_int, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "olap_query_set_shape": (C.c_int, [_P, C.c_int32]),
+    "olap_query_autotune": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double)]),
     "olap_rowwise": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
     "olap_vwm_dv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
     "olap_vwm_sv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
@@ -313,6 +318,20 @@ class Query:
     def agg_cwm_dv(self, out_sum, out_cnt, stream=None):
         self.ctx.check(self.ctx.lib.olap_agg_cwm_dv(self.h, _ptr(out_sum), _ptr(out_cnt), stream),
                        "agg_cwm_dv")
+
+    def autotune(self, strategy, out_sum, out_cnt, stream=None):
+        """Returns (shape index, threads, rows per thread, median ms)."""
+        sh = C.c_int32()
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.olap_query_autotune(self.h, STRATEGIES[strategy], _ptr(out_sum),
+                                                        _ptr(out_cnt), stream, C.byref(sh),
+                                                        C.byref(ms)), "autotune")
+        th, rp = C.c_int32(), C.c_int32()
+        self.ctx.lib.olap_shape_info(STRATEGIES[strategy], sh.value, C.byref(th), C.byref(rp))
+        return sh.value, th.value, rp.value, ms.value
+
+    def set_shape(self, shape: int):
+        self.ctx.check(self.ctx.lib.olap_query_set_shape(self.h, shape), "set_shape")
 
     def device_bytes(self) -> int:
         return int(self.ctx.lib.olap_query_device_bytes(self.h))

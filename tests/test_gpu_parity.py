@@ -352,3 +352,33 @@ def test_agg_expr_substep(ctx):
     q = rng.integers(1, 11, n, dtype=np.int32)
     t = rng.integers(1, 21, n, dtype=np.int32)
     assert ctx.agg_expr(_dev(p, torch), _dev(q, torch), _dev(t, torch)) == po.agg_expr(p, q, t)
+
+
+def test_every_row_shape_matches_oracle(ctx, ncpu):
+    """Each launch shape of the row-wise scan, and the autotuner's pick."""
+    import torch
+
+    rng = np.random.default_rng(51)
+    plan_np, dv, fk, m1, m2 = random_star(rng, 4, 1_234_567, oob=True)
+    star = po.Star(dv, fk, m1, m2)
+    es, ec, _ = po.run("rowwise", plan_np, star, nthreads=ncpu)
+    plan = oc.plan_from(list(plan_np.orders)[: plan_np.dimvec_nums],
+                        list(plan_np.factor)[: plan_np.dimvec_nums], int(plan_np.group_nums))
+    cols = oc.Context.columns([_dev(d, torch) for d in dv], [_dev(f, torch) for f in fk],
+                              _dev(m1, torch), _dev(m2, torch), rows=m1.size)
+    G = int(plan.group_nums)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    for shape in range(ctx.lib.olap_num_shapes(oc.ROWWISE)):
+        _say(f"  row shape {shape}")
+        q.set_shape(shape)
+        q.run("rowwise", osum, ocnt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"shape {shape}")
+        np.testing.assert_array_equal(osum.cpu().numpy(), es, err_msg=f"shape {shape}")
+    shape, threads, rpt, ms = q.autotune("rowwise", osum, ocnt)
+    _say(f"  autotune picked shape {shape} ({threads} threads x {rpt} rows) {ms:.3f} ms")
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
