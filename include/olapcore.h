@@ -175,6 +175,73 @@ olap_status olap_agg_cwm_dv(olap_query q, int64_t *sum, int64_t *count, void *st
  * vector summed into an int (e.g. :1027-1036) -- from int64 sums (host). */
 int32_t olap_legacy_count(const int64_t *sum_host, int64_t group_nums);
 
+/* ---- batched form: many star-join group-by queries in one scan ------------ */
+/* No counterpart in the reference, which runs one query per scan. A batch
+ * holds up to 64 queries over one shared star schema; a single persistent
+ * kernel reads the fact table once and evaluates every query on every row.
+ * Each query is given by its predicates and group-by expressions rather than
+ * by pre-built dimension vectors: step i probes table[i], passes when
+ * lo[i] <= filter[table[i]][key] < hi[i], and contributes the code
+ * (uint32)attr[table[i]][key] % group_mod[i] (0 when group_mod[i] == 0, a
+ * filter-only step) times factor[i] to the gid. The result of query q is the
+ * one olap_query_run gives for the same plan over the int32 dimension vectors
+ * olap_build_dimvec(filter, attr, lo, hi, group_mod > 0, group_mod, 4). */
+#define OLAP_BATCH_MAX 64
+
+typedef struct {
+    int32_t ndims;                      /* probe steps, 1..4, distinct tables */
+    int32_t table[OLAP_MAX_DIMS];       /* table slot of step i */
+    int32_t lo[OLAP_MAX_DIMS];          /* predicate lo <= filter < hi */
+    int32_t hi[OLAP_MAX_DIMS];
+    int32_t group_mod[OLAP_MAX_DIMS];   /* code = attr % group_mod; 0: filter only */
+    int32_t factor[OLAP_MAX_DIMS];      /* multiplier of step i's code */
+    int64_t group_nums;                 /* output slots of the query */
+} olap_batch_query;
+
+/* The shared star schema: per table slot a filter and a group attribute
+ * column (device, dim_rows[t] entries; NULL for tables no query probes), and
+ * the fact columns. */
+typedef struct {
+    const int32_t *filter[OLAP_MAX_DIMS];
+    const int32_t *attr[OLAP_MAX_DIMS];
+    int64_t dim_rows[OLAP_MAX_DIMS];
+    const int32_t *fk[OLAP_MAX_DIMS];
+    const int32_t *m1;
+    const int32_t *m2;                  /* nullable */
+    int64_t rows;
+} olap_star;
+
+typedef struct olap_batch_s *olap_batch;
+
+/* Validate the queries, build the per-table query masks (bit q of mask[t][r]
+ * set when query q passes dimension row r of table t or does not probe t)
+ * and plan the LDS layout (synchronises). */
+olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
+                              int32_t nq, olap_batch *out);
+/* offsets[q] = first output slot of query q, offsets[nq] = total slots (host) */
+olap_status olap_batch_offsets(olap_batch b, int64_t *offsets);
+/* Zero the concatenated outputs (total slots int64 each) and run the fused
+ * scan. Asynchronous, no host synchronisation and no allocation: the call can
+ * be captured into a hipGraph. */
+olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *stream);
+/* Autotuner of the batched scan: times every (threads, rows per thread, LDS
+ * budget for per-query group tables) configuration on this batch, keeps the
+ * fastest; outputs then hold a run of the winner. Synchronises. */
+int32_t olap_batch_num_configs(void);
+olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
+                                   int32_t *lds_table_kib);
+olap_status olap_batch_set_config(olap_batch b, int32_t cfg);
+olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void *stream,
+                                int32_t *cfg_out, double *ms_out);
+/* Capture zero + scan for these outputs into a hipGraph held by the batch,
+ * then replay it; olap_batch_graph_launch is asynchronous. */
+olap_status olap_batch_graph_capture(olap_batch b, int64_t *sum, int64_t *count, void *stream);
+olap_status olap_batch_graph_launch(olap_batch b, void *stream);
+/* Expected (row, query) matches per fact row, from the mask statistics
+ * (uniform keys assumed); host, after create. */
+double olap_batch_expected_events(olap_batch b);
+olap_status olap_batch_destroy(olap_batch b);
+
 /* ---- sub-step operators --------------------------------------------------- */
 /* starjoinalgo_rowwise / _cwm_sv / _cwm_dv (code/starjoin/test_starjoin_option.cpp:37-189):
  * rows passing all four int8 dimension vectors (c, s, p, d). result (device,

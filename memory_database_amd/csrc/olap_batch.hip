@@ -1,0 +1,702 @@
+// olap_batch.hip -- the batched form: up to 64 star-join group-by queries
+// evaluated in one persistent scan of the shared fact table.
+//
+// Per dimension table t the batch keeps a 64-bit query mask per dimension
+// row (bit q: query q passes row r of t, or does not probe t). Per fact row
+// the kernel gathers one mask and one group attribute per probed table --
+// from LDS for the small tables, from L2/HBM for the large ones -- ANDs the
+// masks, and for every surviving query composes its gid from the attributes
+// (code = attr mod group_mod by an exact multiply-high fast modulo) and adds
+// the measure into the query's group table: in LDS when the autotuned LDS
+// budget holds it, else with global atomics on the query's output slice.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "olap_internal.h"
+
+namespace olap {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kNoLds = 0xffffffffu;
+
+// one query as the batched kernel reads it (96 bytes, kept in LDS)
+struct BQDesc {
+    uint32_t nsteps;
+    uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when global
+    uint32_t G;
+    uint32_t out_off;   // first slot in the concatenated outputs
+    uint32_t tab[4];
+    uint32_t fac[4];
+    uint32_t g[4];      // group_mod (0: code 0)
+    uint64_t M[4];      // fast-modulo multiplier: UINT64_MAX / g + 1
+};
+static_assert(sizeof(BQDesc) == 96, "descriptor layout");
+
+struct BatchArgs {
+    const int32_t *fk[4];
+    const int32_t *m1;
+    const int32_t *m2;
+    uint64_t rows;
+    uint64_t ntiles;
+    const unsigned long long *gmask[4];
+    const int32_t *gattr[4];
+    uint32_t dim_rows[4];
+    uint32_t mask_lds[4];   // word offset of the LDS copy or kNoLds
+    uint32_t attr_lds[4];
+    unsigned long long nouse[4];
+    uint32_t used;          // bit t: some query probes table t
+    uint32_t nq;
+    const BQDesc *desc;
+    uint32_t desc_lds;      // word offset of the descriptors in LDS
+    uint32_t tab_begin;     // LDS words [tab_begin, tab_end) hold query tables
+    uint32_t tab_end;
+    unsigned long long *gsum;
+    unsigned long long *gcnt;
+    uint32_t *err;
+};
+
+struct MaskPred {
+    int32_t q;
+    int32_t lo;
+    int32_t hi;
+    int32_t pad;
+};
+
+// explicit address spaces keep the LDS and the HBM atomics apart (a generic
+// 64-bit atomic would need a run-time address-space check)
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
+typedef __attribute__((address_space(3))) unsigned long long l_u64;
+typedef __attribute__((address_space(3))) uint32_t l_u32;
+
+__device__ __forceinline__ void b_atomic_dev(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_add((g_u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void b_atomic_lds(uint32_t *tab, uint32_t G, uint32_t gid,
+                                             unsigned long long v)
+{
+    l_u64 *ls = (l_u64 *)tab;
+    l_u32 *lc = (l_u32 *)(tab + 2u * G);
+    __hip_atomic_fetch_add(ls + gid, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(lc + gid, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---------------------------------------------------------------- masks
+
+__global__ void k_batch_masks(const int32_t *filter, uint32_t D, const MaskPred *preds, int np,
+                              unsigned long long base, unsigned long long *mask, uint32_t *pass)
+{
+    for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < D; r0 += gridDim.x * blockDim.x) {
+        const uint32_t r = r0 + threadIdx.x;
+        const bool in = r < D;
+        const int32_t f = in ? filter[r] : 0;
+        unsigned long long m = base;
+        for (int i = 0; i < np; i++) {
+            const bool ok = in && f >= preds[i].lo && f < preds[i].hi;
+            if (ok) m |= 1ull << preds[i].q;
+            const unsigned long long b = __ballot(ok);
+            if ((threadIdx.x & 63) == 0 && b) atomicAdd(&pass[i], (uint32_t)__popcll(b));
+        }
+        if (in) mask[r] = m;
+    }
+}
+
+// ---------------------------------------------------------------- scan
+
+template <bool VEC>
+__device__ __forceinline__ v4i bload4(const int32_t *__restrict__ p, uint64_t r, uint64_t rows,
+                                      uint32_t &present)
+{
+    v4i v = {0, 0, 0, 0};
+    if (r + 4 <= rows) {
+        present = 0xFu;
+        v = __builtin_nontemporal_load((const v4i *)(p + r));
+    } else {
+        present = 0u;
+        if (r < rows) { v.x = p[r]; present |= 1u; }
+        if (r + 1 < rows) { v.y = p[r + 1]; present |= 2u; }
+        if (r + 2 < rows) { v.z = p[r + 2]; present |= 4u; }
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t sel4(uint32_t t, uint32_t a0, uint32_t a1, uint32_t a2,
+                                         uint32_t a3)
+{
+    return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
+}
+
+template <int BS, int U>
+__global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    // stage small tables' masks/attributes and the descriptors; zero tables
+    for (int t = 0; t < 4; t++) {
+        if (a.mask_lds[t] != kNoLds) {
+            const uint32_t *src = (const uint32_t *)a.gmask[t];
+            for (uint32_t i = threadIdx.x; i < 2u * a.dim_rows[t]; i += BS) lds[a.mask_lds[t] + i] = src[i];
+        }
+        if (a.attr_lds[t] != kNoLds) {
+            const uint32_t *src = (const uint32_t *)a.gattr[t];
+            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = src[i];
+        }
+    }
+    {
+        const uint32_t *src = (const uint32_t *)a.desc;
+        const uint32_t n = a.nq * (uint32_t)(sizeof(BQDesc) / 4);
+        for (uint32_t i = threadIdx.x; i < n; i += BS) lds[a.desc_lds + i] = src[i];
+    }
+    for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
+    __syncthreads();
+    const BQDesc *D = (const BQDesc *)(lds + a.desc_lds);
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t base = tile * TILE;
+        v4i f[U][4];
+        uint32_t present[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
+            present[u] = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                f[u][t] = v4i{0, 0, 0, 0};
+                if ((a.used >> t) & 1u) f[u][t] = bload4<true>(a.fk[t], r, rows, present[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
+            unsigned long long alive[4];
+            uint32_t at[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                alive[k] = ((present[u] >> k) & 1u) ? ~0ull : 0ull;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    at[t][k] = 0u;
+                    if (((a.used >> t) & 1u) && alive[k]) {
+                        const uint32_t key = (uint32_t)f[u][t][k];
+                        if (key < a.dim_rows[t]) {
+                            unsigned long long m;
+                            if (a.mask_lds[t] != kNoLds) {
+                                const uint32_t *w = lds + a.mask_lds[t] + 2u * key;
+                                m = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
+                            } else {
+                                m = a.gmask[t][key];
+                            }
+                            alive[k] &= m;
+                            at[t][k] = a.attr_lds[t] != kNoLds ? lds[a.attr_lds[t] + key]
+                                                               : (uint32_t)a.gattr[t][key];
+                        } else {
+                            alive[k] &= a.nouse[t];
+                        }
+                    }
+                }
+            }
+            if ((alive[0] | alive[1] | alive[2] | alive[3]) == 0ull) continue;
+            uint32_t dummy;
+            const v4i m1 = bload4<true>(a.m1, r, rows, dummy);
+            v4i m2 = {0, 0, 0, 0};
+            if (a.m2) m2 = bload4<true>(a.m2, r, rows, dummy);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                unsigned long long al = alive[k];
+                const unsigned long long mv =
+                    (unsigned long long)((long long)m1[k] + (a.m2 ? (long long)m2[k] : 0ll));
+                while (al) {
+                    const int q = __ffsll(al) - 1;
+                    al &= al - 1ull;
+                    // per-lane descriptor reads come from the (L1-resident)
+                    // global copy: an LDS-derived per-lane struct read here
+                    // trips a gfx950 code-generation error in this compiler
+                    const BQDesc &d = a.desc[q];
+                    uint32_t gid = 0;
+                    for (uint32_t s = 0; s < d.nsteps; s++) {
+                        const uint32_t av = sel4(d.tab[s], at[0][k], at[1][k], at[2][k], at[3][k]);
+                        const uint32_t code =
+                            (uint32_t)__umul64hi(d.M[s] * (uint64_t)av, (uint64_t)d.g[s]);
+                        gid += code * d.fac[s];
+                    }
+                    if (gid >= d.G) {
+                        __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        continue;
+                    }
+                    if (d.lds_tab != kNoLds) {
+                        b_atomic_lds(lds + d.lds_tab, d.G, gid, mv);
+                    } else {
+                        b_atomic_dev(&a.gsum[d.out_off + gid], mv);
+                        b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < a.nq; q++) {
+        const BQDesc &d = D[q];
+        if (d.lds_tab == kNoLds) continue;
+        const unsigned long long *ls = (const unsigned long long *)(lds + d.lds_tab);
+        const uint32_t *lc = lds + d.lds_tab + 2u * d.G;
+        for (uint32_t g = tid; g < d.G; g += BS) {
+            const uint32_t c = lc[g];
+            if (c) {
+                b_atomic_dev(&a.gcnt[d.out_off + g], (unsigned long long)c);
+                b_atomic_dev(&a.gsum[d.out_off + g], ls[g]);
+            }
+        }
+    }
+}
+
+// (threads, 128-bit chunks per thread per tile, LDS KiB for query tables)
+struct BatchCfg {
+    int bs;
+    int unroll;
+    int lds_kib;
+};
+const BatchCfg kBatchCfgs[] = {{256, 1, 0},  {256, 1, 48}, {256, 1, 96}, {256, 2, 48},
+                               {512, 1, 0},  {512, 1, 48}, {512, 1, 96}, {512, 2, 48},
+                               {1024, 1, 48}};
+const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
+
+template <typename K>
+static hipError_t batch_launch(K kernel, BatchArgs &a, int bs, uint32_t dyn, int num_cus,
+                               hipStream_t s, bool prepare_only, int *grid_out)
+{
+    if (prepare_only) {
+        if (dyn > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+            if (e != hipSuccess) return e;
+        }
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, dyn);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) return hipErrorInvalidConfiguration;
+        uint64_t grid = (uint64_t)per_cu * (uint64_t)num_cus;
+        if (grid > a.ntiles) grid = a.ntiles;
+        *grid_out = (int)std::max<uint64_t>(grid, 1);
+        return hipSuccess;
+    }
+    kernel<<<*grid_out, bs, dyn, s>>>(a);
+    return hipGetLastError();
+}
+
+static hipError_t batch_dispatch(int cfg, BatchArgs &a, uint32_t dyn, int num_cus, hipStream_t s,
+                                 bool prepare_only, int *grid)
+{
+    const BatchCfg c = kBatchCfgs[cfg];
+    const uint64_t tile = (uint64_t)c.bs * 4u * c.unroll;
+    a.ntiles = (a.rows + tile - 1) / tile;
+    if (a.ntiles == 0) return hipSuccess;
+    if (c.bs == 256 && c.unroll == 1) return batch_launch(k_batch<256, 1>, a, 256, dyn, num_cus, s, prepare_only, grid);
+    if (c.bs == 256) return batch_launch(k_batch<256, 2>, a, 256, dyn, num_cus, s, prepare_only, grid);
+    if (c.bs == 512 && c.unroll == 1) return batch_launch(k_batch<512, 1>, a, 512, dyn, num_cus, s, prepare_only, grid);
+    if (c.bs == 512) return batch_launch(k_batch<512, 2>, a, 512, dyn, num_cus, s, prepare_only, grid);
+    return batch_launch(k_batch<1024, 1>, a, 1024, dyn, num_cus, s, prepare_only, grid);
+}
+
+}  // namespace olap
+
+using namespace olap;
+
+struct olap_batch_s {
+    olap_ctx ctx = nullptr;
+    olap_star star{};
+    int nq = 0;
+    std::vector<olap_batch_query> queries;
+    std::vector<int64_t> offsets;
+    unsigned long long *dmask[4] = {nullptr, nullptr, nullptr, nullptr};
+    BQDesc *ddesc[kNumBatchCfgs] = {};
+    std::vector<double> sel;
+    double events = 0.0;
+    uint32_t used = 0;
+    unsigned long long nouse[4] = {0, 0, 0, 0};
+    int cfg = 5;
+    // per-config launch plan
+    BatchArgs args[kNumBatchCfgs];
+    uint32_t dyn[kNumBatchCfgs] = {};
+    int grid[kNumBatchCfgs] = {};
+    bool planned[kNumBatchCfgs] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+olap_status bfail(olap_ctx ctx, olap_status st, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return st;
+}
+
+olap_status bhip(olap_ctx ctx, hipError_t e, const char *what)
+{
+    return bfail(ctx, e == hipErrorOutOfMemory ? OLAP_E_NOMEM : OLAP_E_HIP,
+                 std::string(what) + ": " + hipGetErrorString(e));
+}
+
+uint32_t up16(uint32_t bytes) { return (bytes + 15u) & ~15u; }
+
+// LDS layout of one configuration: descriptors, then masks/attributes of the
+// smallest tables, then per-query group tables by events per byte.
+olap_status plan_config(olap_batch b, int c)
+{
+    olap_ctx ctx = b->ctx;
+    const BatchCfg cfg = kBatchCfgs[c];
+    BatchArgs &a = b->args[c];
+    memset(&a, 0, sizeof(a));
+    std::vector<BQDesc> desc((size_t)b->nq);
+    uint32_t cur = 0;
+    a.desc_lds = 0;
+    cur += up16((uint32_t)(sizeof(BQDesc) * b->nq));
+    const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
+    const uint32_t cap = kLdsBytes - 1024u;
+    // masks + attributes of probed tables, smallest first, within cap - table budget
+    std::vector<int> order;
+    for (int t = 0; t < 4; t++)
+        if ((b->used >> t) & 1u) order.push_back(t);
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return b->star.dim_rows[x] < b->star.dim_rows[y]; });
+    for (int t = 0; t < 4; t++) {
+        a.mask_lds[t] = kNoLds;
+        a.attr_lds[t] = kNoLds;
+    }
+    for (int t : order) {
+        const uint32_t need = up16(8u * (uint32_t)b->star.dim_rows[t]) + up16(4u * (uint32_t)b->star.dim_rows[t]);
+        if (cur + need + table_budget <= cap) {
+            a.mask_lds[t] = cur / 4u;
+            cur += up16(8u * (uint32_t)b->star.dim_rows[t]);
+            a.attr_lds[t] = cur / 4u;
+            cur += up16(4u * (uint32_t)b->star.dim_rows[t]);
+        }
+    }
+    a.tab_begin = cur / 4u;
+    // query tables: highest expected events per byte first
+    std::vector<int> qs((size_t)b->nq);
+    std::iota(qs.begin(), qs.end(), 0);
+    std::sort(qs.begin(), qs.end(), [&](int x, int y) {
+        return b->sel[x] / (double)b->queries[x].group_nums > b->sel[y] / (double)b->queries[y].group_nums;
+    });
+    uint32_t tab_used = 0;
+    for (int q : qs) desc[q].lds_tab = kNoLds;
+    for (int q : qs) {
+        const uint64_t need = up16((uint32_t)std::min<int64_t>(b->queries[q].group_nums * 12, 1u << 30));
+        if (b->queries[q].group_nums * 12 <= (int64_t)table_budget && tab_used + need <= table_budget) {
+            desc[q].lds_tab = (cur + tab_used) / 4u;
+            tab_used += (uint32_t)need;
+        }
+    }
+    cur += tab_used;
+    a.tab_end = cur / 4u;
+    for (int q = 0; q < b->nq; q++) {
+        const olap_batch_query &Q = b->queries[q];
+        BQDesc &d = desc[q];
+        d.nsteps = (uint32_t)Q.ndims;
+        d.G = (uint32_t)Q.group_nums;
+        d.out_off = (uint32_t)b->offsets[q];
+        for (int s = 0; s < 4; s++) {
+            const bool on = s < Q.ndims;
+            d.tab[s] = on ? (uint32_t)Q.table[s] : 0u;
+            d.fac[s] = on ? (uint32_t)Q.factor[s] : 0u;
+            d.g[s] = on ? (uint32_t)Q.group_mod[s] : 0u;
+            d.M[s] = d.g[s] ? (UINT64_MAX / d.g[s] + 1ull) : 0ull;
+        }
+    }
+    if (!b->ddesc[c]) {
+        hipError_t e = hipMalloc(&b->ddesc[c], sizeof(BQDesc) * std::max(1, b->nq));
+        if (e != hipSuccess) return bhip(ctx, e, "batch descriptors");
+    }
+    hipError_t e = hipMemcpy(b->ddesc[c], desc.data(), sizeof(BQDesc) * b->nq, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bhip(ctx, e, "batch descriptors");
+    for (int t = 0; t < 4; t++) {
+        a.fk[t] = b->star.fk[t];
+        a.gmask[t] = b->dmask[t];
+        a.gattr[t] = b->star.attr[t];
+        a.dim_rows[t] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
+        a.nouse[t] = b->nouse[t];
+    }
+    a.m1 = b->star.m1;
+    a.m2 = b->star.m2;
+    a.rows = (uint64_t)b->star.rows;
+    a.used = b->used;
+    a.nq = (uint32_t)b->nq;
+    a.desc = b->ddesc[c];
+    a.err = ctx->d_err;
+    b->dyn[c] = std::max<uint32_t>(cur, 16u);
+    e = batch_dispatch(c, a, b->dyn[c], ctx->num_cus, nullptr, true, &b->grid[c]);
+    if (e != hipSuccess) return bhip(ctx, e, "batch launch plan");
+    b->planned[c] = true;
+    return OLAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
+                              int32_t nq, olap_batch *out)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (!star || !queries || !out || nq < 1 || nq > OLAP_BATCH_MAX)
+        return bfail(ctx, OLAP_E_INVALID, "batch_create: 1..64 queries and a star are required");
+    *out = nullptr;
+    if (star->rows < 0 || star->rows > (int64_t)UINT32_MAX - 65536)
+        return bfail(ctx, OLAP_E_CAPACITY, "batch_create: rows out of range");
+    if (!star->m1 && star->rows) return bfail(ctx, OLAP_E_INVALID, "batch_create: m1 is required");
+    auto aligned = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    if (!aligned(star->m1) || !aligned(star->m2))
+        return bfail(ctx, OLAP_E_INVALID, "batch_create: fact columns must be 16-byte aligned");
+    olap_batch b = new olap_batch_s();
+    b->ctx = ctx;
+    b->star = *star;
+    b->nq = nq;
+    b->queries.assign(queries, queries + nq);
+    b->offsets.assign((size_t)nq + 1, 0);
+    for (int q = 0; q < nq; q++) {
+        const olap_batch_query &Q = queries[q];
+        std::string where = "query " + std::to_string(q) + ": ";
+        if (Q.ndims < 1 || Q.ndims > 4 || Q.group_nums < 1 || Q.group_nums > INT32_MAX) {
+            delete b;
+            return bfail(ctx, OLAP_E_INVALID, where + "ndims 1..4 and group_nums in [1, 2^31) required");
+        }
+        int64_t max_gid = 0;
+        uint32_t seen = 0;
+        for (int s = 0; s < Q.ndims; s++) {
+            const int t = Q.table[s];
+            if (t < 0 || t > 3 || ((seen >> t) & 1u) || Q.group_mod[s] < 0 || Q.factor[s] < 0) {
+                delete b;
+                return bfail(ctx, OLAP_E_INVALID, where + "distinct tables 0..3, group_mod >= 0, factor >= 0");
+            }
+            if (!star->filter[t] || !star->fk[t] || (Q.group_mod[s] > 0 && !star->attr[t]) ||
+                star->dim_rows[t] < 0 || star->dim_rows[t] > INT32_MAX || !aligned(star->fk[t])) {
+                delete b;
+                return bfail(ctx, OLAP_E_INVALID, where + "table " + std::to_string(t) +
+                                                      " lacks filter/attr/fk columns (fk 16-byte aligned)");
+            }
+            seen |= 1u << t;
+            if (Q.group_mod[s] > 1) max_gid += (int64_t)(Q.group_mod[s] - 1) * Q.factor[s];
+            b->used |= 1u << t;
+        }
+        if (max_gid >= Q.group_nums) {
+            delete b;
+            return bfail(ctx, OLAP_E_INVALID, where + "largest gid reaches group_nums");
+        }
+        b->offsets[q + 1] = b->offsets[q] + Q.group_nums;
+    }
+    if (b->offsets[nq] > INT32_MAX) {
+        delete b;
+        return bfail(ctx, OLAP_E_CAPACITY, "batch_create: more than 2^31 output slots in total");
+    }
+    for (int t = 0; t < 4; t++) {
+        b->nouse[t] = 0;
+        for (int q = 0; q < nq; q++) {
+            bool uses = false;
+            for (int s = 0; s < queries[q].ndims; s++) uses |= queries[q].table[s] == t;
+            if (!uses) b->nouse[t] |= 1ull << q;
+        }
+    }
+    // masks and their pass statistics
+    hipStream_t s0 = nullptr;
+    b->sel.assign((size_t)nq, 1.0);
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < 4 && e == hipSuccess; t++) {
+        if (!((b->used >> t) & 1u)) continue;
+        std::vector<MaskPred> preds;
+        std::vector<int> pq;
+        for (int q = 0; q < nq; q++)
+            for (int s = 0; s < queries[q].ndims; s++)
+                if (queries[q].table[s] == t) {
+                    preds.push_back({q, queries[q].lo[s], queries[q].hi[s], 0});
+                    pq.push_back(q);
+                }
+        const uint32_t D = (uint32_t)star->dim_rows[t];
+        MaskPred *dp = nullptr;
+        uint32_t *dpass = nullptr;
+        e = hipMalloc(&b->dmask[t], 8 * (size_t)std::max<uint32_t>(D, 1));
+        if (e == hipSuccess) e = hipMalloc(&dp, sizeof(MaskPred) * preds.size());
+        if (e == hipSuccess) e = hipMalloc(&dpass, 4 * preds.size());
+        if (e == hipSuccess) e = hipMemcpy(dp, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_zero_u32(dpass, preds.size(), s0);
+        if (e == hipSuccess && D) {
+            const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
+            k_batch_masks<<<grid, 256, 0, s0>>>(star->filter[t], D, dp, (int)preds.size(), b->nouse[t],
+                                                b->dmask[t], dpass);
+            e = hipGetLastError();
+        }
+        std::vector<uint32_t> pass(preds.size(), 0);
+        if (e == hipSuccess) e = hipMemcpy(pass.data(), dpass, 4 * preds.size(), hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < preds.size() && e == hipSuccess; i++)
+            b->sel[(size_t)pq[i]] *= D ? (double)pass[i] / (double)D : 0.0;
+        if (dp) (void)hipFree(dp);
+        if (dpass) (void)hipFree(dpass);
+    }
+    if (e != hipSuccess) {
+        olap_batch_destroy(b);
+        return bhip(ctx, e, "batch masks");
+    }
+    b->events = 0;
+    for (double v : b->sel) b->events += v;
+    olap_status st = plan_config(b, b->cfg);
+    if (st != OLAP_OK) {
+        olap_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return OLAP_OK;
+}
+
+olap_status olap_batch_offsets(olap_batch b, int64_t *offsets)
+{
+    if (!b || !offsets) return OLAP_E_INVALID;
+    std::copy(b->offsets.begin(), b->offsets.end(), offsets);
+    return OLAP_OK;
+}
+
+double olap_batch_expected_events(olap_batch b) { return b ? b->events : -1.0; }
+
+int32_t olap_batch_num_configs(void) { return kNumBatchCfgs; }
+
+olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
+                                   int32_t *lds_table_kib)
+{
+    if (cfg < 0 || cfg >= kNumBatchCfgs) return OLAP_E_INVALID;
+    if (threads) *threads = kBatchCfgs[cfg].bs;
+    if (rows_per_thread) *rows_per_thread = 4 * kBatchCfgs[cfg].unroll;
+    if (lds_table_kib) *lds_table_kib = kBatchCfgs[cfg].lds_kib;
+    return OLAP_OK;
+}
+
+olap_status olap_batch_set_config(olap_batch b, int32_t cfg)
+{
+    if (!b) return OLAP_E_INVALID;
+    if (cfg < 0 || cfg >= kNumBatchCfgs) return bfail(b->ctx, OLAP_E_INVALID, "config out of range");
+    if (!b->planned[cfg]) {
+        olap_status st = plan_config(b, cfg);
+        if (st != OLAP_OK) return st;
+    }
+    b->cfg = cfg;
+    return OLAP_OK;
+}
+
+olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *stream)
+{
+    if (!b) return OLAP_E_INVALID;
+    if (!sum || !count) return bfail(b->ctx, OLAP_E_INVALID, "batch_run: null output");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t slots = (size_t)b->offsets[b->nq];
+    hipError_t e = launch_zero_u64((unsigned long long *)sum, slots, s);
+    if (e == hipSuccess) e = launch_zero_u64((unsigned long long *)count, slots, s);
+    if (e != hipSuccess) return bhip(b->ctx, e, "batch zero");
+    BatchArgs a = b->args[b->cfg];
+    a.gsum = (unsigned long long *)sum;
+    a.gcnt = (unsigned long long *)count;
+    e = batch_dispatch(b->cfg, a, b->dyn[b->cfg], b->ctx->num_cus, s, false, &b->grid[b->cfg]);
+    if (e != hipSuccess) return bhip(b->ctx, e, "batch scan");
+    return OLAP_OK;
+}
+
+olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void *stream,
+                                int32_t *cfg_out, double *ms_out)
+{
+    if (!b) return OLAP_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return bfail(b->ctx, OLAP_E_HIP, "autotune events");
+    int best = b->cfg;
+    double best_ms = 1e30;
+    olap_status st = OLAP_OK;
+    for (int c = 0; c < kNumBatchCfgs && st == OLAP_OK; c++) {
+        st = olap_batch_set_config(b, c);
+        if (st != OLAP_OK) break;
+        st = olap_batch_run(b, sum, count, stream);
+        std::vector<double> ts;
+        for (int r = 0; r < 3 && st == OLAP_OK; r++) {
+            (void)hipEventRecord(e0, s);
+            st = olap_batch_run(b, sum, count, stream);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) st = bfail(b->ctx, OLAP_E_HIP, "autotune sync");
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ts.push_back(ms);
+        }
+        if (st != OLAP_OK) break;
+        std::sort(ts.begin(), ts.end());
+        if (ts[1] < best_ms) {
+            best_ms = ts[1];
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != OLAP_OK) return st;
+    b->cfg = best;
+    if (cfg_out) *cfg_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return olap_batch_run(b, sum, count, stream);
+}
+
+olap_status olap_batch_graph_capture(olap_batch b, int64_t *sum, int64_t *count, void *stream)
+{
+    if (!b) return OLAP_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return bfail(b->ctx, OLAP_E_INVALID, "graph capture needs a non-null stream");
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bhip(b->ctx, e, "begin capture");
+    olap_status st = olap_batch_run(b, sum, count, stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (st != OLAP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return bhip(b->ctx, e, "end capture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return bhip(b->ctx, e, "graph instantiate");
+    return OLAP_OK;
+}
+
+olap_status olap_batch_graph_launch(olap_batch b, void *stream)
+{
+    if (!b) return OLAP_E_INVALID;
+    if (!b->exec) return bfail(b->ctx, OLAP_E_INVALID, "no captured graph");
+    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return bhip(b->ctx, e, "graph launch");
+    return OLAP_OK;
+}
+
+olap_status olap_batch_destroy(olap_batch b)
+{
+    if (!b) return OLAP_E_INVALID;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int t = 0; t < 4; t++)
+        if (b->dmask[t]) (void)hipFree(b->dmask[t]);
+    for (int c = 0; c < kNumBatchCfgs; c++)
+        if (b->ddesc[c]) (void)hipFree(b->ddesc[c]);
+    delete b;
+    return OLAP_OK;
+}
+
+}  // extern "C"

@@ -311,6 +311,59 @@ olap_status one_shot(olap_ctx ctx, olap_strategy strat, const olap_plan *plan,
 
 }  // namespace
 
+namespace olap {
+
+olap_status agg_gid_vector(olap_ctx ctx, const int32_t *gvec, int64_t rows, const int32_t *m1,
+                           const int32_t *m2, int64_t G, int64_t *sum, int64_t *count,
+                           hipStream_t s)
+{
+    olap_query_s q;
+    q.ctx = ctx;
+    q.plan.group_nums = G;
+    q.cols.m1 = m1;
+    q.cols.m2 = m2;
+    q.cols.rows = rows;
+    olap_status st = zero_outputs(&q, sum, count, s);
+    if (st != OLAP_OK) return st;
+    return run_cwm_sv_aggregate(&q, gvec, sum, count, s);
+}
+
+olap_status agg_row_list(olap_ctx ctx, const uint32_t *oid, const int32_t *gid, int64_t n,
+                         const int32_t *m1, const int32_t *m2, int64_t G, int64_t *sum,
+                         int64_t *count, hipStream_t s)
+{
+    // lay the list out as full kTileV-entry segments; the last one is short
+    const int64_t tiles = (n + kTileV - 1) / kTileV;
+    std::vector<uint32_t> cnt((size_t)std::max<int64_t>(tiles, 1));
+    for (int64_t t = 0; t < tiles; t++)
+        cnt[(size_t)t] = (uint32_t)std::min<int64_t>(kTileV, n - t * kTileV);
+    uint32_t *dcnt = nullptr;
+    HIPC(ctx, hipMalloc(&dcnt, cnt.size() * 4));
+    hipError_t e = hipMemcpyAsync(dcnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, s);
+    olap_query_s q;
+    q.ctx = ctx;
+    q.plan.group_nums = G;
+    q.cols.m1 = m1;
+    q.cols.m2 = m2;
+    q.cols.rows = n;
+    q.list_oid = const_cast<uint32_t *>(oid);
+    q.list_gid = const_cast<int32_t *>(gid);
+    q.list_cnt = dcnt;
+    olap_status st = e == hipSuccess ? zero_outputs(&q, sum, count, s) : hip_fail(ctx, e, "agg_row_list");
+    if (st == OLAP_OK) st = run_cwm_dv_aggregate(&q, sum, count, s);
+    if (st == OLAP_OK) {
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_fail(ctx, e, "agg_row_list sync");
+    }
+    q.list_oid = nullptr;  // caller-owned
+    q.list_gid = nullptr;
+    q.list_cnt = nullptr;
+    dev_free(dcnt);
+    return st;
+}
+
+}  // namespace olap
+
 extern "C" {
 
 olap_status olap_ctx_create(int device, olap_ctx *out)

@@ -100,6 +100,16 @@ struct olap_query_s {
 
 namespace olap {
 
+// aggregation passes over caller-laid-out intermediates (olap_capi.cpp), used
+// by the reference-signature layer: a per-row gid vector (NULL -1), or a flat
+// (row, gid) list of n entries
+olap_status agg_gid_vector(olap_ctx ctx, const int32_t *gvec, int64_t rows, const int32_t *m1,
+                           const int32_t *m2, int64_t G, int64_t *sum, int64_t *count,
+                           hipStream_t s);
+olap_status agg_row_list(olap_ctx ctx, const uint32_t *oid, const int32_t *gid, int64_t n,
+                         const int32_t *m1, const int32_t *m2, int64_t G, int64_t *sum,
+                         int64_t *count, hipStream_t s);
+
 // launchers, olap_kernels.hip
 hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);

@@ -55,6 +55,43 @@ class Columns(C.Structure):
                 ("rows", C.c_int64)]
 
 
+class BatchQuery(C.Structure):
+    _fields_ = [("ndims", C.c_int32),
+                ("table", C.c_int32 * MAX_DIMS),
+                ("lo", C.c_int32 * MAX_DIMS),
+                ("hi", C.c_int32 * MAX_DIMS),
+                ("group_mod", C.c_int32 * MAX_DIMS),
+                ("factor", C.c_int32 * MAX_DIMS),
+                ("group_nums", C.c_int64)]
+
+    @classmethod
+    def make(cls, steps, factor, group_nums):
+        """steps: [(table, lo, hi, group_mod)] in probe order."""
+        q = cls()
+        q.ndims = len(steps)
+        for i, (t, lo, hi, g) in enumerate(steps):
+            q.table[i], q.lo[i], q.hi[i], q.group_mod[i] = t, lo, hi, g
+            q.factor[i] = factor[i]
+        q.group_nums = group_nums
+        return q
+
+    def as_dict(self):
+        n = self.ndims
+        return {"steps": [(self.table[i], self.lo[i], self.hi[i], self.group_mod[i])
+                          for i in range(n)],
+                "factor": list(self.factor)[:n], "group_nums": self.group_nums}
+
+
+class Star(C.Structure):
+    _fields_ = [("filter", C.c_void_p * MAX_DIMS),
+                ("attr", C.c_void_p * MAX_DIMS),
+                ("dim_rows", C.c_int64 * MAX_DIMS),
+                ("fk", C.c_void_p * MAX_DIMS),
+                ("m1", C.c_void_p),
+                ("m2", C.c_void_p),
+                ("rows", C.c_int64)]
+
+
 _P = C.c_void_p
 _I64P = C.POINTER(C.c_int64)
 
@@ -95,6 +132,18 @@ SIGNATURES = {
     "olap_agg_cwm_sv": (C.c_int, [_P, _P, _P, _P, _P]),
     "olap_join_cwm_dv": (C.c_int, [_P, _P]),
     "olap_agg_cwm_dv": (C.c_int, [_P, _P, _P, _P]),
+    "olap_batch_create": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(_P)]),
+    "olap_batch_offsets": (C.c_int, [_P, _P]),
+    "olap_batch_run": (C.c_int, [_P, _P, _P, _P]),
+    "olap_batch_num_configs": (C.c_int32, []),
+    "olap_batch_config_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]),
+    "olap_batch_set_config": (C.c_int, [_P, C.c_int32]),
+    "olap_batch_autotune": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "olap_batch_graph_capture": (C.c_int, [_P, _P, _P, _P]),
+    "olap_batch_graph_launch": (C.c_int, [_P, _P]),
+    "olap_batch_expected_events": (C.c_double, [_P]),
+    "olap_batch_destroy": (C.c_int, [_P]),
     "olap_legacy_count": (C.c_int32, [_P, C.c_int64]),
     "olap_starjoin": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, _I64P, _P]),
     "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
@@ -251,6 +300,28 @@ class Context:
     def query(self, plan: Plan, cols: Columns) -> "Query":
         return Query(self, plan, cols)
 
+    @staticmethod
+    def star(filters, attrs, fks, m1, m2=None, rows: Optional[int] = None) -> Star:
+        s = Star()
+        for t in range(MAX_DIMS):
+            if filters[t] is not None:
+                s.filter[t] = filters[t].data_ptr()
+                s.dim_rows[t] = filters[t].numel()
+            if attrs[t] is not None:
+                s.attr[t] = attrs[t].data_ptr()
+            if fks[t] is not None:
+                s.fk[t] = fks[t].data_ptr()
+                if rows is None:
+                    rows = fks[t].numel()
+        s.m1 = _ptr(m1)
+        s.m2 = _ptr(m2)
+        s.rows = rows if rows is not None else 0
+        s._keep = (list(filters), list(attrs), list(fks), m1, m2)
+        return s
+
+    def batch(self, star: Star, queries) -> "Batch":
+        return Batch(self, star, queries)
+
     def run(self, strategy: str, plan: Plan, cols: Columns, out_sum, out_cnt, stream=None):
         """One-shot reference-named entry point; returns the total count."""
         fn = getattr(self.lib, "olap_" + strategy)
@@ -339,6 +410,68 @@ class Query:
     def close(self):
         if self.h:
             self.ctx.lib.olap_query_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """olap_batch: up to 64 queries evaluated by one fused scan."""
+
+    def __init__(self, ctx: Context, star: Star, queries):
+        self.ctx = ctx
+        self.star = star
+        self.queries = list(queries)
+        arr = (BatchQuery * len(self.queries))(*self.queries)
+        self._arr = arr
+        h = _P()
+        self.h = None
+        ctx.check(ctx.lib.olap_batch_create(ctx.h, C.byref(star), arr, len(self.queries),
+                                            C.byref(h)), "batch_create")
+        self.h = h
+        import numpy as np
+        off = np.zeros(len(self.queries) + 1, dtype=np.int64)
+        ctx.check(ctx.lib.olap_batch_offsets(self.h, off.ctypes.data), "batch_offsets")
+        self.offsets = off
+
+    @property
+    def total_slots(self) -> int:
+        return int(self.offsets[-1])
+
+    def expected_events(self) -> float:
+        return float(self.ctx.lib.olap_batch_expected_events(self.h))
+
+    def run(self, out_sum, out_cnt, stream=None):
+        self.ctx.check(self.ctx.lib.olap_batch_run(self.h, _ptr(out_sum), _ptr(out_cnt), stream),
+                       "batch_run")
+
+    def set_config(self, cfg: int):
+        self.ctx.check(self.ctx.lib.olap_batch_set_config(self.h, cfg), "batch_set_config")
+
+    def autotune(self, out_sum, out_cnt, stream=None):
+        """Returns (config, threads, rows per thread, LDS KiB for tables, ms)."""
+        cfg = C.c_int32()
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.olap_batch_autotune(self.h, _ptr(out_sum), _ptr(out_cnt), stream,
+                                                        C.byref(cfg), C.byref(ms)), "batch_autotune")
+        th, rp, kib = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.lib.olap_batch_config_info(cfg.value, C.byref(th), C.byref(rp), C.byref(kib))
+        return cfg.value, th.value, rp.value, kib.value, ms.value
+
+    def capture(self, out_sum, out_cnt, stream):
+        self.ctx.check(self.ctx.lib.olap_batch_graph_capture(self.h, _ptr(out_sum), _ptr(out_cnt),
+                                                             stream), "batch_graph_capture")
+
+    def launch(self, stream):
+        self.ctx.check(self.ctx.lib.olap_batch_graph_launch(self.h, stream), "batch_graph_launch")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.olap_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -137,3 +137,98 @@ class DeviceStar:
 
     def fact_bytes(self) -> int:
         return self.cfg.rows * self.cfg.fact_bytes_per_row()
+
+
+# ---------------------------------------------------------------- batched form
+
+ATTR_HI = 1 << 20   # group attribute of every dimension: U[0, 2^20), stream 500+t
+
+
+def batch_queries(cfg: StarConfig, nq: int = 64, seed: int = 7, max_groups: int = 100_000):
+    """Seeded mixed star-join group-by queries over cfg's dimensions: each
+    draws 1-4 distinct dimensions, per dimension a predicate range lo <= filter
+    < hi on the U[0,100) filter column giving 1%-50% selectivity, and a total
+    group cardinality G uniform over [1, max_groups] split over its dimensions
+    as GROUP BY attr % g_i (G = prod g_i). Steps are ordered most selective
+    first. Returns [(steps [(table, lo, hi, g)], factor, G)]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    slots = [d.slot for d in cfg.dims]
+    out = []
+    for _ in range(nq):
+        k = int(rng.integers(1, min(4, len(slots)) + 1))
+        tables = [int(t) for t in rng.permutation(slots)[:k]]
+        widths = [int(rng.integers(1, 51)) for _ in range(k)]
+        target = int(rng.integers(1, max_groups + 1))
+        g = []
+        rest = target
+        for i in range(k):
+            if i == k - 1:
+                gi = max(1, rest)
+            else:
+                gi = max(1, int(round(target ** (1.0 / k))))
+                rest = max(1, rest // gi)
+            g.append(gi)
+        steps = []
+        for t, w, gi in zip(tables, widths, g):
+            lo = int(rng.integers(0, 101 - w))
+            steps.append((t, lo, lo + w, gi))
+        steps.sort(key=lambda s: s[2] - s[1])
+        gs = [s[3] for s in steps]
+        factor = [int(np.prod(gs[i + 1:], dtype=np.int64)) for i in range(k)]
+        G = int(np.prod(gs, dtype=np.int64))
+        out.append((steps, factor, G))
+    return out
+
+
+class DeviceBatchStar:
+    """The shared star of the batched form in HBM: cfg's fact table plus per
+    dimension a filter column U[0,100) and a group attribute U[0, 2^20)."""
+
+    def __init__(self, ctx, cfg: StarConfig, device: str = "cuda", stream=None, fact=None):
+        import torch
+
+        from . import olapcore as oc
+
+        self.cfg = cfg
+        self.ctx = ctx
+        i32 = torch.int32
+        self.filters = [None] * 4
+        self.attrs = [None] * 4
+        self.fks = [None] * 4
+        for d in cfg.dims:
+            f = torch.empty(d.rows, dtype=i32, device=device)
+            a = torch.empty(d.rows, dtype=i32, device=device)
+            ctx.gen_uniform(f, cfg.seed, 100 + d.slot, 0, 100, stream=stream)
+            ctx.gen_uniform(a, cfg.seed, 500 + d.slot, 0, ATTR_HI, stream=stream)
+            self.filters[d.slot], self.attrs[d.slot] = f, a
+            if fact is not None:
+                self.fks[d.slot] = fact.fks[d.slot]
+            else:
+                fk = torch.empty(cfg.rows, dtype=i32, device=device)
+                ctx.gen_uniform(fk, cfg.seed, 300 + d.slot, 0, d.rows, stream=stream)
+                self.fks[d.slot] = fk
+        if fact is not None:
+            self.m1 = fact.m1
+        else:
+            self.m1 = torch.empty(cfg.rows, dtype=i32, device=device)
+            ctx.gen_uniform(self.m1, cfg.seed, 400, cfg.m_lo, cfg.m_hi + 1, stream=stream)
+        self.star = oc.Context.star(self.filters, self.attrs, self.fks, self.m1, None, rows=cfg.rows)
+
+    def plain_query(self, steps, factor, G, stream=None):
+        """The same query as a plain star join: int32 dimension vectors built
+        with olap_build_dimvec, and its plan."""
+        import torch
+
+        from . import olapcore as oc
+
+        dvs = [None] * 4
+        for (t, lo, hi, g) in steps:
+            dv = torch.empty(self.filters[t].numel(), dtype=torch.int32, device=self.filters[t].device)
+            self.ctx.build_dimvec(self.filters[t], self.attrs[t], lo, hi, 1 if g > 0 else 0, dv,
+                                  group_mod=g, stream=stream)
+            dvs[t] = dv
+        plan = oc.plan_from([s[0] for s in steps], factor, G)
+        cols = oc.Context.columns(dvs, self.fks, self.m1, None, rows=self.cfg.rows)
+        return plan, cols

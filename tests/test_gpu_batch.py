@@ -1,0 +1,165 @@
+"""Batched form on the GPU: one fused scan for up to 64 queries, checked per
+query against the oracle (each query restated as a plain star join over int32
+dimension vectors), across autotuner configurations and hipGraph replays."""
+import numpy as np
+import pytest
+
+from memory_database_amd import olapcore as oc
+from memory_database_amd import workloads as wl
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def host_star(cfg, rows):
+    """Host copy of the batch star (same generator streams as the device)."""
+    filters, attrs, fks = [None] * 4, [None] * 4, [None] * 4
+    for d in cfg.dims:
+        filters[d.slot] = po.gen_uniform(d.rows, cfg.seed, 100 + d.slot, 0, 100)
+        attrs[d.slot] = po.gen_uniform(d.rows, cfg.seed, 500 + d.slot, 0, wl.ATTR_HI)
+        fks[d.slot] = po.gen_uniform(rows, cfg.seed, 300 + d.slot, 0, d.rows)
+    m1 = po.gen_uniform(rows, cfg.seed, 400, cfg.m_lo, cfg.m_hi + 1)
+    return filters, attrs, fks, m1
+
+
+def oracle_query(host, steps, factor, G, ncpu):
+    filters, attrs, fks, m1 = host
+    dvs = [None] * 4
+    for (t, lo, hi, g) in steps:
+        dvs[t] = po.build_dimvec(filters[t], attrs[t], lo, hi, 1 if g > 0 else 0, 4, group_mod=g)
+    p = po.plan([s[0] for s in steps], factor, G)
+    s, c, bad = po.run("rowwise", p, po.Star(dvs, fks, m1), nthreads=ncpu)
+    assert bad == 0
+    return s, c
+
+
+def make_batch(ctx, rows, nq, seed, max_groups=100_000):
+    cfg = wl.scaled(wl.C3, rows, name=f"batch_{rows}")
+    ds = wl.DeviceBatchStar(ctx, cfg)
+    qs = wl.batch_queries(cfg, nq, seed, max_groups=max_groups)
+    bq = [oc.BatchQuery.make(*q) for q in qs]
+    return cfg, ds, qs, ctx.batch(ds.star, bq)
+
+
+def check_outputs(b, qs, host, osum, ocnt, ncpu, label):
+    s = osum.cpu().numpy()
+    c = ocnt.cpu().numpy()
+    for i, (steps, factor, G) in enumerate(qs):
+        es, ec = oracle_query(host, steps, factor, G, ncpu)
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        np.testing.assert_array_equal(c[lo:hi], ec, err_msg=f"{label} query {i} counts")
+        np.testing.assert_array_equal(s[lo:hi], es, err_msg=f"{label} query {i} sums")
+
+
+def test_batch_matches_oracle_every_config(ctx, ncpu):
+    import torch
+
+    rows = 1_000_003
+    _say(f"batch of 64 queries over {rows} rows")
+    cfg, ds, qs, b = make_batch(ctx, rows, 64, seed=11)
+    host = host_star(cfg, rows)
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ref = None
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        _say(f"  config {k}")
+        b.set_config(k)
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        got = (osum.cpu().numpy(), ocnt.cpu().numpy())
+        if ref is None:
+            check_outputs(b, qs, host, osum, ocnt, ncpu, f"config {k}")
+            ref = got
+        else:
+            np.testing.assert_array_equal(got[0], ref[0], err_msg=f"config {k} sums")
+            np.testing.assert_array_equal(got[1], ref[1], err_msg=f"config {k} counts")
+    cfgk, th, rp, kib, ms = b.autotune(osum, ocnt)
+    _say(f"  autotune: config {cfgk} ({th} threads, {rp} rows/thread, {kib} KiB tables) {ms:.3f} ms")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(osum.cpu().numpy(), ref[0])
+    b.close()
+
+
+def test_batch_small_groups_and_ragged_rows(ctx, ncpu):
+    """Small group counts (LDS tables for every query), a ragged row count and
+    a batch of one."""
+    import torch
+
+    for rows, nq, maxg in [(8191, 17, 50), (5, 1, 3), (300_001, 64, 2000)]:
+        _say(f"batch rows={rows} nq={nq} max_groups={maxg}")
+        cfg, ds, qs, b = make_batch(ctx, rows, nq, seed=rows, max_groups=maxg)
+        host = host_star(cfg, rows)
+        osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+        ocnt = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        check_outputs(b, qs, host, osum, ocnt, ncpu, f"rows={rows}")
+        b.close()
+
+
+def test_batch_equals_plain_queries(ctx):
+    """Each query of the batch equals the plain star join of the same query."""
+    import torch
+
+    rows = 2_000_000
+    cfg, ds, qs, b = make_batch(ctx, rows, 24, seed=5)
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    b.run(osum, ocnt)
+    torch.cuda.synchronize()
+    for i, (steps, factor, G) in enumerate(qs):
+        plan, cols = ds.plain_query(steps, factor, G)
+        ps = torch.empty(G, dtype=torch.int64, device="cuda")
+        pc = torch.empty(G, dtype=torch.int64, device="cuda")
+        q = ctx.query(plan, cols)
+        q.run("rowwise", ps, pc)
+        torch.cuda.synchronize()
+        lo, hi = b.offsets[i], b.offsets[i + 1]
+        assert torch.equal(ps, osum[lo:hi]) and torch.equal(pc, ocnt[lo:hi]), f"query {i}"
+        q.close()
+    b.close()
+
+
+def test_batch_graph_replays(ctx):
+    """A captured graph replayed several times, with eager work and
+    synchronisations between replays: every replay equals the eager run."""
+    import torch
+
+    rows = 3_000_000
+    cfg, ds, qs, b = make_batch(ctx, rows, 64, seed=9)
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    b.run(osum, ocnt)
+    torch.cuda.synchronize()
+    es, ec = osum.clone(), ocnt.clone()
+    stream = torch.cuda.Stream()
+    b.capture(osum, ocnt, stream.cuda_stream)
+    for rep in range(4):
+        osum.fill_(-1)
+        ocnt.fill_(-1)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        b.launch(stream.cuda_stream)
+        torch.cuda.synchronize()
+        assert torch.equal(osum, es) and torch.equal(ocnt, ec), f"replay {rep}"
+        del junk
+    b.close()
+
+
+def test_batch_rejects_bad_queries(ctx):
+    import torch
+
+    cfg, ds, qs, b = make_batch(ctx, 1000, 2, seed=1)
+    b.close()
+    bad = oc.BatchQuery.make([(0, 0, 10, 5), (0, 0, 10, 5)], [5, 1], 25)  # table twice
+    with pytest.raises(oc.OlapError):
+        ctx.batch(ds.star, [bad])
+    bad = oc.BatchQuery.make([(0, 0, 10, 5)], [1], 4)  # gid 4 reaches group_nums
+    with pytest.raises(oc.OlapError):
+        ctx.batch(ds.star, [bad])
+    with pytest.raises(oc.OlapError):
+        ctx.batch(ds.star, [oc.BatchQuery.make([(1, 0, 10, 2)], [1], 2)] * 65)
