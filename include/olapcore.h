@@ -142,6 +142,12 @@ int32_t olap_num_shapes(olap_strategy strategy);
 olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *threads,
                             int32_t *rows_per_thread);
 olap_status olap_query_set_shape(olap_query q, int32_t shape);
+/* Aggregation layout of row-wise runs whose group table exceeds LDS:
+ * -1 automatic (from the expected matches), 0 one global atomic pair per
+ * matching row, 1 staged (count, scatter records to HBM, reduce per
+ * 4096-slot partition in LDS; synchronises once per run to size the record
+ * buffer). The autotuner also tries both. */
+olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out);
 /* bytes of device memory the query holds (packed vectors, scratch) */
@@ -225,11 +231,14 @@ olap_status olap_batch_offsets(olap_batch b, int64_t *offsets);
  * be captured into a hipGraph. */
 olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *stream);
 /* Autotuner of the batched scan: times every (threads, rows per thread, LDS
- * budget for per-query group tables) configuration on this batch, keeps the
- * fastest; outputs then hold a run of the winner. Synchronises. */
+ * budget for per-query group tables, aggregation of the other queries by
+ * global atomics or staged) configuration on this batch, keeps the fastest;
+ * outputs then hold a run of the winner. Synchronises. A staged run reads
+ * its record total back once per olap_batch_run (one synchronisation); a
+ * captured graph reuses the buffers sized by the run made before capture. */
 int32_t olap_batch_num_configs(void);
 olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
-                                   int32_t *lds_table_kib);
+                                   int32_t *lds_table_kib, int32_t *staged);
 olap_status olap_batch_set_config(olap_batch b, int32_t cfg);
 olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void *stream,
                                 int32_t *cfg_out, double *ms_out);

@@ -19,7 +19,8 @@ DRIVER_PATH = os.path.join(HERE, "olapcore_test")
 
 ARCH = os.environ.get("OLAP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["olap_kernels.hip", "olap_batch.hip", "olap_capi.cpp", "olapcore_compat.cpp",
+SOURCES = ["olap_kernels.hip", "olap_stage.hip", "olap_batch.hip", "olap_capi.cpp",
+           "olapcore_compat.cpp",
            "olap_server.cpp"]
 
 HIP_FLAGS = [
@@ -48,13 +49,24 @@ def _run(cmd, verbose):
 
 
 def build_library(verbose: bool = False) -> str:
-    """Compile every source into one shared library; returns its path."""
+    """Compile every source (in parallel) and link one shared library; returns
+    its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = _hipcc()
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as pool:
+        futs = [pool.submit(_run, [hipcc, *HIP_FLAGS, *inc, "-c", "-o", o, s], verbose)
+                for s, o in zip(srcs, objs)]
+        for f in futs:
+            f.result()
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc, *HIP_FLAGS, *inc, "-shared", "-o", tmp, *srcs, "-lpthread"]
-    _run(cmd, verbose)
+    _run([hipcc, *HIP_FLAGS, "-shared", "-o", tmp, *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB_PATH)
     drv = os.path.join(CSRC, "olapcore_driver.cpp")
     if os.path.exists(drv):

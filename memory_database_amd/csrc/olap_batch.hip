@@ -27,18 +27,21 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kNoLds = 0xffffffffu;
 
-// one query as the batched kernel reads it (96 bytes, kept in LDS)
+// one query as the batched kernel reads it (112 bytes)
 struct BQDesc {
     uint32_t nsteps;
-    uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when global
+    uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when in HBM
     uint32_t G;
     uint32_t out_off;   // first slot in the concatenated outputs
     uint32_t tab[4];
     uint32_t fac[4];
     uint32_t g[4];      // group_mod (0: code 0)
     uint64_t M[4];      // fast-modulo multiplier: UINT64_MAX / g + 1
+    uint32_t pbase;     // first staged-aggregation partition of the query
+    uint32_t pad[3];
 };
-static_assert(sizeof(BQDesc) == 96, "descriptor layout");
+static_assert(sizeof(BQDesc) == 112, "descriptor layout");
+constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;  // word 16: M[], word 24: pbase
 
 struct BatchArgs {
     const int32_t *fk[4];
@@ -48,6 +51,7 @@ struct BatchArgs {
     uint64_t ntiles;
     const unsigned long long *gmask[4];
     const int32_t *gattr[4];
+    const uint4 *grec[4];   // per dimension row: mask (2 words), attribute, 0
     uint32_t dim_rows[4];
     uint32_t mask_lds[4];   // word offset of the LDS copy or kNoLds
     uint32_t attr_lds[4];
@@ -61,6 +65,7 @@ struct BatchArgs {
     unsigned long long *gsum;
     unsigned long long *gcnt;
     uint32_t *err;
+    StageArgs stage;
 };
 
 struct MaskPred {
@@ -92,8 +97,9 @@ __device__ __forceinline__ void b_atomic_lds(uint32_t *tab, uint32_t G, uint32_t
 
 // ---------------------------------------------------------------- masks
 
-__global__ void k_batch_masks(const int32_t *filter, uint32_t D, const MaskPred *preds, int np,
-                              unsigned long long base, unsigned long long *mask, uint32_t *pass)
+__global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32_t D,
+                              const MaskPred *preds, int np, unsigned long long base,
+                              unsigned long long *mask, uint4 *rec, uint32_t *pass)
 {
     for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < D; r0 += gridDim.x * blockDim.x) {
         const uint32_t r = r0 + threadIdx.x;
@@ -106,7 +112,10 @@ __global__ void k_batch_masks(const int32_t *filter, uint32_t D, const MaskPred 
             const unsigned long long b = __ballot(ok);
             if ((threadIdx.x & 63) == 0 && b) atomicAdd(&pass[i], (uint32_t)__popcll(b));
         }
-        if (in) mask[r] = m;
+        if (in) {
+            mask[r] = m;
+            rec[r] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), attr ? (uint32_t)attr[r] : 0u, 0u);
+        }
     }
 }
 
@@ -135,10 +144,14 @@ __device__ __forceinline__ uint32_t sel4(uint32_t t, uint32_t a0, uint32_t a1, u
     return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
 }
 
-template <int BS, int U>
+// MODE AGG_DIRECT: queries without an LDS table use global atomics.
+// AGG_COUNT / AGG_SCATTER: the two passes of the staged aggregation; queries
+// with an LDS table are aggregated in the COUNT pass only.
+template <int BS, int U, int MODE>
 __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
 {
     extern __shared__ __align__(16) uint32_t lds[];
+    stage_begin<MODE>(a.stage, lds);
     // stage small tables' masks/attributes and the descriptors; zero tables
     for (int t = 0; t < 4; t++) {
         if (a.mask_lds[t] != kNoLds) {
@@ -152,12 +165,13 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
     }
     {
         const uint32_t *src = (const uint32_t *)a.desc;
-        const uint32_t n = a.nq * (uint32_t)(sizeof(BQDesc) / 4);
+        const uint32_t n = a.nq * kDescWords;
         for (uint32_t i = threadIdx.x; i < n; i += BS) lds[a.desc_lds + i] = src[i];
     }
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
     __syncthreads();
-    const BQDesc *D = (const BQDesc *)(lds + a.desc_lds);
+    // the uniform flush loop reads descriptors from the global copy
+    const BQDesc *D = a.desc;
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
     constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
@@ -189,16 +203,16 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
                     if (((a.used >> t) & 1u) && alive[k]) {
                         const uint32_t key = (uint32_t)f[u][t][k];
                         if (key < a.dim_rows[t]) {
-                            unsigned long long m;
                             if (a.mask_lds[t] != kNoLds) {
-                                const uint32_t *w = lds + a.mask_lds[t] + 2u * key;
-                                m = (unsigned long long)w[0] | ((unsigned long long)w[1] << 32);
+                                const l_u64 *w = (const l_u64 *)((const l_u32 *)lds + a.mask_lds[t]);
+                                alive[k] &= w[key];
+                                at[t][k] = ((const l_u32 *)lds)[a.attr_lds[t] + key];
                             } else {
-                                m = a.gmask[t][key];
+                                // one 16-byte gather: mask (2 words), attribute
+                                const uint4 rec = a.grec[t][key];
+                                alive[k] &= (unsigned long long)rec.x | ((unsigned long long)rec.y << 32);
+                                at[t][k] = rec.z;
                             }
-                            alive[k] &= m;
-                            at[t][k] = a.attr_lds[t] != kNoLds ? lds[a.attr_lds[t] + key]
-                                                               : (uint32_t)a.gattr[t][key];
                         } else {
                             alive[k] &= a.nouse[t];
                         }
@@ -218,37 +232,44 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
                 while (al) {
                     const int q = __ffsll(al) - 1;
                     al &= al - 1ull;
-                    // per-lane descriptor reads come from the (L1-resident)
-                    // global copy: an LDS-derived per-lane struct read here
-                    // trips a gfx950 code-generation error in this compiler
-                    const BQDesc &d = a.desc[q];
+                    // descriptor words of query q (BQDesc layout) read from the
+                    // LDS copy through an explicit address-space-3 pointer
+                    const l_u32 *dw = (const l_u32 *)lds + a.desc_lds + kDescWords * (uint32_t)q;
+                    const uint32_t nsteps = dw[0], lds_tab = dw[1], G = dw[2];
                     uint32_t gid = 0;
-                    for (uint32_t s = 0; s < d.nsteps; s++) {
-                        const uint32_t av = sel4(d.tab[s], at[0][k], at[1][k], at[2][k], at[3][k]);
-                        const uint32_t code =
-                            (uint32_t)__umul64hi(d.M[s] * (uint64_t)av, (uint64_t)d.g[s]);
-                        gid += code * d.fac[s];
+                    for (uint32_t s = 0; s < nsteps; s++) {
+                        const uint32_t av = sel4(dw[4 + s], at[0][k], at[1][k], at[2][k], at[3][k]);
+                        const uint64_t M = ((const l_u64 *)(dw + 16))[s];
+                        const uint32_t code = (uint32_t)__umul64hi(M * (uint64_t)av, (uint64_t)dw[12 + s]);
+                        gid += code * dw[8 + s];
                     }
-                    if (gid >= d.G) {
+                    if (gid >= G) {
                         __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         continue;
                     }
-                    if (d.lds_tab != kNoLds) {
-                        b_atomic_lds(lds + d.lds_tab, d.G, gid, mv);
+                    if (lds_tab != kNoLds) {
+                        if (MODE != AGG_SCATTER) b_atomic_lds(lds + lds_tab, G, gid, mv);
+                    } else if (MODE == AGG_DIRECT) {
+                        b_atomic_dev(&a.gsum[dw[3] + gid], mv);
+                        b_atomic_dev(&a.gcnt[dw[3] + gid], 1ull);
+                    } else if (MODE == AGG_COUNT) {
+                        stage_count(a.stage, lds, dw[24] + (gid >> kPartBits));
                     } else {
-                        b_atomic_dev(&a.gsum[d.out_off + gid], mv);
-                        b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+                        stage_scatter(a.stage, lds, dw[24] + (gid >> kPartBits),
+                                      gid & (kPartSlots - 1u), m1[k], a.m2 ? m2[k] : 0);
                     }
                 }
             }
         }
     }
     __syncthreads();
+    if (MODE == AGG_COUNT) stage_end<MODE>(a.stage, lds);
+    if (MODE == AGG_SCATTER) return;
     for (uint32_t q = 0; q < a.nq; q++) {
         const BQDesc &d = D[q];
         if (d.lds_tab == kNoLds) continue;
-        const unsigned long long *ls = (const unsigned long long *)(lds + d.lds_tab);
-        const uint32_t *lc = lds + d.lds_tab + 2u * d.G;
+        const l_u64 *ls = (const l_u64 *)((l_u32 *)lds + d.lds_tab);
+        const l_u32 *lc = (l_u32 *)lds + d.lds_tab + 2u * d.G;
         for (uint32_t g = tid; g < d.G; g += BS) {
             const uint32_t c = lc[g];
             if (c) {
@@ -259,52 +280,70 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
     }
 }
 
-// (threads, 128-bit chunks per thread per tile, LDS KiB for query tables)
+// (threads, 128-bit chunks per thread per tile, LDS KiB for query tables,
+// staged aggregation of the queries without an LDS table)
 struct BatchCfg {
     int bs;
     int unroll;
     int lds_kib;
+    int staged;
 };
-const BatchCfg kBatchCfgs[] = {{256, 1, 0},  {256, 1, 48}, {256, 1, 96}, {256, 2, 48},
-                               {512, 1, 0},  {512, 1, 48}, {512, 1, 96}, {512, 2, 48},
-                               {1024, 1, 48}};
+const BatchCfg kBatchCfgs[] = {{512, 1, 48, 0},  {512, 1, 96, 0},  {256, 1, 48, 0},
+                               {256, 1, 48, 1},  {256, 1, 96, 1},  {512, 1, 0, 1},
+                               {512, 1, 48, 1},  {512, 1, 96, 1},  {512, 2, 48, 1},
+                               {1024, 1, 48, 1}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
-template <typename K>
-static hipError_t batch_launch(K kernel, BatchArgs &a, int bs, uint32_t dyn, int num_cus,
-                               hipStream_t s, bool prepare_only, int *grid_out)
+template <int MODE>
+static const void *batch_kernel(int bs, int unroll)
 {
-    if (prepare_only) {
+    if (bs == 256) return unroll == 1 ? (const void *)k_batch<256, 1, MODE> : (const void *)k_batch<256, 2, MODE>;
+    if (bs == 512) return unroll == 1 ? (const void *)k_batch<512, 1, MODE> : (const void *)k_batch<512, 2, MODE>;
+    return (const void *)k_batch<1024, 1, MODE>;
+}
+
+static const void *batch_kernel_mode(int mode, int bs, int unroll)
+{
+    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(bs, unroll);
+    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(bs, unroll);
+    return batch_kernel<AGG_DIRECT>(bs, unroll);
+}
+
+// grid of a configuration: persistent, limited by the least occupant of the
+// kernels it runs (COUNT and SCATTER of a staged run share one grid)
+static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, int *grid)
+{
+    const BatchCfg c = kBatchCfgs[cfg];
+    const int modes[2] = {c.staged ? AGG_COUNT : AGG_DIRECT, c.staged ? AGG_SCATTER : AGG_DIRECT};
+    int best = 1 << 30;
+    for (int m : modes) {
+        const void *k = batch_kernel_mode(m, c.bs, c.unroll);
         if (dyn > 64 * 1024) {
-            hipError_t e = hipFuncSetAttribute((const void *)kernel,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
             if (e != hipSuccess) return e;
         }
         int per_cu = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, dyn);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, c.bs, dyn);
         if (e != hipSuccess) return e;
-        if (per_cu < 1) return hipErrorInvalidConfiguration;
-        uint64_t grid = (uint64_t)per_cu * (uint64_t)num_cus;
-        if (grid > a.ntiles) grid = a.ntiles;
-        *grid_out = (int)std::max<uint64_t>(grid, 1);
-        return hipSuccess;
+        best = std::min(best, per_cu);
     }
-    kernel<<<*grid_out, bs, dyn, s>>>(a);
-    return hipGetLastError();
+    if (best < 1) return hipErrorInvalidConfiguration;
+    const uint64_t tile = (uint64_t)c.bs * 4u * c.unroll;
+    const uint64_t tiles = (rows + tile - 1) / tile;
+    *grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best * num_cus, tiles));
+    return hipSuccess;
 }
 
-static hipError_t batch_dispatch(int cfg, BatchArgs &a, uint32_t dyn, int num_cus, hipStream_t s,
-                                 bool prepare_only, int *grid)
+static hipError_t batch_launch(int cfg, int mode, BatchArgs &a, uint32_t dyn, int grid,
+                               hipStream_t s)
 {
     const BatchCfg c = kBatchCfgs[cfg];
     const uint64_t tile = (uint64_t)c.bs * 4u * c.unroll;
     a.ntiles = (a.rows + tile - 1) / tile;
     if (a.ntiles == 0) return hipSuccess;
-    if (c.bs == 256 && c.unroll == 1) return batch_launch(k_batch<256, 1>, a, 256, dyn, num_cus, s, prepare_only, grid);
-    if (c.bs == 256) return batch_launch(k_batch<256, 2>, a, 256, dyn, num_cus, s, prepare_only, grid);
-    if (c.bs == 512 && c.unroll == 1) return batch_launch(k_batch<512, 1>, a, 512, dyn, num_cus, s, prepare_only, grid);
-    if (c.bs == 512) return batch_launch(k_batch<512, 2>, a, 512, dyn, num_cus, s, prepare_only, grid);
-    return batch_launch(k_batch<1024, 1>, a, 1024, dyn, num_cus, s, prepare_only, grid);
+    void *args[] = {&a};
+    return hipLaunchKernel(batch_kernel_mode(mode, c.bs, c.unroll), dim3(grid), dim3(c.bs), args,
+                           dyn, s);
 }
 
 }  // namespace olap
@@ -318,17 +357,25 @@ struct olap_batch_s {
     std::vector<olap_batch_query> queries;
     std::vector<int64_t> offsets;
     unsigned long long *dmask[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint4 *drec[4] = {nullptr, nullptr, nullptr, nullptr};
     BQDesc *ddesc[kNumBatchCfgs] = {};
     std::vector<double> sel;
     double events = 0.0;
     uint32_t used = 0;
     unsigned long long nouse[4] = {0, 0, 0, 0};
-    int cfg = 5;
+    int cfg = 6;
     // per-config launch plan
     BatchArgs args[kNumBatchCfgs];
     uint32_t dyn[kNumBatchCfgs] = {};
     int grid[kNumBatchCfgs] = {};
+    uint32_t stage_lds[kNumBatchCfgs] = {};  // word offset of the stage region
     bool planned[kNumBatchCfgs] = {};
+    // staged aggregation scratch, set up for one configuration's grid
+    Stager stager;
+    int stager_cfg = -1;
+    std::vector<unsigned long long> out_off;
+    std::vector<uint32_t> gsz;
+    uint32_t nparts = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -362,7 +409,9 @@ olap_status plan_config(olap_batch b, int c)
     a.desc_lds = 0;
     cur += up16((uint32_t)(sizeof(BQDesc) * b->nq));
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
-    const uint32_t cap = kLdsBytes - 1024u;
+    // staged configurations keep a u32 counter and a u64 range start per partition
+    const uint32_t stage_bytes = cfg.staged ? up16(b->nparts * 4u) + b->nparts * 8u : 0u;
+    const uint32_t cap = kLdsBytes - 1024u - stage_bytes;
     // masks + attributes of probed tables, smallest first, within cap - table budget
     std::vector<int> order;
     for (int t = 0; t < 4; t++)
@@ -400,12 +449,17 @@ olap_status plan_config(olap_batch b, int c)
     }
     cur += tab_used;
     a.tab_end = cur / 4u;
+    b->stage_lds[c] = cur / 4u;
+    cur += stage_bytes;
+    uint32_t pbase = 0;
     for (int q = 0; q < b->nq; q++) {
         const olap_batch_query &Q = b->queries[q];
         BQDesc &d = desc[q];
         d.nsteps = (uint32_t)Q.ndims;
         d.G = (uint32_t)Q.group_nums;
         d.out_off = (uint32_t)b->offsets[q];
+        d.pbase = pbase;
+        pbase += (d.G + kPartSlots - 1) / kPartSlots;
         for (int s = 0; s < 4; s++) {
             const bool on = s < Q.ndims;
             d.tab[s] = on ? (uint32_t)Q.table[s] : 0u;
@@ -423,6 +477,7 @@ olap_status plan_config(olap_batch b, int c)
     for (int t = 0; t < 4; t++) {
         a.fk[t] = b->star.fk[t];
         a.gmask[t] = b->dmask[t];
+        a.grec[t] = b->drec[t];
         a.gattr[t] = b->star.attr[t];
         a.dim_rows[t] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
         a.nouse[t] = b->nouse[t];
@@ -435,10 +490,60 @@ olap_status plan_config(olap_batch b, int c)
     a.desc = b->ddesc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);
-    e = batch_dispatch(c, a, b->dyn[c], ctx->num_cus, nullptr, true, &b->grid[c]);
+    e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, &b->grid[c]);
     if (e != hipSuccess) return bhip(ctx, e, "batch launch plan");
     b->planned[c] = true;
     return OLAP_OK;
+}
+
+olap_status stager_for(olap_batch b, int c)
+{
+    if (b->stager_cfg == c) return OLAP_OK;
+    hipError_t e = b->stager.setup(b->out_off.data(), b->gsz.data(), b->nq, (uint32_t)b->grid[c],
+                                   b->star.m2 != nullptr);
+    if (e != hipSuccess) return bhip(b->ctx, e, "stager setup");
+    b->stager_cfg = c;
+    return OLAP_OK;
+}
+
+// zero + scan(s) + reduce of the current configuration; `sized` skips the
+// host read of the record total (graph capture: the buffers were sized by an
+// earlier run of the same batch)
+olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s, bool sized)
+{
+    olap_ctx ctx = b->ctx;
+    const int c = b->cfg;
+    const size_t slots = (size_t)b->offsets[b->nq];
+    hipError_t e = launch_zero_u64((unsigned long long *)sum, slots, s);
+    if (e == hipSuccess) e = launch_zero_u64((unsigned long long *)count, slots, s);
+    if (e != hipSuccess) return bhip(ctx, e, "batch zero");
+    BatchArgs a = b->args[c];
+    a.gsum = (unsigned long long *)sum;
+    a.gcnt = (unsigned long long *)count;
+    if (!kBatchCfgs[c].staged) {
+        e = batch_launch(c, AGG_DIRECT, a, b->dyn[c], b->grid[c], s);
+        return e == hipSuccess ? OLAP_OK : bhip(ctx, e, "batch scan");
+    }
+    olap_status st = stager_for(b, c);
+    if (st != OLAP_OK) return st;
+    Stager &S = b->stager;
+    const uint32_t cur = b->stage_lds[c];
+    const uint32_t start = cur + up16(b->nparts * 4u) / 4u;
+    a.stage = S.args(cur, start);
+    e = batch_launch(c, AGG_COUNT, a, b->dyn[c], b->grid[c], s);
+    if (e == hipSuccess) e = S.prefix(s);
+    if (e != hipSuccess) return bhip(ctx, e, "batch count pass");
+    if (!sized || !S.rec) {
+        unsigned long long total = 0;
+        e = S.total(&total, s);
+        if (e == hipSuccess) e = S.reserve(total);
+        if (e != hipSuccess) return bhip(ctx, e, "batch staging buffers");
+        a.stage = S.args(cur, start);
+    }
+    e = batch_launch(c, AGG_SCATTER, a, b->dyn[c], b->grid[c], s);
+    if (e == hipSuccess) e = S.reduce((unsigned long long *)sum, (unsigned long long *)count,
+                                      ctx->num_cus, s);
+    return e == hipSuccess ? OLAP_OK : bhip(ctx, e, "batch scatter/reduce");
 }
 
 }  // namespace
@@ -525,14 +630,15 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
         MaskPred *dp = nullptr;
         uint32_t *dpass = nullptr;
         e = hipMalloc(&b->dmask[t], 8 * (size_t)std::max<uint32_t>(D, 1));
+        if (e == hipSuccess) e = hipMalloc(&b->drec[t], 16 * (size_t)std::max<uint32_t>(D, 1));
         if (e == hipSuccess) e = hipMalloc(&dp, sizeof(MaskPred) * preds.size());
         if (e == hipSuccess) e = hipMalloc(&dpass, 4 * preds.size());
         if (e == hipSuccess) e = hipMemcpy(dp, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = launch_zero_u32(dpass, preds.size(), s0);
         if (e == hipSuccess && D) {
             const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
-            k_batch_masks<<<grid, 256, 0, s0>>>(star->filter[t], D, dp, (int)preds.size(), b->nouse[t],
-                                                b->dmask[t], dpass);
+            k_batch_masks<<<grid, 256, 0, s0>>>(star->filter[t], star->attr[t], D, dp, (int)preds.size(), b->nouse[t],
+                                                b->dmask[t], b->drec[t], dpass);
             e = hipGetLastError();
         }
         std::vector<uint32_t> pass(preds.size(), 0);
@@ -548,6 +654,15 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
     }
     b->events = 0;
     for (double v : b->sel) b->events += v;
+    for (int q = 0; q < nq; q++) {
+        b->out_off.push_back((unsigned long long)b->offsets[q]);
+        b->gsz.push_back((uint32_t)queries[q].group_nums);
+        b->nparts += (uint32_t)((queries[q].group_nums + kPartSlots - 1) / kPartSlots);
+    }
+    if (b->nparts > kMaxParts) {
+        olap_batch_destroy(b);
+        return bfail(ctx, OLAP_E_CAPACITY, "batch_create: too many aggregation partitions");
+    }
     olap_status st = plan_config(b, b->cfg);
     if (st != OLAP_OK) {
         olap_batch_destroy(b);
@@ -569,12 +684,13 @@ double olap_batch_expected_events(olap_batch b) { return b ? b->events : -1.0; }
 int32_t olap_batch_num_configs(void) { return kNumBatchCfgs; }
 
 olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
-                                   int32_t *lds_table_kib)
+                                   int32_t *lds_table_kib, int32_t *staged)
 {
     if (cfg < 0 || cfg >= kNumBatchCfgs) return OLAP_E_INVALID;
     if (threads) *threads = kBatchCfgs[cfg].bs;
     if (rows_per_thread) *rows_per_thread = 4 * kBatchCfgs[cfg].unroll;
     if (lds_table_kib) *lds_table_kib = kBatchCfgs[cfg].lds_kib;
+    if (staged) *staged = kBatchCfgs[cfg].staged;
     return OLAP_OK;
 }
 
@@ -594,17 +710,7 @@ olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *str
 {
     if (!b) return OLAP_E_INVALID;
     if (!sum || !count) return bfail(b->ctx, OLAP_E_INVALID, "batch_run: null output");
-    hipStream_t s = (hipStream_t)stream;
-    const size_t slots = (size_t)b->offsets[b->nq];
-    hipError_t e = launch_zero_u64((unsigned long long *)sum, slots, s);
-    if (e == hipSuccess) e = launch_zero_u64((unsigned long long *)count, slots, s);
-    if (e != hipSuccess) return bhip(b->ctx, e, "batch zero");
-    BatchArgs a = b->args[b->cfg];
-    a.gsum = (unsigned long long *)sum;
-    a.gcnt = (unsigned long long *)count;
-    e = batch_dispatch(b->cfg, a, b->dyn[b->cfg], b->ctx->num_cus, s, false, &b->grid[b->cfg]);
-    if (e != hipSuccess) return bhip(b->ctx, e, "batch scan");
-    return OLAP_OK;
+    return batch_run(b, sum, count, (hipStream_t)stream, false);
 }
 
 olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void *stream,
@@ -661,9 +767,15 @@ olap_status olap_batch_graph_capture(olap_batch b, int64_t *sum, int64_t *count,
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    // one eager run sizes the staging buffers (the record total of a batch
+    // is fixed by its queries and data), then the capture skips that read
+    olap_status st = batch_run(b, sum, count, s, false);
+    if (st != OLAP_OK) return st;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return bhip(b->ctx, e, "pre-capture run");
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return bhip(b->ctx, e, "begin capture");
-    olap_status st = olap_batch_run(b, sum, count, stream);
+    st = batch_run(b, sum, count, s, true);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     if (st != OLAP_OK) {
@@ -691,8 +803,10 @@ olap_status olap_batch_destroy(olap_batch b)
     if (!b) return OLAP_E_INVALID;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < 4; t++) {
         if (b->dmask[t]) (void)hipFree(b->dmask[t]);
+        if (b->drec[t]) (void)hipFree(b->drec[t]);
+    }
     for (int c = 0; c < kNumBatchCfgs; c++)
         if (b->ddesc[c]) (void)hipFree(b->ddesc[c]);
     delete b;

@@ -77,7 +77,7 @@ struct Layout {
 };
 
 Layout plan_layout(const olap_query_s *q, bool with_steps, bool with_table, bool with_list,
-                   int only_step)
+                   int only_step, uint32_t reserve = 0)
 {
     Layout L;
     const int nd = q->plan.dimvec_nums;
@@ -86,7 +86,7 @@ Layout plan_layout(const olap_query_s *q, bool with_steps, bool with_table, bool
         L.st[s].in_lds = 0;
         L.st[s].lds_off = 0;
     }
-    uint32_t cap = kLdsBytes - (with_list ? kLdsListBytes : 0u) - 16u;
+    uint32_t cap = kLdsBytes - (with_list ? kLdsListBytes : 0u) - 16u - reserve;
     uint32_t cur = 0;  // bytes
     if (with_steps) {
         for (int s = 0; s < nd; s++) {
@@ -264,6 +264,49 @@ int effective_shape(const olap_query_s *q)
     for (const auto &kv : q->ctx->tuned)
         if (kv.first == key) return kv.second;
     return 0;
+}
+
+// Group table beyond LDS: staged aggregation when the expected matches make
+// global atomics (~24 G/s) cost more than a second scan of the key columns.
+bool use_staged(const olap_query_s *q)
+{
+    if (q->agg_layout >= 0) return q->agg_layout == 1;
+    return q->est_events > 2.0e6 && q->est_events > 0.03 * (double)q->cols.rows;
+}
+
+olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hipStream_t s)
+{
+    olap_ctx ctx = q->ctx;
+    const int nd = q->plan.dimvec_nums;
+    const uint32_t G = (uint32_t)q->plan.group_nums;
+    const uint32_t nparts = (G + kPartSlots - 1) / kPartSlots;
+    if (nparts > kMaxParts) return fail(ctx, OLAP_E_CAPACITY, "too many partitions");
+    const uint32_t cnt_bytes = (nparts * 4u + 15u) & ~15u;
+    const uint32_t stage_bytes = cnt_bytes + nparts * 8u;
+    Layout L = plan_layout(q, true, false, false, -1, stage_bytes);
+    const uint32_t cur_lds = align_up(L.dyn_lds, 16u) / 4u;
+    const uint32_t start_lds = cur_lds + cnt_bytes / 4u;
+    const uint32_t dyn = start_lds * 4u + nparts * 8u;
+    uint32_t grid = 0;
+    HIPC(ctx, rowwise_staged_grid(nd, ctx->num_cus, dyn, (uint64_t)q->cols.rows, &grid));
+    if (!q->stager) q->stager = new Stager();
+    Stager &S = *q->stager;
+    if (S.nparts != nparts || S.nwg != grid) {
+        const unsigned long long off0 = 0;
+        HIPC(ctx, S.setup(&off0, &G, 1, grid, q->cols.m2 != nullptr));
+    }
+    ScanArgs a;
+    fill_args(a, q, L, sum, count);
+    a.stage = S.args(cur_lds, start_lds);
+    HIPC(ctx, run_rowwise_staged(a, nd, dyn, AGG_COUNT, grid, s));
+    HIPC(ctx, S.prefix(s));
+    unsigned long long total = 0;
+    HIPC(ctx, S.total(&total, s));
+    HIPC(ctx, S.reserve(total));
+    a.stage = S.args(cur_lds, start_lds);
+    HIPC(ctx, run_rowwise_staged(a, nd, dyn, AGG_SCATTER, grid, s));
+    HIPC(ctx, S.reduce((unsigned long long *)sum, (unsigned long long *)count, ctx->num_cus, s));
+    return OLAP_OK;
 }
 
 struct DeviceGuard {
@@ -598,21 +641,22 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
     q->plan = *plan;
     q->cols = *cols;
 
-    // validation: max code and negative non-NULL codes per step
-    std::vector<int32_t> hstats(2 * nd);
+    // validation: max code, negative non-NULL codes and non-NULL codes per step
+    std::vector<int32_t> hstats(3 * nd);
     for (int s = 0; s < nd; s++) {
-        hstats[2 * s] = -1;
-        hstats[2 * s + 1] = 0;
+        hstats[3 * s] = -1;
+        hstats[3 * s + 1] = 0;
+        hstats[3 * s + 2] = 0;
     }
     int32_t *dstats = (int32_t *)ctx->d_scratch;
     hipStream_t s0 = nullptr;
-    hipError_t e = hipMemcpyAsync(dstats, hstats.data(), 8 * nd, hipMemcpyHostToDevice, s0);
+    hipError_t e = hipMemcpyAsync(dstats, hstats.data(), 12 * nd, hipMemcpyHostToDevice, s0);
     for (int s = 0; s < nd && e == hipSuccess; s++) {
         const int t = plan->orders[s];
         e = launch_dim_stats(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t],
-                             dstats + 2 * s, s0);
+                             dstats + 3 * s, s0);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(hstats.data(), dstats, 8 * nd, hipMemcpyDeviceToHost, s0);
+    if (e == hipSuccess) e = hipMemcpyAsync(hstats.data(), dstats, 12 * nd, hipMemcpyDeviceToHost, s0);
     if (e == hipSuccess) e = hipStreamSynchronize(s0);
     if (e != hipSuccess) {
         delete q;
@@ -622,14 +666,16 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
     uint32_t lw[OLAP_MAX_DIMS] = {0, 0, 0, 0};
     uint32_t nwords[OLAP_MAX_DIMS] = {0, 0, 0, 0};
     size_t total_words = 0;
+    double pass = (double)cols->rows;
     for (int s = 0; s < nd; s++) {
         const int t = plan->orders[s];
-        if (hstats[2 * s + 1] != 0) {
+        if (hstats[3 * s + 1] != 0) {
             delete q;
             return fail(ctx, OLAP_E_INVALID, "table %d holds %d negative non-NULL codes", t,
-                        hstats[2 * s + 1]);
+                        hstats[3 * s + 1]);
         }
-        const int64_t mx = hstats[2 * s];
+        pass *= cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
+        const int64_t mx = hstats[3 * s];
         if (mx > 0) max_gid += mx * (int64_t)plan->factor[s];
         // smallest power-of-two width whose all-ones value exceeds every code
         uint32_t l = 0;
@@ -646,6 +692,7 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
                     (long long)max_gid, (long long)plan->group_nums);
     }
     q->max_gid = max_gid;
+    q->est_events = pass;
     e = hipMalloc(&q->packed, std::max<size_t>(total_words, 4) * 4);
     if (e != hipSuccess) {
         delete q;
@@ -702,6 +749,8 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
     case OLAP_VWM_SV:
     case OLAP_VWM_DV: {
         Layout L = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1);
+        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q))
+            return run_rowwise_stage(q, sum_p, count, s);
         ScanArgs a;
         fill_args(a, q, L, sum_p, count);
         hipError_t e;
@@ -737,6 +786,7 @@ olap_status olap_query_destroy(olap_query q)
     if (q->list_gid) dev_free(q->list_gid);
     if (q->list_cnt) dev_free(q->list_cnt);
     if (q->gvec) dev_free(q->gvec);
+    delete q->stager;
     delete q;
     return OLAP_OK;
 }
@@ -765,23 +815,40 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape)
     return OLAP_OK;
 }
 
+olap_status olap_query_set_aggregation(olap_query q, int32_t layout)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (layout < -1 || layout > 1) return fail(q->ctx, OLAP_E_INVALID, "layout must be -1, 0 or 1");
+    q->agg_layout = layout;
+    return OLAP_OK;
+}
+
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out)
 {
     if (!q) return OLAP_E_INVALID;
     olap_ctx ctx = q->ctx;
-    const int n = olap_num_shapes(strategy);
     DeviceGuard g(ctx->device);
     hipStream_t s = as_stream(stream);
+    // candidates: every launch shape; a row-wise run whose group table exceeds
+    // LDS also tries the staged aggregation (one fixed shape)
+    std::vector<std::pair<int, int>> cands;  // (shape, aggregation layout)
+    const bool table_in_lds = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1).gt_lds != 0;
+    for (int i = 0; i < olap_num_shapes(strategy); i++)
+        cands.push_back({i, strategy == OLAP_ROWWISE && !table_in_lds ? 0 : -1});
+    if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) cands.push_back({0, 1});
+    const int n = (int)cands.size();
     hipEvent_t e0, e1;
     HIPC(ctx, hipEventCreate(&e0));
     HIPC(ctx, hipEventCreate(&e1));
     int best = 0;
     double best_ms = 1e30;
     const int saved = q->row_shape;
+    const int saved_layout = q->agg_layout;
     olap_status st = OLAP_OK;
     for (int i = 0; i < n && st == OLAP_OK; i++) {
-        q->row_shape = i;
+        q->row_shape = cands[(size_t)i].first;
+        q->agg_layout = cands[(size_t)i].second;
         st = olap_query_run(q, strategy, sum, count, stream);  // warm
         std::vector<double> t;
         for (int r = 0; r < 3 && st == OLAP_OK; r++) {
@@ -805,18 +872,21 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
     (void)hipEventDestroy(e1);
     if (st != OLAP_OK) {
         q->row_shape = saved;
+        q->agg_layout = saved_layout;
         return st;
     }
-    q->row_shape = best;
+    q->row_shape = cands[(size_t)best].first;
+    q->agg_layout = cands[(size_t)best].second;
     const uint64_t key = query_class(q, strategy);
     bool found = false;
     for (auto &kv : ctx->tuned)
         if (kv.first == key) {
-            kv.second = best;
+            kv.second = q->row_shape;
             found = true;
         }
-    if (!found) ctx->tuned.push_back({key, best});
-    if (shape_out) *shape_out = best;
+    if (!found) ctx->tuned.push_back({key, q->row_shape});
+    // shape index; num_shapes(strategy) stands for the staged aggregation
+    if (shape_out) *shape_out = q->agg_layout == 1 ? olap_num_shapes(strategy) : q->row_shape;
     if (ms_out) *ms_out = best_ms;
     // leave the outputs holding a complete run of the chosen shape
     return olap_query_run(q, strategy, sum, count, stream);

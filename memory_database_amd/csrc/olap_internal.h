@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "olapcore.h"
+#include "olap_stage.h"
 
 namespace olap {
 
@@ -62,6 +63,7 @@ struct ScanArgs {
     uint32_t *list_oid;         // per-tile segments (dynamic-vector column-wise)
     int32_t *list_gid;
     uint32_t *list_cnt;         // entries per tile
+    StageArgs stage;            // staged aggregation passes
 };
 
 }  // namespace olap
@@ -96,6 +98,12 @@ struct olap_query_s {
     size_t gvec_rows = 0;
     bool joined_dv = false;
     int row_shape = -1;                // row-wise launch shape (-1: default / cached)
+    // aggregation layout of a row-wise run whose group table exceeds LDS:
+    // -1 automatic (from est_events), 0 global atomics, 1 staged
+    int agg_layout = -1;
+    double est_events = 0.0;           // expected matches (uniform keys assumed)
+    olap::Stager *stager = nullptr;
+    uint32_t stage_grid = 0;
 };
 
 namespace olap {
@@ -120,7 +128,8 @@ hipError_t launch_gen_zipf(int32_t *out, int64_t n, int64_t first, uint64_t key,
 hipError_t launch_build_dimvec(const int32_t *filter, const int32_t *group, int64_t rows,
                                int32_t lo, int32_t hi, int32_t bitmap, int32_t group_mod,
                                int32_t code_bytes, void *out, hipStream_t s);
-// stats[0] = max code (-1 when all NULL), stats[1] = negative non-NULL codes
+// stats[0] = max code (-1 when all NULL), stats[1] = negative non-NULL codes,
+// stats[2] = non-NULL codes
 hipError_t launch_dim_stats(const void *dimvec, int32_t code_bytes, int64_t rows,
                             int32_t *stats, hipStream_t s);
 hipError_t launch_pack(const void *dimvec, int32_t code_bytes, int64_t rows, uint32_t lw,
@@ -131,6 +140,11 @@ hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bo
                        int shape, hipStream_t s);
 int num_row_shapes();
 ScanShape row_shape(int i);
+// staged row-wise scan (group table beyond LDS): grid shared by both passes
+hipError_t rowwise_staged_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows,
+                               uint32_t *grid);
+hipError_t run_rowwise_staged(ScanArgs &a, int ndims, uint32_t dyn_lds, int mode, uint32_t grid,
+                              hipStream_t s);
 hipError_t run_vwm_sv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
                       hipStream_t s);
 hipError_t run_vwm_dv(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,

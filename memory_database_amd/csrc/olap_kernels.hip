@@ -178,11 +178,13 @@ __global__ void k_dim_stats(const void *dv, int32_t cb, int64_t rows, int32_t *s
 {
     int32_t mx = -1;
     int32_t neg = 0;
+    int32_t nonnull = 0;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows;
          r += (int64_t)gridDim.x * blockDim.x) {
         bool nul;
         int32_t v = read_code(dv, cb, r, nul);
         if (!nul) {
+            nonnull++;
             if (v < 0) neg++;
             else mx = v > mx ? v : mx;
         }
@@ -190,12 +192,15 @@ __global__ void k_dim_stats(const void *dv, int32_t cb, int64_t rows, int32_t *s
     for (int o = 32; o > 0; o >>= 1) {
         int32_t omx = __shfl_xor(mx, o);
         int32_t oneg = __shfl_xor(neg, o);
+        int32_t onn = __shfl_xor(nonnull, o);
         mx = omx > mx ? omx : mx;
         neg += oneg;
+        nonnull += onn;
     }
     if ((threadIdx.x & 63) == 0) {
         atomicMax(&stats[0], mx);
         if (neg) atomicAdd(&stats[1], neg);
+        if (nonnull) atomicAdd(&stats[2], nonnull);
     }
 }
 
@@ -401,12 +406,29 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want)
 // chunks that still hold a live row, so dead lines are never fetched.
 // ------------------------------------------------------------------------
 
-template <int NDIM, bool VEC, bool GLDS, int BS, int U>
+// one match of a staged scan: count it, or write its record
+template <int MODE>
+__device__ __forceinline__ void emit_staged(const ScanArgs &a, uint32_t *lds, uint32_t gid,
+                                            int32_t m1, int32_t m2)
+{
+    if (gid >= a.G) {
+        atomic_add_u32_dev(a.err, 1u);
+        return;
+    }
+    if (MODE == AGG_COUNT) stage_count(a.stage, lds, gid >> kPartBits);
+    else stage_scatter(a.stage, lds, gid >> kPartBits, gid & (kPartSlots - 1u), m1, m2);
+}
+
+// MODE: AGG_DIRECT aggregates into the LDS table (GLDS) or with global
+// atomics; AGG_COUNT / AGG_SCATTER are the two scan passes of the staged
+// aggregation (the COUNT pass never reads the measures)
+template <int NDIM, bool VEC, bool GLDS, int BS, int U, int MODE = AGG_DIRECT>
 __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
 {
     constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
     extern __shared__ __align__(16) uint32_t lds[];
     lds_init<NDIM, GLDS>(a, lds);
+    stage_begin<MODE>(a.stage, lds);
     __syncthreads();
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
@@ -458,7 +480,7 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
         for (int u = 0; u < U; u++) {
             m1[u] = v4i{0, 0, 0, 0};
             m2[u] = v4i{0, 0, 0, 0};
-            if (alive[u] && !a.count_only) {
+            if (MODE != AGG_COUNT && alive[u] && !a.count_only) {
                 uint32_t dummy;
                 const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
                 m1[u] = load4<VEC>(a.m1, r, rows, dummy);
@@ -470,12 +492,21 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
             if (alive[u]) {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
-                    if ((alive[u] >> k) & 1u)
-                        aggregate<GLDS>(a, lds, gid[u][k], measure4(a, m1[u], m2[u], k));
+                    if ((alive[u] >> k) & 1u) {
+                        if (MODE == AGG_DIRECT)
+                            aggregate<GLDS>(a, lds, gid[u][k], measure4(a, m1[u], m2[u], k));
+                        else
+                            emit_staged<MODE>(a, lds, gid[u][k], m1[u][k], a.m2 ? m2[u][k] : 0);
+                    }
             }
         }
     }
-    flush<GLDS>(a, lds);
+    if (MODE == AGG_DIRECT) {
+        flush<GLDS>(a, lds);
+    } else if (MODE == AGG_COUNT) {
+        __syncthreads();
+        stage_end<MODE>(a.stage, lds);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -938,6 +969,52 @@ hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bo
     case 3: return rowwise_shape<3, true, false>(a, num_cus, dyn_lds, shape, s);
     default: return rowwise_shape<4, true, false>(a, num_cus, dyn_lds, shape, s);
     }
+}
+
+// Staged row-wise scan: one fixed launch shape and an explicit grid, so that
+// the COUNT and the SCATTER pass give every workgroup the same tiles.
+constexpr int kStageBS = 512;
+constexpr int kStageU = 4;
+
+template <int MODE>
+static const void *staged_kernel(int ndims)
+{
+    switch (ndims) {
+    case 1: return (const void *)k_rowwise<1, true, false, kStageBS, kStageU, MODE>;
+    case 2: return (const void *)k_rowwise<2, true, false, kStageBS, kStageU, MODE>;
+    case 3: return (const void *)k_rowwise<3, true, false, kStageBS, kStageU, MODE>;
+    default: return (const void *)k_rowwise<4, true, false, kStageBS, kStageU, MODE>;
+    }
+}
+
+hipError_t rowwise_staged_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows,
+                               uint32_t *grid)
+{
+    int best = 1 << 30;
+    for (const void *k : {staged_kernel<AGG_COUNT>(ndims), staged_kernel<AGG_SCATTER>(ndims)}) {
+        if (dyn_lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds);
+            if (e != hipSuccess) return e;
+        }
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kStageBS, dyn_lds);
+        if (e != hipSuccess) return e;
+        best = std::min(best, per_cu);
+    }
+    if (best < 1) return hipErrorInvalidConfiguration;
+    const uint64_t tiles = (rows + (uint64_t)kStageBS * 4u * kStageU - 1) / ((uint64_t)kStageBS * 4u * kStageU);
+    *grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best * num_cus, tiles));
+    return hipSuccess;
+}
+
+hipError_t run_rowwise_staged(ScanArgs &a, int ndims, uint32_t dyn_lds, int mode, uint32_t grid,
+                              hipStream_t s)
+{
+    a.ntiles = (a.rows + (uint64_t)kStageBS * 4u * kStageU - 1) / ((uint64_t)kStageBS * 4u * kStageU);
+    if (a.ntiles == 0) return hipSuccess;
+    const void *k = mode == AGG_COUNT ? staged_kernel<AGG_COUNT>(ndims) : staged_kernel<AGG_SCATTER>(ndims);
+    void *args[] = {&a};
+    return hipLaunchKernel(k, dim3(grid), dim3(kStageBS), args, dyn_lds, s);
 }
 
 int num_row_shapes() { return kNumRowShapes; }

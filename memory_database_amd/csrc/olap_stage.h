@@ -1,0 +1,145 @@
+// olap_stage.h -- staged (radix-partitioned) group-by aggregation.
+//
+// Global 64-bit atomics sustain ~24 G updates/s on MI355X while LDS atomics
+// sustain ~1.6 T/s, so group tables that do not fit in LDS are aggregated in
+// three passes instead of with one global atomic pair per matching row:
+//   COUNT   the scan counts its matches per (workgroup, partition) in LDS;
+//           a partition is kPartSlots consecutive gids of one query;
+//   prefix  exclusive sums give every (partition, workgroup) a private range
+//           of the record buffer and every partition its record range;
+//   SCATTER the same scan (same grid, same tile order) writes one record per
+//           match -- gid within the partition and the raw measure(s) -- to
+//           its workgroup's range, with LDS cursors (no global atomics);
+//   reduce  workgroups take chunks of a partition's records, aggregate them
+//           in an LDS table of kPartSlots slots and add the non-empty slots
+//           to the int64 outputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace olap {
+
+constexpr uint32_t kPartBits = 13;
+constexpr uint32_t kPartSlots = 1u << kPartBits;       // 8192 gids per partition (96 KB LDS table)
+constexpr uint32_t kReduceChunk = 1u << 21;             // records per reduce work item
+constexpr uint32_t kMaxParts = 1u << 16;
+
+enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2 };
+
+struct StageArgs {
+    uint32_t nparts;                  // partitions of this scan
+    uint32_t nwg;                     // workgroups of the scan (both passes)
+    uint32_t cur_lds;                 // word offset: u32 counter / cursor per partition
+    uint32_t start_lds;               // word offset: u64 range start per partition (scatter)
+    uint32_t *wg_counts;              // [nparts][nwg]
+    const unsigned long long *wg_start;  // [nparts][nwg]
+    // one record per match, written with a single store: {gid within the
+    // partition, m1} (uint2), or {gid, m1, m2, 0} (uint4) with two measures
+    uint2 *rec2;
+    uint4 *rec4;
+    unsigned long long cap;           // records the buffer holds
+    uint32_t *overflow;               // set when a range would exceed cap
+};
+
+#if defined(__HIPCC__)
+// LDS accesses with per-lane indexes go through explicit address-space-3
+// pointers (generic pointers derived from LDS trip a gfx950 code-generation
+// error in this compiler inside the batched kernel's event loop)
+typedef __attribute__((address_space(3))) uint32_t stage_l_u32;
+typedef __attribute__((address_space(3))) unsigned long long stage_l_u64;
+
+// count one match of partition p (LDS counter)
+__device__ __forceinline__ void stage_count(const StageArgs &s, uint32_t *lds, uint32_t p)
+{
+    stage_l_u32 *c = (stage_l_u32 *)lds + s.cur_lds;
+    __hip_atomic_fetch_add(c + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// write one record of partition p
+__device__ __forceinline__ void stage_scatter(const StageArgs &s, uint32_t *lds, uint32_t p,
+                                              uint32_t gid_low, int32_t m1, int32_t m2)
+{
+    stage_l_u32 *c = (stage_l_u32 *)lds + s.cur_lds;
+    const uint32_t pos = __hip_atomic_fetch_add(c + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const stage_l_u64 *st = (const stage_l_u64 *)((stage_l_u32 *)lds + s.start_lds);
+    const unsigned long long i = st[p] + pos;
+    if (i >= s.cap) {
+        __hip_atomic_fetch_add(s.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if (s.rec4) s.rec4[i] = make_uint4(gid_low, (uint32_t)m1, (uint32_t)m2, 0u);
+    else s.rec2[i] = make_uint2(gid_low, (uint32_t)m1);
+}
+
+// pass prologue: zero the counters (count) or load the range starts (scatter)
+template <int MODE>
+__device__ __forceinline__ void stage_begin(const StageArgs &s, uint32_t *lds)
+{
+    if (MODE == AGG_DIRECT) return;
+    for (uint32_t p = threadIdx.x; p < s.nparts; p += blockDim.x) {
+        lds[s.cur_lds + p] = 0u;
+        if (MODE == AGG_SCATTER) {
+            unsigned long long *st = (unsigned long long *)(lds + s.start_lds);
+            st[p] = s.wg_start[(size_t)p * s.nwg + blockIdx.x];
+        }
+    }
+}
+
+// pass epilogue of COUNT: publish this workgroup's counts (after a barrier)
+template <int MODE>
+__device__ __forceinline__ void stage_end(const StageArgs &s, uint32_t *lds)
+{
+    if (MODE != AGG_COUNT) return;
+    for (uint32_t p = threadIdx.x; p < s.nparts; p += blockDim.x)
+        s.wg_counts[(size_t)p * s.nwg + blockIdx.x] = lds[s.cur_lds + p];
+}
+#endif
+
+// Host side, olap_stage.hip. A Stager owns the scratch of one scan.
+struct Stager {
+    uint32_t nparts = 0;
+    uint32_t nwg = 0;
+    bool two_measures = false;
+    uint32_t *wg_counts = nullptr;              // [nparts][nwg]
+    unsigned long long *wg_start = nullptr;     // [nparts][nwg]
+    unsigned long long *part_tot = nullptr;     // records per partition
+    unsigned long long *part_begin = nullptr;   // nparts + 1
+    uint32_t *item_begin = nullptr;             // nparts + 1
+    uint32_t *work = nullptr;                   // [0] reduce work counter, [1] overflow
+    unsigned long long *part_out = nullptr;     // first output slot of each partition
+    uint32_t *part_slots = nullptr;             // slots of each partition
+    void *rec = nullptr;                        // uint2 or uint4 records
+    bool rec_wide = false;                      // rec holds uint4 records
+    unsigned long long cap = 0;
+
+    // partitions of every query: out_off[q] (first output slot) and G[q]
+    hipError_t setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg,
+                     bool two_measures);
+    // after the COUNT pass: the prefix sums (device only)
+    hipError_t prefix(hipStream_t s);
+    // the record total of the last COUNT pass (synchronises)
+    hipError_t total(unsigned long long *out, hipStream_t s);
+    hipError_t reserve(unsigned long long records);
+    // first partition of query q
+    uint32_t part_base(int q) const { return qbase_.empty() ? 0 : qbase_[(size_t)q]; }
+    hipError_t reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus,
+                      hipStream_t s);
+    StageArgs args(uint32_t cur_lds, uint32_t start_lds) const;
+    // LDS bytes of the counters/cursors plus the u64 range starts
+    uint32_t lds_bytes() const;
+    // records dropped because the buffers were too small (synchronises)
+    hipError_t overflowed(uint32_t *out, hipStream_t s);
+    void release();
+    Stager() = default;
+    Stager(const Stager &) = delete;
+    Stager &operator=(const Stager &) = delete;
+    ~Stager() { release(); }
+
+  private:
+    std::vector<uint32_t> qbase_;
+};
+
+}  // namespace olap

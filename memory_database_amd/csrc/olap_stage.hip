@@ -1,0 +1,332 @@
+// olap_stage.hip -- prefix and reduce passes of the staged aggregation
+// (olap_stage.h), and the Stager host object that owns their scratch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "olap_internal.h"
+#include "olap_stage.h"
+
+namespace olap {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+
+// per partition (one block each): exclusive sums of its nwg counts -> starts,
+// and the partition total -> totals[p]
+__global__ __launch_bounds__(kScanThreads) void k_stage_scan_local(const uint32_t *counts,
+                                                                   unsigned long long *starts,
+                                                                   unsigned long long *totals,
+                                                                   uint32_t nwg)
+{
+    __shared__ unsigned long long part[kScanThreads];
+    const uint32_t p = blockIdx.x;
+    const uint32_t per = (nwg + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = min(threadIdx.x * per, nwg);
+    const uint32_t e = min(b + per, nwg);
+    unsigned long long acc = 0;
+    for (uint32_t i = b; i < e; i++) acc += counts[(size_t)p * nwg + i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int t = 0; t < kScanThreads; t++) {
+            const unsigned long long v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        totals[p] = run;
+    }
+    __syncthreads();
+    unsigned long long run = part[threadIdx.x];
+    for (uint32_t i = b; i < e; i++) {
+        const unsigned long long v = counts[(size_t)p * nwg + i];
+        starts[(size_t)p * nwg + i] = run;
+        run += v;
+    }
+}
+
+// part_begin / item_begin = exclusive sums over partitions of the record
+// totals and of the reduce work items (nparts + 1 entries each)
+__global__ __launch_bounds__(1024) void k_stage_scan_parts(const unsigned long long *totals,
+                                                           unsigned long long *part_begin,
+                                                           uint32_t *item_begin, uint32_t nparts)
+{
+    __shared__ unsigned long long tot[1024];
+    __shared__ uint32_t items[1024];
+    const uint32_t per = (nparts + 1023) / 1024;
+    const uint32_t b = min(threadIdx.x * per, nparts);
+    const uint32_t e = min(b + per, nparts);
+    unsigned long long acc = 0;
+    uint32_t it = 0;
+    for (uint32_t p = b; p < e; p++) {
+        acc += totals[p];
+        it += (uint32_t)((totals[p] + kReduceChunk - 1) / kReduceChunk);
+    }
+    tot[threadIdx.x] = acc;
+    items[threadIdx.x] = it;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = 0;
+        uint32_t ri = 0;
+        for (int t = 0; t < 1024; t++) {
+            const unsigned long long v = tot[t];
+            const uint32_t vi = items[t];
+            tot[t] = r;
+            items[t] = ri;
+            r += v;
+            ri += vi;
+        }
+        part_begin[nparts] = r;
+        item_begin[nparts] = ri;
+    }
+    __syncthreads();
+    unsigned long long r = tot[threadIdx.x];
+    uint32_t ri = items[threadIdx.x];
+    for (uint32_t p = b; p < e; p++) {
+        part_begin[p] = r;
+        item_begin[p] = ri;
+        r += totals[p];
+        ri += (uint32_t)((totals[p] + kReduceChunk - 1) / kReduceChunk);
+    }
+}
+
+__global__ void k_stage_finalize(unsigned long long *starts, const unsigned long long *part_begin,
+                                 uint32_t nparts, uint32_t nwg)
+{
+    const size_t n = (size_t)nparts * nwg;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        starts[i] += part_begin[i / nwg];
+}
+
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
+
+// work items: chunks of kReduceChunk records of one partition, claimed with
+// one global atomic each; the partition of an item is found by binary search
+template <bool WIDE>
+__global__ __launch_bounds__(512) void k_stage_reduce(
+    const unsigned long long *part_begin, const uint32_t *item_begin, uint32_t nparts,
+    const void *rec, const unsigned long long *part_out, const uint32_t *part_slots,
+    uint32_t *work, unsigned long long *gsum, unsigned long long *gcnt)
+{
+    __shared__ unsigned long long lsum[kPartSlots];
+    __shared__ uint32_t lcnt[kPartSlots];
+    __shared__ uint32_t s_item, s_part;
+    const uint32_t total_items = item_begin[nparts];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t item =
+                __hip_atomic_fetch_add(work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t lo = 0;
+            if (item < total_items) {
+                uint32_t hi = nparts;  // last p with item_begin[p] <= item
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (item_begin[mid] <= item) lo = mid; else hi = mid;
+                }
+            }
+            s_item = item;
+            s_part = lo;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        const uint32_t p = s_part;
+        if (item >= total_items) break;
+        const uint32_t slots = part_slots[p];
+        const unsigned long long b =
+            part_begin[p] + (unsigned long long)(item - item_begin[p]) * kReduceChunk;
+        const unsigned long long e = min(b + kReduceChunk, part_begin[p + 1]);
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            lsum[s] = 0ull;
+            lcnt[s] = 0u;
+        }
+        __syncthreads();
+        for (unsigned long long i = b + threadIdx.x; i < e; i += blockDim.x) {
+            uint32_t g;
+            long long m;
+            if (WIDE) {
+                const uint4 r = ((const uint4 *)rec)[i];
+                g = r.x;
+                m = (long long)(int32_t)r.y + (long long)(int32_t)r.z;
+            } else {
+                const uint2 r = ((const uint2 *)rec)[i];
+                g = r.x;
+                m = (long long)(int32_t)r.y;
+            }
+            if (g < slots) {
+                __hip_atomic_fetch_add(&lsum[g], (unsigned long long)m, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&lcnt[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        __syncthreads();
+        const unsigned long long out = part_out[p];
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            const uint32_t c = lcnt[s];
+            if (c) {
+                __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+hipError_t realloc_dev(T *&p, size_t n)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+}  // namespace
+
+hipError_t Stager::setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg_,
+                         bool two)
+{
+    qbase_.assign((size_t)nq, 0);
+    std::vector<unsigned long long> pout;
+    std::vector<uint32_t> pslots;
+    for (int q = 0; q < nq; q++) {
+        qbase_[(size_t)q] = (uint32_t)pout.size();
+        const uint32_t np = (G[q] + kPartSlots - 1) / kPartSlots;
+        for (uint32_t i = 0; i < np; i++) {
+            pout.push_back(out_off[q] + (unsigned long long)i * kPartSlots);
+            pslots.push_back(std::min<uint32_t>(kPartSlots, G[q] - i * kPartSlots));
+        }
+    }
+    if (pout.size() > kMaxParts) return hipErrorInvalidValue;
+    nparts = (uint32_t)pout.size();
+    nwg = nwg_;
+    two_measures = two;
+    const size_t np = std::max<uint32_t>(nparts, 1);
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = realloc_dev(wg_counts, np * nwg);
+    if (e == hipSuccess) e = realloc_dev(wg_start, np * nwg);
+    if (e == hipSuccess) e = realloc_dev(part_tot, np);
+    if (e == hipSuccess) e = realloc_dev(part_begin, np + 1);
+    if (e == hipSuccess) e = realloc_dev(item_begin, np + 1);
+    if (e == hipSuccess) e = realloc_dev(work, 4);
+    if (e == hipSuccess) e = realloc_dev(part_out, np);
+    if (e == hipSuccess) e = realloc_dev(part_slots, np);
+    if (e == hipSuccess && nparts) {
+        e = hipMemcpy(part_out, pout.data(), nparts * sizeof(unsigned long long), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(part_slots, pslots.data(), nparts * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = launch_zero_u32(work, 4, nullptr);
+    return e;
+}
+
+hipError_t Stager::prefix(hipStream_t s)
+{
+    if (!nparts) return hipSuccess;
+    k_stage_scan_local<<<nparts, kScanThreads, 0, s>>>(wg_counts, wg_start, part_tot, nwg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_stage_scan_parts<<<1, 1024, 0, s>>>(part_tot, part_begin, item_begin, nparts);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t n = (size_t)nparts * nwg;
+    k_stage_finalize<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(
+        wg_start, part_begin, nparts, nwg);
+    return hipGetLastError();
+}
+
+hipError_t Stager::total(unsigned long long *out, hipStream_t s)
+{
+    *out = 0;
+    if (!nparts) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(out, part_begin + nparts, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+hipError_t Stager::reserve(unsigned long long records)
+{
+    if (records <= cap && rec && rec_wide == two_measures) return hipSuccess;
+    const unsigned long long want = records + records / 4 + 4096;
+    if (rec) (void)hipFree(rec);
+    rec = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&rec, want * (two_measures ? 16u : 8u));
+    if (e == hipSuccess) {
+        cap = want;
+        rec_wide = two_measures;
+    }
+    return e;
+}
+
+hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus,
+                          hipStream_t s)
+{
+    if (!nparts) return hipSuccess;
+    hipError_t e = launch_zero_u32(work, 1, s);
+    if (e != hipSuccess) return e;
+    // 96 KB of LDS per workgroup: one per CU
+    if (rec_wide)
+        k_stage_reduce<true><<<num_cus, 512, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
+                                                     part_slots, work, gsum, gcnt);
+    else
+        k_stage_reduce<false><<<num_cus, 512, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
+                                                      part_slots, work, gsum, gcnt);
+    return hipGetLastError();
+}
+
+StageArgs Stager::args(uint32_t cur_lds, uint32_t start_lds) const
+{
+    StageArgs a;
+    a.nparts = nparts;
+    a.nwg = nwg;
+    a.cur_lds = cur_lds;
+    a.start_lds = start_lds;
+    a.wg_counts = wg_counts;
+    a.wg_start = wg_start;
+    a.rec2 = rec_wide ? nullptr : (uint2 *)rec;
+    a.rec4 = rec_wide ? (uint4 *)rec : nullptr;
+    a.cap = cap;
+    a.overflow = work + 1;
+    return a;
+}
+
+uint32_t Stager::lds_bytes() const
+{
+    // u32 counters/cursors, then 8-byte aligned u64 range starts
+    return ((nparts * 4u + 15u) & ~15u) + nparts * 8u;
+}
+
+hipError_t Stager::overflowed(uint32_t *out, hipStream_t s)
+{
+    *out = 0;
+    if (!work) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(out, work + 1, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+void Stager::release()
+{
+    void *ptrs[] = {wg_counts, wg_start, part_tot, part_begin, item_begin,
+                    work,      part_out, part_slots, rec};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    wg_counts = nullptr;
+    wg_start = nullptr;
+    part_tot = nullptr;
+    part_begin = nullptr;
+    item_begin = nullptr;
+    work = nullptr;
+    part_out = nullptr;
+    part_slots = nullptr;
+    rec = nullptr;
+    cap = 0;
+    nparts = 0;
+}
+
+}  // namespace olap

@@ -121,6 +121,7 @@ SIGNATURES = {
     "olap_num_shapes": (C.c_int32, [C.c_int]),
     "olap_shape_info": (C.c_int, [C.c_int, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "olap_query_set_shape": (C.c_int, [_P, C.c_int32]),
+    "olap_query_set_aggregation": (C.c_int, [_P, C.c_int32]),
     "olap_query_autotune": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_double)]),
     "olap_rowwise": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
@@ -137,7 +138,7 @@ SIGNATURES = {
     "olap_batch_run": (C.c_int, [_P, _P, _P, _P]),
     "olap_batch_num_configs": (C.c_int32, []),
     "olap_batch_config_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                         C.POINTER(C.c_int32)]),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "olap_batch_set_config": (C.c_int, [_P, C.c_int32]),
     "olap_batch_autotune": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "olap_batch_graph_capture": (C.c_int, [_P, _P, _P, _P]),
@@ -204,6 +205,13 @@ def legacy_count(sums) -> int:
     import numpy as np
     a = np.ascontiguousarray(np.asarray(sums, dtype=np.int64))
     return int(load().olap_legacy_count(a.ctypes.data, a.size))
+
+
+def config_info(cfg: int):
+    """(threads, rows per thread, LDS KiB for query tables, staged) of a batch config."""
+    th, rp, kib, st = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    load().olap_batch_config_info(cfg, C.byref(th), C.byref(rp), C.byref(kib), C.byref(st))
+    return th.value, rp.value, kib.value, st.value
 
 
 def zipf_table(d: int, theta: float):
@@ -404,6 +412,10 @@ class Query:
     def set_shape(self, shape: int):
         self.ctx.check(self.ctx.lib.olap_query_set_shape(self.h, shape), "set_shape")
 
+    def set_aggregation(self, layout: int):
+        """-1 automatic, 0 global atomics, 1 staged (group tables beyond LDS)."""
+        self.ctx.check(self.ctx.lib.olap_query_set_aggregation(self.h, layout), "set_aggregation")
+
     def device_bytes(self) -> int:
         return int(self.ctx.lib.olap_query_device_bytes(self.h))
 
@@ -453,14 +465,13 @@ class Batch:
         self.ctx.check(self.ctx.lib.olap_batch_set_config(self.h, cfg), "batch_set_config")
 
     def autotune(self, out_sum, out_cnt, stream=None):
-        """Returns (config, threads, rows per thread, LDS KiB for tables, ms)."""
+        """Returns (config, threads, rows per thread, LDS KiB for tables,
+        staged, ms)."""
         cfg = C.c_int32()
         ms = C.c_double()
         self.ctx.check(self.ctx.lib.olap_batch_autotune(self.h, _ptr(out_sum), _ptr(out_cnt), stream,
                                                         C.byref(cfg), C.byref(ms)), "batch_autotune")
-        th, rp, kib = C.c_int32(), C.c_int32(), C.c_int32()
-        self.ctx.lib.olap_batch_config_info(cfg.value, C.byref(th), C.byref(rp), C.byref(kib))
-        return cfg.value, th.value, rp.value, kib.value, ms.value
+        return (cfg.value, *config_info(cfg.value), ms.value)
 
     def capture(self, out_sum, out_cnt, stream):
         self.ctx.check(self.ctx.lib.olap_batch_graph_capture(self.h, _ptr(out_sum), _ptr(out_cnt),

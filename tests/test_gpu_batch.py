@@ -77,8 +77,9 @@ def test_batch_matches_oracle_every_config(ctx, ncpu):
         else:
             np.testing.assert_array_equal(got[0], ref[0], err_msg=f"config {k} sums")
             np.testing.assert_array_equal(got[1], ref[1], err_msg=f"config {k} counts")
-    cfgk, th, rp, kib, ms = b.autotune(osum, ocnt)
-    _say(f"  autotune: config {cfgk} ({th} threads, {rp} rows/thread, {kib} KiB tables) {ms:.3f} ms")
+    cfgk, th, rp, kib, staged, ms = b.autotune(osum, ocnt)
+    _say(f"  autotune: config {cfgk} ({th} threads, {rp} rows/thread, {kib} KiB tables, "
+         f"staged={staged}) {ms:.3f} ms")
     torch.cuda.synchronize()
     np.testing.assert_array_equal(osum.cpu().numpy(), ref[0])
     b.close()

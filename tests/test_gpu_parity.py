@@ -160,6 +160,50 @@ def test_wide_codes_and_global_group_table(ctx, ncpu):
     check_all(ctx, p, dvs, fks, m, None, ncpu)
 
 
+@pytest.mark.parametrize("G,sel,rows,two", [(100_000, 60, 3_000_001, False),
+                                            (1_000_000, 90, 2_000_000, True),
+                                            (5_000, 100, 1_000_003, False)])
+def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two):
+    """Group tables beyond LDS: global atomics, staged (count / scatter /
+    reduce) and the automatic choice all give the oracle's result."""
+    import torch
+
+    rng = np.random.default_rng(G + sel)
+    _say(f"staged aggregation G={G} sel={sel}% rows={rows} two_measures={two}")
+    D = 400_000
+    f = rng.integers(0, 100, D, dtype=np.int32)
+    g = rng.integers(0, G, D, dtype=np.int32)
+    dv = po.build_dimvec(f, g, 0, sel, 1, 4)
+    D2 = 3000
+    dv2 = po.build_dimvec(rng.integers(0, 100, D2, dtype=np.int32), np.zeros(D2, np.int32),
+                          0, 95, 0, 1)
+    dvs = [dv, None, dv2, None]
+    fks = [rng.integers(0, D, rows, dtype=np.int32), None,
+           rng.integers(0, D2 + 5, rows, dtype=np.int32), None]
+    m1 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    m2 = rng.integers(0, 1000, rows, dtype=np.int32) if two else None
+    p = po.plan([2, 0], [0, 1], G)
+    es, ec, _ = po.run("rowwise", p, po.Star(dvs, fks, m1, m2), nthreads=ncpu)
+    plan = oc.plan_from([2, 0], [0, 1], G)
+    cols = oc.Context.columns([_dev(d, torch) for d in dvs], [_dev(x, torch) for x in fks],
+                              _dev(m1, torch), _dev(m2, torch), rows=rows)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    for layout in (0, 1, -1, 1):
+        _say(f"  layout {layout}")
+        q.set_aggregation(layout)
+        q.run("rowwise", osum, ocnt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"layout {layout}")
+        np.testing.assert_array_equal(osum.cpu().numpy(), es, err_msg=f"layout {layout}")
+    q.set_aggregation(-1)
+    shape, threads, rpt, ms = q.autotune("rowwise", osum, ocnt)
+    _say(f"  autotune -> {shape} ({ms:.3f} ms)")
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
+
+
 def test_large_dimensions_spill_to_hbm(ctx, ncpu):
     """Dimension vectors larger than LDS are probed from HBM."""
     rng = np.random.default_rng(11)
