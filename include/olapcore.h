@@ -251,6 +251,84 @@ olap_status olap_batch_graph_launch(olap_batch b, void *stream);
 double olap_batch_expected_events(olap_batch b);
 olap_status olap_batch_destroy(olap_batch b);
 
+/* ---- serving: continuous batching over resident datasets ----------------- */
+/* No counterpart in the reference. A server keeps star schemas resident in
+ * HBM and runs `streams` worker threads, each with its own context and
+ * non-blocking HIP stream. An idle worker takes the oldest pending request,
+ * admits further pending requests of the same dataset in arrival order while
+ * their device bytes fit the free budget (at most max_batch), and runs them
+ * as one fused batched scan; requests arriving meanwhile are admitted by the
+ * next idle worker, so batches overlap on the device. Results are exact:
+ * every request gets the same sums and counts alone or in any mix. */
+typedef struct {
+    int32_t max_batch;       /* queries per fused scan, 1..64 */
+    int32_t streams;         /* concurrent batches (worker threads), 1..8 */
+    int64_t memory_budget;   /* device bytes admitted requests may hold */
+} olap_server_config;
+
+typedef struct {
+    int64_t submitted, completed, failed, rejected, cancelled, batches;
+    double mean_batch;        /* requests per fused scan */
+    double p50_us, p99_us, max_us, mean_us;  /* submit-to-result latency */
+    int64_t bytes_in_use, bytes_peak;
+} olap_server_stats;
+
+typedef struct olap_server_s *olap_server;
+
+olap_status olap_server_create(olap_ctx ctx, const olap_server_config *cfg, olap_server *out);
+/* A batch executor: runs queries[0..n) of one dataset into the concatenated
+ * outputs sum/count (query i owns [offsets[i], offsets[i+1]), zeroed by the
+ * executor). The library's own executor is the fused GPU scan (olap_batch);
+ * a server made with olap_server_create_host calls `exec` instead, from its
+ * worker threads, on host memory -- the seam that lets the admission,
+ * batching, failure and cancellation logic run without a device. */
+typedef olap_status (*olap_server_exec_fn)(void *user, int32_t dataset,
+                                           const olap_batch_query *queries, int32_t n,
+                                           const int64_t *offsets, int64_t *sum, int64_t *count);
+olap_status olap_server_create_host(const olap_server_config *cfg, olap_server_exec_fn exec,
+                                    void *exec_user, olap_server *out);
+/* Output staging allocator (device memory for the GPU executor, host memory
+ * for a host executor). A NULL return fails that batch's requests with
+ * OLAP_E_NOMEM; the server keeps serving. Set before the first submit. */
+typedef void *(*olap_server_alloc_fn)(void *user, size_t bytes);
+typedef void (*olap_server_free_fn)(void *user, void *ptr);
+olap_status olap_server_set_allocator(olap_server s, olap_server_alloc_fn alloc_fn,
+                                      olap_server_free_fn free_fn, void *user);
+/* register a resident star schema (pointers stay owned by the caller) */
+olap_status olap_server_add_dataset(olap_server s, const olap_star *star, int32_t *dataset_id);
+/* Queue one query. sum_host / count_host receive group_nums int64 each and
+ * must stay valid until olap_server_wait returns for the ticket. Returns
+ * OLAP_E_CAPACITY when the query alone exceeds the memory budget. */
+olap_status olap_server_submit(olap_server s, int32_t dataset, const olap_batch_query *q,
+                               int64_t *sum_host, int64_t *count_host, uint64_t *ticket);
+/* Wait up to timeout_us (negative: forever) for a ticket; OLAP_E_BUSY on
+ * timeout. On return *status holds the request's own status (OLAP_OK or the
+ * error of its batch), *latency_us (optional) its submit-to-result time, and
+ * the ticket is released. */
+olap_status olap_server_wait(olap_server s, uint64_t ticket, int64_t timeout_us, int32_t *status,
+                             double *latency_us);
+/* Stop (paused = 1) or resume (0) admission; batches already admitted finish.
+ * Requests submitted while paused are admitted together on resume. */
+olap_status olap_server_pause(olap_server s, int32_t paused);
+/* Withdraw a request that no worker has admitted yet (OLAP_E_BUSY otherwise). */
+olap_status olap_server_cancel(olap_server s, uint64_t ticket);
+olap_status olap_server_get_stats(olap_server s, olap_server_stats *out);
+olap_status olap_server_reset_stats(olap_server s);
+const char *olap_server_last_error(olap_server s);
+/* Stops the workers after their running batches; requests still pending are
+ * dropped, so call it once every wait has returned. */
+olap_status olap_server_destroy(olap_server s);
+/* The admission accounting and policy, as pure host functions: the device
+ * bytes a request holds (an int32 dimension vector per probed table plus its
+ * int64 sum and count outputs), and the batch an idle worker takes from n
+ * pending requests in arrival order -- the oldest request's dataset, arrival
+ * order, at most max_batch, skipping requests that would overflow
+ * budget_left; none when the oldest does not fit. Returns the number chosen
+ * (indices into the pending list written to chosen). */
+int64_t olap_server_request_bytes(const olap_star *star, const olap_batch_query *q);
+int32_t olap_server_plan(const int32_t *datasets, const int64_t *bytes, int32_t n,
+                         int64_t budget_left, int32_t max_batch, int32_t *chosen);
+
 /* ---- sub-step operators --------------------------------------------------- */
 /* starjoinalgo_rowwise / _cwm_sv / _cwm_dv (code/starjoin/test_starjoin_option.cpp:37-189):
  * rows passing all four int8 dimension vectors (c, s, p, d). result (device,

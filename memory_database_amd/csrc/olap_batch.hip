@@ -348,6 +348,58 @@ static hipError_t batch_launch(int cfg, int mode, BatchArgs &a, uint32_t dyn, in
 
 }  // namespace olap
 
+namespace olap {
+
+struct BatchWorkspace {
+    Stager stager;
+    unsigned long long *mask[4] = {nullptr, nullptr, nullptr, nullptr};  // [dim rows] per table
+    uint4 *rec[4] = {nullptr, nullptr, nullptr, nullptr};                 // [dim rows] per table
+    size_t dim_cap[4] = {0, 0, 0, 0};
+    BQDesc *desc[kNumBatchCfgs] = {};   // OLAP_BATCH_MAX descriptors per configuration
+    MaskPred *preds = nullptr;          // OLAP_BATCH_MAX predicates of one table
+    uint32_t *pass = nullptr;
+
+    hipError_t fit_table(int t, size_t rows)
+    {
+        rows = std::max<size_t>(rows, 1);
+        if (mask[t] && rows <= dim_cap[t]) return hipSuccess;
+        if (mask[t]) (void)hipFree(mask[t]);
+        if (rec[t]) (void)hipFree(rec[t]);
+        mask[t] = nullptr;
+        rec[t] = nullptr;
+        dim_cap[t] = 0;
+        hipError_t e = hipMalloc(&mask[t], 8 * rows);
+        if (e == hipSuccess) e = hipMalloc(&rec[t], 16 * rows);
+        if (e == hipSuccess) dim_cap[t] = rows;
+        return e;
+    }
+    hipError_t fit_fixed()
+    {
+        hipError_t e = hipSuccess;
+        for (int c = 0; c < kNumBatchCfgs && e == hipSuccess; c++)
+            if (!desc[c]) e = hipMalloc(&desc[c], sizeof(BQDesc) * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !preds) e = hipMalloc(&preds, sizeof(MaskPred) * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * OLAP_BATCH_MAX);
+        return e;
+    }
+    ~BatchWorkspace()
+    {
+        for (int t = 0; t < 4; t++) {
+            if (mask[t]) (void)hipFree(mask[t]);
+            if (rec[t]) (void)hipFree(rec[t]);
+        }
+        for (int c = 0; c < kNumBatchCfgs; c++)
+            if (desc[c]) (void)hipFree(desc[c]);
+        if (preds) (void)hipFree(preds);
+        if (pass) (void)hipFree(pass);
+    }
+};
+
+BatchWorkspace *batch_workspace_create() { return new BatchWorkspace(); }
+void batch_workspace_destroy(BatchWorkspace *ws) { delete ws; }
+
+}  // namespace olap
+
 using namespace olap;
 
 struct olap_batch_s {
@@ -356,9 +408,9 @@ struct olap_batch_s {
     int nq = 0;
     std::vector<olap_batch_query> queries;
     std::vector<int64_t> offsets;
-    unsigned long long *dmask[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint4 *drec[4] = {nullptr, nullptr, nullptr, nullptr};
-    BQDesc *ddesc[kNumBatchCfgs] = {};
+    BatchWorkspace *ws = nullptr;       // masks, descriptors, staging scratch
+    bool ws_owned = false;
+    hipStream_t setup_s = nullptr;      // stream of mask builds and descriptor uploads
     std::vector<double> sel;
     double events = 0.0;
     uint32_t used = 0;
@@ -370,8 +422,7 @@ struct olap_batch_s {
     int grid[kNumBatchCfgs] = {};
     uint32_t stage_lds[kNumBatchCfgs] = {};  // word offset of the stage region
     bool planned[kNumBatchCfgs] = {};
-    // staged aggregation scratch, set up for one configuration's grid
-    Stager stager;
+    // the workspace's staging scratch is set up for one configuration's grid
     int stager_cfg = -1;
     std::vector<unsigned long long> out_off;
     std::vector<uint32_t> gsz;
@@ -468,16 +519,14 @@ olap_status plan_config(olap_batch b, int c)
             d.M[s] = d.g[s] ? (UINT64_MAX / d.g[s] + 1ull) : 0ull;
         }
     }
-    if (!b->ddesc[c]) {
-        hipError_t e = hipMalloc(&b->ddesc[c], sizeof(BQDesc) * std::max(1, b->nq));
-        if (e != hipSuccess) return bhip(ctx, e, "batch descriptors");
-    }
-    hipError_t e = hipMemcpy(b->ddesc[c], desc.data(), sizeof(BQDesc) * b->nq, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(b->ws->desc[c], desc.data(), sizeof(BQDesc) * b->nq, hipMemcpyHostToDevice,
+                                  b->setup_s);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->setup_s);  // desc is read by the copy
     if (e != hipSuccess) return bhip(ctx, e, "batch descriptors");
     for (int t = 0; t < 4; t++) {
         a.fk[t] = b->star.fk[t];
-        a.gmask[t] = b->dmask[t];
-        a.grec[t] = b->drec[t];
+        a.gmask[t] = b->ws->mask[t];
+        a.grec[t] = b->ws->rec[t];
         a.gattr[t] = b->star.attr[t];
         a.dim_rows[t] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
         a.nouse[t] = b->nouse[t];
@@ -487,7 +536,7 @@ olap_status plan_config(olap_batch b, int c)
     a.rows = (uint64_t)b->star.rows;
     a.used = b->used;
     a.nq = (uint32_t)b->nq;
-    a.desc = b->ddesc[c];
+    a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);
     e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, &b->grid[c]);
@@ -496,11 +545,11 @@ olap_status plan_config(olap_batch b, int c)
     return OLAP_OK;
 }
 
-olap_status stager_for(olap_batch b, int c)
+olap_status stager_for(olap_batch b, int c, hipStream_t s)
 {
     if (b->stager_cfg == c) return OLAP_OK;
-    hipError_t e = b->stager.setup(b->out_off.data(), b->gsz.data(), b->nq, (uint32_t)b->grid[c],
-                                   b->star.m2 != nullptr);
+    hipError_t e = b->ws->stager.setup(b->out_off.data(), b->gsz.data(), b->nq, (uint32_t)b->grid[c],
+                                       b->star.m2 != nullptr, s);
     if (e != hipSuccess) return bhip(b->ctx, e, "stager setup");
     b->stager_cfg = c;
     return OLAP_OK;
@@ -524,9 +573,9 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
         e = batch_launch(c, AGG_DIRECT, a, b->dyn[c], b->grid[c], s);
         return e == hipSuccess ? OLAP_OK : bhip(ctx, e, "batch scan");
     }
-    olap_status st = stager_for(b, c);
+    olap_status st = stager_for(b, c, s);
     if (st != OLAP_OK) return st;
-    Stager &S = b->stager;
+    Stager &S = b->ws->stager;
     const uint32_t cur = b->stage_lds[c];
     const uint32_t start = cur + up16(b->nparts * 4u) / 4u;
     a.stage = S.args(cur, start);
@@ -548,13 +597,11 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
 
 }  // namespace
 
-extern "C" {
-
-olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
-                              int32_t nq, olap_batch *out)
+olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
+                                  int32_t nq, BatchWorkspace *ws, hipStream_t strm, olap_batch *out)
 {
     if (!ctx) return OLAP_E_INVALID;
-    if (!star || !queries || !out || nq < 1 || nq > OLAP_BATCH_MAX)
+    if (!star || !queries || !out || !ws || nq < 1 || nq > OLAP_BATCH_MAX)
         return bfail(ctx, OLAP_E_INVALID, "batch_create: 1..64 queries and a star are required");
     *out = nullptr;
     if (star->rows < 0 || star->rows > (int64_t)UINT32_MAX - 65536)
@@ -565,6 +612,8 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
         return bfail(ctx, OLAP_E_INVALID, "batch_create: fact columns must be 16-byte aligned");
     olap_batch b = new olap_batch_s();
     b->ctx = ctx;
+    b->ws = ws;
+    b->setup_s = strm;
     b->star = *star;
     b->nq = nq;
     b->queries.assign(queries, queries + nq);
@@ -612,10 +661,9 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
             if (!uses) b->nouse[t] |= 1ull << q;
         }
     }
-    // masks and their pass statistics
-    hipStream_t s0 = nullptr;
+    // masks and their pass statistics, on the setup stream
     b->sel.assign((size_t)nq, 1.0);
-    hipError_t e = hipSuccess;
+    hipError_t e = ws->fit_fixed();
     for (int t = 0; t < 4 && e == hipSuccess; t++) {
         if (!((b->used >> t) & 1u)) continue;
         std::vector<MaskPred> preds;
@@ -627,29 +675,25 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
                     pq.push_back(q);
                 }
         const uint32_t D = (uint32_t)star->dim_rows[t];
-        MaskPred *dp = nullptr;
-        uint32_t *dpass = nullptr;
-        e = hipMalloc(&b->dmask[t], 8 * (size_t)std::max<uint32_t>(D, 1));
-        if (e == hipSuccess) e = hipMalloc(&b->drec[t], 16 * (size_t)std::max<uint32_t>(D, 1));
-        if (e == hipSuccess) e = hipMalloc(&dp, sizeof(MaskPred) * preds.size());
-        if (e == hipSuccess) e = hipMalloc(&dpass, 4 * preds.size());
-        if (e == hipSuccess) e = hipMemcpy(dp, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = launch_zero_u32(dpass, preds.size(), s0);
+        e = ws->fit_table(t, D);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ws->preds, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice,
+                               strm);
+        if (e == hipSuccess) e = launch_zero_u32(ws->pass, preds.size(), strm);
         if (e == hipSuccess && D) {
             const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
-            k_batch_masks<<<grid, 256, 0, s0>>>(star->filter[t], star->attr[t], D, dp, (int)preds.size(), b->nouse[t],
-                                                b->dmask[t], b->drec[t], dpass);
+            k_batch_masks<<<grid, 256, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds, (int)preds.size(),
+                                                  b->nouse[t], ws->mask[t], ws->rec[t], ws->pass);
             e = hipGetLastError();
         }
         std::vector<uint32_t> pass(preds.size(), 0);
-        if (e == hipSuccess) e = hipMemcpy(pass.data(), dpass, 4 * preds.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpyAsync(pass.data(), ws->pass, 4 * preds.size(), hipMemcpyDeviceToHost, strm);
+        if (e == hipSuccess) e = hipStreamSynchronize(strm);
         for (size_t i = 0; i < preds.size() && e == hipSuccess; i++)
             b->sel[(size_t)pq[i]] *= D ? (double)pass[i] / (double)D : 0.0;
-        if (dp) (void)hipFree(dp);
-        if (dpass) (void)hipFree(dpass);
     }
     if (e != hipSuccess) {
-        olap_batch_destroy(b);
+        delete b;
         return bhip(ctx, e, "batch masks");
     }
     b->events = 0;
@@ -660,15 +704,30 @@ olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_ba
         b->nparts += (uint32_t)((queries[q].group_nums + kPartSlots - 1) / kPartSlots);
     }
     if (b->nparts > kMaxParts) {
-        olap_batch_destroy(b);
+        delete b;
         return bfail(ctx, OLAP_E_CAPACITY, "batch_create: too many aggregation partitions");
     }
     olap_status st = plan_config(b, b->cfg);
     if (st != OLAP_OK) {
-        olap_batch_destroy(b);
+        delete b;
         return st;
     }
     *out = b;
+    return OLAP_OK;
+}
+
+extern "C" {
+
+olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
+                              int32_t nq, olap_batch *out)
+{
+    BatchWorkspace *ws = batch_workspace_create();
+    olap_status st = batch_create_in(ctx, star, queries, nq, ws, nullptr, out);
+    if (st != OLAP_OK) {
+        batch_workspace_destroy(ws);
+        return st;
+    }
+    (*out)->ws_owned = true;
     return OLAP_OK;
 }
 
@@ -803,12 +862,7 @@ olap_status olap_batch_destroy(olap_batch b)
     if (!b) return OLAP_E_INVALID;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (int t = 0; t < 4; t++) {
-        if (b->dmask[t]) (void)hipFree(b->dmask[t]);
-        if (b->drec[t]) (void)hipFree(b->drec[t]);
-    }
-    for (int c = 0; c < kNumBatchCfgs; c++)
-        if (b->ddesc[c]) (void)hipFree(b->ddesc[c]);
+    if (b->ws_owned) batch_workspace_destroy(b->ws);
     delete b;
     return OLAP_OK;
 }

@@ -293,7 +293,7 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     Stager &S = *q->stager;
     if (S.nparts != nparts || S.nwg != grid) {
         const unsigned long long off0 = 0;
-        HIPC(ctx, S.setup(&off0, &G, 1, grid, q->cols.m2 != nullptr));
+        HIPC(ctx, S.setup(&off0, &G, 1, grid, q->cols.m2 != nullptr, s));
     }
     ScanArgs a;
     fill_args(a, q, L, sum, count);

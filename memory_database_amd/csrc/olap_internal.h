@@ -163,4 +163,16 @@ hipError_t launch_sum_u64(const unsigned long long *v, int64_t n, unsigned long 
 hipError_t launch_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t,
                            unsigned long long *out, int num_cus, hipStream_t s);
 
+// Scratch of batched scans that outlives them: query masks, descriptors and
+// the staged-aggregation buffers, grown on demand and never shrunk, so a
+// caller running batch after batch (the server's workers) stops allocating
+// and freeing device memory once warm. One batch at a time may use it.
+struct BatchWorkspace;
+BatchWorkspace *batch_workspace_create();
+void batch_workspace_destroy(BatchWorkspace *ws);
+// olap_batch_create on a borrowed workspace, with the mask builds and
+// uploads ordered on stream s (the call synchronises s)
+olap_status batch_create_in(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
+                            int32_t nq, BatchWorkspace *ws, hipStream_t s, olap_batch *out);
+
 }  // namespace olap

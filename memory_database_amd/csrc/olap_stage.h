@@ -115,9 +115,11 @@ struct Stager {
     bool rec_wide = false;                      // rec holds uint4 records
     unsigned long long cap = 0;
 
-    // partitions of every query: out_off[q] (first output slot) and G[q]
+    // partitions of every query: out_off[q] (first output slot) and G[q];
+    // buffers only grow, so a Stager reused across scans stops allocating;
+    // the uploads and the clear are ordered on stream s (synchronises s)
     hipError_t setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg,
-                     bool two_measures);
+                     bool two_measures, hipStream_t s);
     // after the COUNT pass: the prefix sums (device only)
     hipError_t prefix(hipStream_t s);
     // the record total of the last COUNT pass (synchronises)
@@ -140,6 +142,7 @@ struct Stager {
 
   private:
     std::vector<uint32_t> qbase_;
+    size_t cap_[8] = {};   // element capacities of the eight scratch arrays
 };
 
 }  // namespace olap

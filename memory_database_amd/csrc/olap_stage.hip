@@ -177,18 +177,24 @@ __global__ __launch_bounds__(512) void k_stage_reduce(
     }
 }
 
+// grow-only device array: reallocated only when n exceeds its capacity
 template <typename T>
-hipError_t realloc_dev(T *&p, size_t n)
+hipError_t grow_dev(T *&p, size_t &cap, size_t n)
 {
+    n = std::max<size_t>(n, 1);
+    if (p && n <= cap) return hipSuccess;
     if (p) (void)hipFree(p);
     p = nullptr;
-    return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+    cap = 0;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e == hipSuccess) cap = n;
+    return e;
 }
 
 }  // namespace
 
 hipError_t Stager::setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg_,
-                         bool two)
+                         bool two, hipStream_t s)
 {
     qbase_.assign((size_t)nq, 0);
     std::vector<unsigned long long> pout;
@@ -207,20 +213,22 @@ hipError_t Stager::setup(const unsigned long long *out_off, const uint32_t *G, i
     two_measures = two;
     const size_t np = std::max<uint32_t>(nparts, 1);
     hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = realloc_dev(wg_counts, np * nwg);
-    if (e == hipSuccess) e = realloc_dev(wg_start, np * nwg);
-    if (e == hipSuccess) e = realloc_dev(part_tot, np);
-    if (e == hipSuccess) e = realloc_dev(part_begin, np + 1);
-    if (e == hipSuccess) e = realloc_dev(item_begin, np + 1);
-    if (e == hipSuccess) e = realloc_dev(work, 4);
-    if (e == hipSuccess) e = realloc_dev(part_out, np);
-    if (e == hipSuccess) e = realloc_dev(part_slots, np);
+    if (e == hipSuccess) e = grow_dev(wg_counts, cap_[0], np * nwg);
+    if (e == hipSuccess) e = grow_dev(wg_start, cap_[1], np * nwg);
+    if (e == hipSuccess) e = grow_dev(part_tot, cap_[2], np);
+    if (e == hipSuccess) e = grow_dev(part_begin, cap_[3], np + 1);
+    if (e == hipSuccess) e = grow_dev(item_begin, cap_[4], np + 1);
+    if (e == hipSuccess) e = grow_dev(work, cap_[5], 4);
+    if (e == hipSuccess) e = grow_dev(part_out, cap_[6], np);
+    if (e == hipSuccess) e = grow_dev(part_slots, cap_[7], np);
     if (e == hipSuccess && nparts) {
-        e = hipMemcpy(part_out, pout.data(), nparts * sizeof(unsigned long long), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(part_out, pout.data(), nparts * sizeof(unsigned long long), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
-            e = hipMemcpy(part_slots, pslots.data(), nparts * sizeof(uint32_t), hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(part_slots, pslots.data(), nparts * sizeof(uint32_t), hipMemcpyHostToDevice, s);
     }
-    if (e == hipSuccess) e = launch_zero_u32(work, 4, nullptr);
+    if (e == hipSuccess) e = launch_zero_u32(work, 4, s);
+    // the host vectors are read by the copies: wait for them
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
 }
 
@@ -327,6 +335,7 @@ void Stager::release()
     rec = nullptr;
     cap = 0;
     nparts = 0;
+    for (size_t &c : cap_) c = 0;
 }
 
 }  // namespace olap

@@ -92,8 +92,33 @@ class Star(C.Structure):
                 ("rows", C.c_int64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32),
+                ("streams", C.c_int32),
+                ("memory_budget", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("rejected", C.c_int64),
+                ("cancelled", C.c_int64), ("batches", C.c_int64),
+                ("mean_batch", C.c_double),
+                ("p50_us", C.c_double), ("p99_us", C.c_double), ("max_us", C.c_double),
+                ("mean_us", C.c_double),
+                ("bytes_in_use", C.c_int64), ("bytes_peak", C.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _P = C.c_void_p
 _I64P = C.POINTER(C.c_int64)
+
+# olap_server_exec_fn / olap_server_alloc_fn / olap_server_free_fn
+SERVER_EXEC_FN = C.CFUNCTYPE(C.c_int, _P, C.c_int32, C.POINTER(BatchQuery), C.c_int32, _I64P, _I64P,
+                             _I64P)
+SERVER_ALLOC_FN = C.CFUNCTYPE(_P, _P, C.c_size_t)
+SERVER_FREE_FN = C.CFUNCTYPE(None, _P, _P)
 
 # name -> (restype, argtypes); the export test checks this list against the header
 SIGNATURES = {
@@ -145,6 +170,23 @@ SIGNATURES = {
     "olap_batch_graph_launch": (C.c_int, [_P, _P]),
     "olap_batch_expected_events": (C.c_double, [_P]),
     "olap_batch_destroy": (C.c_int, [_P]),
+    "olap_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "olap_server_create_host": (C.c_int, [C.POINTER(ServerConfig), SERVER_EXEC_FN, _P,
+                                          C.POINTER(_P)]),
+    "olap_server_set_allocator": (C.c_int, [_P, SERVER_ALLOC_FN, SERVER_FREE_FN, _P]),
+    "olap_server_last_error": (C.c_char_p, [_P]),
+    "olap_server_add_dataset": (C.c_int, [_P, C.POINTER(Star), C.POINTER(C.c_int32)]),
+    "olap_server_submit": (C.c_int, [_P, C.c_int32, C.POINTER(BatchQuery), _P, _P,
+                                     C.POINTER(C.c_uint64)]),
+    "olap_server_wait": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]),
+    "olap_server_pause": (C.c_int, [_P, C.c_int32]),
+    "olap_server_cancel": (C.c_int, [_P, C.c_uint64]),
+    "olap_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "olap_server_reset_stats": (C.c_int, [_P]),
+    "olap_server_destroy": (C.c_int, [_P]),
+    "olap_server_request_bytes": (C.c_int64, [C.POINTER(Star), C.POINTER(BatchQuery)]),
+    "olap_server_plan": (C.c_int32, [_P, _P, C.c_int32, C.c_int64, C.c_int32, _P]),
     "olap_legacy_count": (C.c_int32, [_P, C.c_int64]),
     "olap_starjoin": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, _I64P, _P]),
     "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
@@ -172,9 +214,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 
 def _ptr(t) -> Optional[int]:
+    """Address of a torch tensor's or numpy array's data."""
     if t is None:
         return None
-    return t.data_ptr()
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
 
 
 def make_plan(groups: Sequence[int], bitmap: Sequence[int], sele: Sequence[float]) -> Plan:
@@ -198,6 +243,23 @@ def plan_from(orders: Sequence[int], factor: Sequence[int], group_nums: int) -> 
         p.factor[i] = factor[i]
     p.group_nums = group_nums
     return p
+
+
+def server_request_bytes(star: Star, query: BatchQuery) -> int:
+    """Device bytes the server charges a request against its memory budget."""
+    return int(load().olap_server_request_bytes(C.byref(star), C.byref(query)))
+
+
+def server_plan(datasets, request_bytes, budget_left: int, max_batch: int):
+    """The batch an idle server worker admits from pending requests (arrival
+    order): indices into the pending list."""
+    import numpy as np
+    ds = np.ascontiguousarray(datasets, dtype=np.int32)
+    by = np.ascontiguousarray(request_bytes, dtype=np.int64)
+    out = np.zeros(max(len(ds), 1), dtype=np.int32)
+    n = load().olap_server_plan(ds.ctypes.data, by.ctypes.data, len(ds), budget_left, max_batch,
+                                out.ctypes.data)
+    return [int(i) for i in out[:n]]
 
 
 def legacy_count(sums) -> int:
@@ -313,19 +375,23 @@ class Context:
         s = Star()
         for t in range(MAX_DIMS):
             if filters[t] is not None:
-                s.filter[t] = filters[t].data_ptr()
-                s.dim_rows[t] = filters[t].numel()
+                s.filter[t] = _ptr(filters[t])
+                s.dim_rows[t] = filters[t].shape[0]
             if attrs[t] is not None:
-                s.attr[t] = attrs[t].data_ptr()
+                s.attr[t] = _ptr(attrs[t])
             if fks[t] is not None:
-                s.fk[t] = fks[t].data_ptr()
+                s.fk[t] = _ptr(fks[t])
                 if rows is None:
-                    rows = fks[t].numel()
+                    rows = fks[t].shape[0]
         s.m1 = _ptr(m1)
         s.m2 = _ptr(m2)
         s.rows = rows if rows is not None else 0
         s._keep = (list(filters), list(attrs), list(fks), m1, m2)
         return s
+
+    def server(self, max_batch: int = 64, streams: int = 2,
+               memory_budget: int = 8 << 30) -> "Server":
+        return Server(self, max_batch, streams, memory_budget)
 
     def batch(self, star: Star, queries) -> "Batch":
         return Batch(self, star, queries)
@@ -484,6 +550,123 @@ class Batch:
         if self.h:
             self.ctx.lib.olap_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """olap_server: continuous batching of submitted queries over resident
+    star schemas. submit() returns a ticket; wait() returns (sum, count) as
+    int64 numpy arrays of group_nums entries.
+
+    Server(ctx, ...) runs batches with the library's fused GPU scan;
+    Server.host(executor, ...) runs them with a Python executor
+    `executor(dataset, queries, offsets, sum, count) -> status` over numpy
+    arrays (no device involved)."""
+
+    def __init__(self, ctx: Optional[Context], max_batch: int = 64, streams: int = 2,
+                 memory_budget: int = 8 << 30, _executor=None):
+        self.lib = load()
+        self.ctx = ctx
+        self.h = None
+        self._stars = []
+        self._out = {}
+        self._keep = []
+        cfg = ServerConfig(max_batch, streams, memory_budget)
+        h = _P()
+        if _executor is None:
+            ctx.check(self.lib.olap_server_create(ctx.h, C.byref(cfg), C.byref(h)), "server_create")
+        else:
+            cb = SERVER_EXEC_FN(_executor)
+            self._keep.append(cb)
+            st = self.lib.olap_server_create_host(C.byref(cfg), cb, None, C.byref(h))
+            if st != OK:
+                raise OlapError(st, "server_create_host: max_batch 1..64, streams 1..8, budget > 0")
+        self.h = h
+
+    @classmethod
+    def host(cls, executor, max_batch: int = 64, streams: int = 2, memory_budget: int = 8 << 30):
+        import numpy as np
+
+        def trampoline(user, dataset, queries, n, offsets, osum, ocnt):
+            try:
+                off = np.ctypeslib.as_array(offsets, shape=(n + 1,)).copy()
+                total = max(int(off[-1]), 1)
+                s = np.ctypeslib.as_array(osum, shape=(total,))
+                c = np.ctypeslib.as_array(ocnt, shape=(total,))
+                qs = [queries[i] for i in range(n)]
+                return int(executor(int(dataset), qs, off, s, c))
+            except Exception:  # an executor exception fails its batch
+                return E_INVALID
+
+        return cls(None, max_batch, streams, memory_budget, _executor=trampoline)
+
+    def check(self, st: int, what: str):
+        if st != OK:
+            msg = self.lib.olap_server_last_error(self.h) if self.h else b""
+            raise OlapError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_allocator(self, alloc, free):
+        """alloc(bytes) -> address or 0; free(address)."""
+        a = SERVER_ALLOC_FN(lambda user, n: alloc(int(n)) or None)
+        f = SERVER_FREE_FN(lambda user, p: free(p))
+        self._keep += [a, f]
+        self.check(self.lib.olap_server_set_allocator(self.h, a, f, None), "server_set_allocator")
+
+    def add_dataset(self, star: Star) -> int:
+        i = C.c_int32()
+        self.check(self.lib.olap_server_add_dataset(self.h, C.byref(star), C.byref(i)),
+                   "server_add_dataset")
+        self._stars.append(star)
+        return i.value
+
+    def submit(self, dataset: int, query: BatchQuery) -> int:
+        import numpy as np
+        g = max(int(query.group_nums), 1)
+        s = np.empty(g, dtype=np.int64)
+        c = np.empty(g, dtype=np.int64)
+        t = C.c_uint64()
+        self.check(self.lib.olap_server_submit(self.h, dataset, C.byref(query), s.ctypes.data,
+                                               c.ctypes.data, C.byref(t)), "server_submit")
+        self._out[t.value] = (s, c, query)
+        return t.value
+
+    def wait(self, ticket: int, timeout_us: int = -1, with_latency: bool = False):
+        """(sum, count), plus the submit-to-result latency in us when asked.
+        OlapError(E_BUSY) on timeout; the request's own error otherwise."""
+        st = C.c_int32()
+        lat = C.c_double()
+        self.check(self.lib.olap_server_wait(self.h, ticket, timeout_us, C.byref(st), C.byref(lat)),
+                   "server_wait")
+        s, c, _ = self._out.pop(ticket)
+        if st.value != OK:
+            raise OlapError(st.value, f"request {ticket} failed in its batch")
+        return (s, c, lat.value) if with_latency else (s, c)
+
+    def pause(self, paused: bool = True):
+        self.check(self.lib.olap_server_pause(self.h, int(paused)), "server_pause")
+
+    def cancel(self, ticket: int):
+        self.check(self.lib.olap_server_cancel(self.h, ticket), "server_cancel")
+        self._out.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.check(self.lib.olap_server_get_stats(self.h, C.byref(st)), "server_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        self.check(self.lib.olap_server_reset_stats(self.h), "server_reset_stats")
+
+    def close(self):
+        if self.h:
+            self.lib.olap_server_destroy(self.h)
+            self.h = None
+            self._out.clear()
 
     def __del__(self):
         try:

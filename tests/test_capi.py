@@ -20,6 +20,7 @@ def declared_functions():
             continue
         text = open(os.path.join(INCLUDE, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"\btypedef\b[^;]*;", "", text)  # function-pointer types are not symbols
         names |= set(re.findall(r"\b(olap_[a-z0-9_]+)\s*\(", text))
     return names
 
