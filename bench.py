@@ -9,7 +9,14 @@ selectivities 0.5 / 0.2 / 0.04 / 0.2 and 7 / 25 / 1 / 5 groups (G = 875).
 
 Prints ONE JSON line (rank 0) with value = fact rows/s, the roofline of the
 scan kernel (algorithmic bytes = rows x 20 B per launch over the HIP-event
-average duration) and the CPU baseline timed on this host.
+average duration), the CPU baseline timed on this host, and beside them:
+  batched  64 mixed queries (1-4 dimensions, 1-50% selectivity per
+           dimension, 1..10^5 groups) over the same fact table in one fused
+           scan replayed as a hipGraph, against the same 64 queries run as
+           64 plain scans (queries/s of both forms and their ratio);
+  served   an open-loop Poisson stream of random 1-4-dimension queries over
+           resident datasets of 6M, 60M and 600M fact rows through the
+           continuous-batching server (queries/s, p50/p99 latency per rate).
 """
 from __future__ import annotations
 
@@ -109,8 +116,114 @@ def cpu_baseline(ds, sample_rows, log_fn=log):
     t = statistics.median(ts)
     log_fn(f"cpu baseline ({kind}, {cores} threads): {n} rows in {t * 1e3:.1f} ms")
     return {"value": n / t, "unit": "fact rows/s", "cores": cores, "kind": kind,
-            "sample": f"first {n} fact rows of the workload, OLAPcore_vwm_dv, "
+            "sample": f"first {n} fact rows of the workload, OLAPcore_vwm_dv "
+                      f"({'reference sources built -O2 (the reference builds -O0)' if kind == 'reference' else 'C restatement -O3'}), "
                       f"median of 5 runs after 1 warmup, {cores} pinned threads"}
+
+
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None, None
+    m = re.search(r"HBM traffic ([0-9.]+) GB/dispatch", text)
+    if not m:
+        return None, None
+    return float(m.group(1)) * 1e9, os.path.relpath(path, ROOT)
+
+
+def bench_batched(ctx, ds, stream, nq=64, seed=7):
+    """The batched form on the plain workload's fact table: nq seeded mixed
+    queries in one fused scan (autotuned, then captured and replayed as a
+    hipGraph) against the same queries as nq plain scans."""
+    import torch
+
+    from memory_database_amd import olapcore as oc
+    from memory_database_amd import workloads as wl
+
+    sh = stream.cuda_stream
+    bs = wl.DeviceBatchStar(ctx, ds.cfg, fact=ds, stream=sh)
+    torch.cuda.synchronize()
+    qs = wl.batch_queries(ds.cfg, nq, seed)
+    b = ctx.batch(bs.star, [oc.BatchQuery.make(*q) for q in qs])
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    cfg, threads, rpt, kib, staged, tune_ms = b.autotune(osum, ocnt, sh)
+    b.capture(osum, ocnt, sh)
+    _, ev = time_steps(lambda: b.launch(sh), 5, 1, stream)
+    batch_ms = statistics.mean(ev)
+    plain_ms = 0.0
+    for steps, factor, G in qs:
+        plan, cols = bs.plain_query(steps, factor, G, stream=sh)
+        q = ctx.query(plan, cols)
+        ps = torch.empty(G, dtype=torch.int64, device="cuda")
+        pc = torch.empty_like(ps)
+        _, e = time_steps(lambda: q.run("rowwise", ps, pc, stream=sh), 2, 1, stream)
+        plain_ms += statistics.mean(e)
+        q.close()
+    tables = set(t for steps, _, _ in qs for (t, _, _, _) in steps)
+    rows = ds.cfg.rows
+    union_bytes = rows * 4 * (len(tables) + 1)
+    per_query_bytes = sum(rows * 4 * (len(steps) + 1) for steps, _, _ in qs)
+    events = b.expected_events()
+    b.close()
+    log(f"batched: {nq} queries {batch_ms:.2f} ms (config {cfg}: {threads} threads, {rpt} rows/thread, "
+        f"{kib} KiB LDS tables, staged={staged}); plain {plain_ms:.2f} ms; x{plain_ms / batch_ms:.2f}")
+    return {
+        "queries": nq, "seed": seed, "batch_ms": batch_ms, "plain_ms_total": plain_ms,
+        "qps_batched": nq / (batch_ms / 1e3), "qps_plain": nq / (plain_ms / 1e3),
+        "speedup": plain_ms / batch_ms,
+        "matches_per_row": events,
+        "config": {"id": cfg, "threads": threads, "rows_per_thread": rpt, "lds_table_kib": kib,
+                   "staged": bool(staged)},
+        "union_bytes": union_bytes, "union_GBps": union_bytes / (batch_ms / 1e3) / 1e9,
+        "effective_GBps": per_query_bytes / (batch_ms / 1e3) / 1e9,
+        "timing": "hipGraph replay, HIP events, mean of 5 after 1 warmup; plain = sum of the "
+                  "64 plain row-wise scans, each mean of 2 after 1 warmup",
+    }
+
+
+def bench_served(ctx, ds, rates, seconds, streams=2):
+    """Open-loop Poisson arrivals through olap_server over resident datasets
+    of 6M, 60M and 600M fact rows (the 600M one is the plain workload's)."""
+    import torch
+
+    from memory_database_amd import olapcore as oc
+    from memory_database_amd import serving
+    from memory_database_amd import workloads as wl
+
+    stars = []
+    for c in serving.serve_configs():
+        if c.rows == ds.cfg.rows:
+            stars.append(wl.DeviceBatchStar(ctx, c, fact=ds))
+        else:
+            stars.append(wl.DeviceBatchStar(ctx, c))
+    torch.cuda.synchronize()
+    qs = [oc.BatchQuery.make(*t) for t in serving.templates(256, seed=3)]
+    srv = ctx.server(max_batch=64, streams=streams, memory_budget=32 << 30)
+    ids = [srv.add_dataset(s.star) for s in stars]
+    serving.run_open_loop(srv, ids, qs, serving.arrivals(200.0, 60, len(ids), len(qs), seed=99))
+    out = []
+    for r in rates:
+        n = int(max(100, min(r * seconds, 6000)))
+        srv.reset_stats()
+        res = serving.run_open_loop(srv, ids, qs, serving.arrivals(r, n, len(ids), len(qs), seed=1))
+        st = srv.stats()
+        row = {"offered_qps": round(res["offered_qps"], 1), "requests": n, "qps": round(res["qps"], 1),
+               "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+               "max_ms": round(res["max_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+               "failed": res["failed"]}
+        log(f"served at {r:g} q/s: {row}")
+        out.append(row)
+    srv.close()
+    return {"datasets_rows": [s.cfg.rows for s in stars], "templates": len(qs), "workers": streams,
+            "max_batch": 64, "rates": out,
+            "latency": "scheduled arrival to result on the host (submit lag + server time)"}
 
 
 def host_info():
@@ -138,12 +251,17 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other strategies / configs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served form")
+    ap.add_argument("--serve-rates", default="50,500,1000,2000,5000")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the star-join path is measured on a single "
-              "MI355X (see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+              "MI355X (multi-GPU sharding is a design, not a measured path here); use --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -204,10 +322,18 @@ def main():
         log(f"copy calibration: {nbytes} bytes read + written")
         del src, dst
 
+    if not args.no_batch:
+        extra["batched"] = bench_batched(ctx, ds, stream)
+    if not args.no_serve:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        extra["served"] = bench_served(ctx, ds, rates, args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, args.cpu_rows)
     q.close()
+    kernel = "k_rowwise" if args.strategy == "rowwise" else None
+    traffic, traffic_src = profiled_traffic(kernel, cfg.name) if kernel else (None, None)
 
     rec = {
         "metric": METRIC,
@@ -229,8 +355,13 @@ def main():
                    "launch": {"threads": shape[1], "rows_per_thread": shape[2]},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": alg_bytes, "kernel_ms": ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "traffic_GBps": traffic / (ms / 1e3) / 1e9 if traffic else None,
+                     "algorithmic_bytes": alg_bytes, "kernel_ms": ms,
+                     "note": "algorithmic bytes count every FK and measure byte; the scan's early "
+                             "exit loads later columns only for live rows, so HBM traffic is "
+                             "below them and frac can exceed 1"},
         "cpu_baseline": cpu,
         "passing_rows": passing,
         "wall_ms_per_step": wall_ms / args.steps,
