@@ -145,7 +145,7 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape);
 /* Aggregation layout of row-wise runs whose group table exceeds LDS:
  * -1 automatic (from the expected matches), 0 one global atomic pair per
  * matching row, 1 staged (count, scatter records to HBM, reduce per
- * 4096-slot partition in LDS; synchronises once per run to size the record
+ * 8192-slot partition in LDS; synchronises once per run to size the record
  * buffer). The autotuner also tries both. */
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,

@@ -25,13 +25,13 @@ SERVE_ROWS = (6_000_000, 60_000_000, 600_000_000)
 
 
 def serve_configs(rows: Sequence[int] = SERVE_ROWS) -> List[wl.StarConfig]:
-    return [wl.scaled(wl.C3, r, name=f"serve_4dim_{r // 1_000_000}M" if r >= 1_000_000
+    return [wl.scaled(wl.STAR_4DIM_600M, r, name=f"serve_4dim_{r // 1_000_000}M" if r >= 1_000_000
                       else f"serve_4dim_{r}") for r in rows]
 
 
 def templates(n: int = 256, seed: int = 3, max_groups: int = 100_000):
     """n query templates [(steps, factor, G)] over the star's four tables."""
-    return wl.batch_queries(wl.C3, n, seed, max_groups=max_groups)
+    return wl.batch_queries(wl.STAR_4DIM_600M, n, seed, max_groups=max_groups)
 
 
 @dataclass

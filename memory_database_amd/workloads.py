@@ -67,20 +67,20 @@ class StarConfig:
 
 # Plain 1-dim star-join group-by on the reference CPU path: 6M fact rows, FK
 # U[0,100000), measure U[1,100]; 100,000-row dimension, filter < 10, 16 groups.
-C1 = StarConfig("c1_1dim_6M", 6_000_000, [Dim(0, 100_000, 10, 16)], 1, 100)
+STAR_1DIM_6M = StarConfig("star_1dim_6M", 6_000_000, [Dim(0, 100_000, 10, 16)], 1, 100)
 
 # Plain 4-dim star-join group-by: 60M fact rows; dimensions date 2,556 /
 # customer 30,000 / part 200,000 / supplier 2,000 rows with selectivities
 # 0.5 / 0.2 / 0.04 / 0.2 and group cardinalities 7 / 25 / 1 / 5 (G = 875);
 # measure U[1000, 1000000].
-C2_DIMS = [Dim(3, 2_556, 50, 7), Dim(0, 30_000, 20, 25), Dim(2, 200_000, 4, 1),
+STAR_DIMS = [Dim(3, 2_556, 50, 7), Dim(0, 30_000, 20, 25), Dim(2, 200_000, 4, 1),
            Dim(1, 2_000, 20, 5)]
-C2 = StarConfig("c2_4dim_60M", 60_000_000, C2_DIMS, 1000, 1_000_000)
+STAR_4DIM_60M = StarConfig("star_4dim_60M", 60_000_000, STAR_DIMS, 1000, 1_000_000)
 
 # The same star at 600M fact rows (12 GB of fact columns).
-C3 = StarConfig("c3_4dim_600M", 600_000_000, C2_DIMS, 1000, 1_000_000)
+STAR_4DIM_600M = StarConfig("star_4dim_600M", 600_000_000, STAR_DIMS, 1000, 1_000_000)
 
-CONFIGS = {c.name: c for c in (C1, C2, C3)}
+CONFIGS = {c.name: c for c in (STAR_1DIM_6M, STAR_4DIM_60M, STAR_4DIM_600M)}
 
 
 def scaled(cfg: StarConfig, rows: int, name: Optional[str] = None, **kw) -> StarConfig:

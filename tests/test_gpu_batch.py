@@ -38,7 +38,7 @@ def oracle_query(host, steps, factor, G, ncpu):
 
 
 def make_batch(ctx, rows, nq, seed, max_groups=100_000):
-    cfg = wl.scaled(wl.C3, rows, name=f"batch_{rows}")
+    cfg = wl.scaled(wl.STAR_4DIM_600M, rows, name=f"batch_{rows}")
     ds = wl.DeviceBatchStar(ctx, cfg)
     qs = wl.batch_queries(cfg, nq, seed, max_groups=max_groups)
     bq = [oc.BatchQuery.make(*q) for q in qs]

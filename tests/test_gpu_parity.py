@@ -298,10 +298,10 @@ def test_invalid_inputs_rejected(ctx):
 
 
 def test_c1_full_size(ctx, ncpu):
-    """BASELINE config 1 at full size: 6M fact rows, one dimension."""
+    """The 1-dimension star at full size: 6M fact rows, one 100,000-row dimension."""
     import torch
 
-    cfg = wl.C1
+    cfg = wl.STAR_1DIM_6M
     _say(f"{cfg.name}: generating on device")
     ds = wl.DeviceStar(ctx, cfg)
     dimvecs, _, _, fks, m1, m2 = cpu_star(cfg)
@@ -323,10 +323,10 @@ def test_c1_full_size(ctx, ncpu):
 
 
 def test_c2_full_size(ctx, ncpu):
-    """BASELINE config 2 at full size: 60M fact rows, four dimensions, G=875."""
+    """The 4-dimension star at full size: 60M fact rows, four dimensions, G=875."""
     import torch
 
-    cfg = wl.C2
+    cfg = wl.STAR_4DIM_60M
     _say(f"{cfg.name}: generating on device")
     ds = wl.DeviceStar(ctx, cfg)
     dimvecs, _, _, fks, m1, m2 = cpu_star(cfg)

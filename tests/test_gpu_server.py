@@ -17,10 +17,10 @@ def _say(*a):
 
 
 def test_server_alone_equals_mix_equals_oracle(ctx, ncpu):
-    cfgs = [wl.scaled(wl.C3, r, name=f"srv_{r}") for r in (1_000_003, 300_007)]
+    cfgs = [wl.scaled(wl.STAR_4DIM_600M, r, name=f"srv_{r}") for r in (1_000_003, 300_007)]
     dstars = [wl.DeviceBatchStar(ctx, c) for c in cfgs]
     hosts = [host_star(c, c.rows) for c in cfgs]
-    tmpl = wl.batch_queries(wl.C3, 48, seed=21, max_groups=20_000)
+    tmpl = wl.batch_queries(wl.STAR_4DIM_600M, 48, seed=21, max_groups=20_000)
     rng = np.random.default_rng(4)
     ds = [int(x) for x in rng.integers(0, 2, len(tmpl))]
     qs = [oc.BatchQuery.make(*t) for t in tmpl]
@@ -57,9 +57,9 @@ def test_server_staged_batch_equals_plain_batch(ctx):
     per request as olap_batch run directly with the atomics layout."""
     import torch
 
-    cfg = wl.scaled(wl.C3, 40_000_000, name="srv_staged")
+    cfg = wl.scaled(wl.STAR_4DIM_600M, 40_000_000, name="srv_staged")
     dstar = wl.DeviceBatchStar(ctx, cfg)
-    tmpl = wl.batch_queries(wl.C3, 64, seed=33)
+    tmpl = wl.batch_queries(wl.STAR_4DIM_600M, 64, seed=33)
     qs = [oc.BatchQuery.make(*t) for t in tmpl]
     b = ctx.batch(dstar.star, qs)
     _say(f"expected matches {b.expected_events() * cfg.rows:.3g}")

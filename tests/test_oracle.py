@@ -26,7 +26,8 @@ needs_ref = pytest.mark.skipif(not REF.ok, reason="oracle/_ref not built (no ref
 # --------------------------------------------------------------------- plan
 
 def test_plan_known_case():
-    # BASELINE config 2 in the reference's table slots c, s, p, d
+    # the 60M-row 4-dimension star (dims 30,000 / 2,000 / 200,000 / 2,556 rows) in the
+    # reference's table slots c, s, p, d
     p = po.make_plan([25, 5, 1, 7], [1, 1, 1, 1], [0.2, 0.2, 0.04, 0.5])
     assert list(p.orders) == [2, 0, 1, 3]
     assert list(p.factor) == [875, 35, 7, 1]

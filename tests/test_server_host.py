@@ -27,7 +27,7 @@ class HostStar:
     """A small batch star on the host (the device generator's streams)."""
 
     def __init__(self, rows, seed):
-        cfg = wl.scaled(wl.C3, rows, seed=seed)
+        cfg = wl.scaled(wl.STAR_4DIM_600M, rows, seed=seed)
         self.cfg = cfg
         self.filters, self.attrs, self.fks = [None] * 4, [None] * 4, [None] * 4
         for d in cfg.dims:
@@ -69,7 +69,7 @@ def stars():
 
 
 def queries(n, seed, max_groups=500):
-    return [oc.BatchQuery.make(*q) for q in wl.batch_queries(wl.C3, n, seed, max_groups=max_groups)]
+    return [oc.BatchQuery.make(*q) for q in wl.batch_queries(wl.STAR_4DIM_600M, n, seed, max_groups=max_groups)]
 
 
 # ---------------------------------------------------------------- policy
