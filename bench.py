@@ -137,6 +137,48 @@ def profiled_traffic(kernel: str, workload: str):
     return float(m.group(1)) * 1e9, os.path.relpath(path, ROOT)
 
 
+def bench_sweep(ctx, stream, rows, strategy="rowwise"):
+    """The plain scan across the sweep of the 4-dimension star (Zipf skew of
+    the foreign keys x total groups x per-dimension selectivity), each
+    configuration autotuned, then timed as the headline scan is."""
+    import torch
+
+    from memory_database_amd import workloads as wl
+
+    sh = stream.cuda_stream
+    out = []
+    fact = None
+    theta = None
+    for cfg in wl.sweep_configs(rows):
+        if cfg.zipf_theta != theta:
+            fact, theta = None, cfg.zipf_theta
+        with torch.cuda.stream(stream):
+            ds = wl.DeviceStar(ctx, cfg, stream=sh, fact=fact)
+        torch.cuda.synchronize()
+        fact = fact or ds
+        G = int(ds.plan.group_nums)
+        q = ctx.query(ds.plan, ds.cols)
+        osum = torch.empty(G, dtype=torch.int64, device="cuda")
+        ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+        shape = q.autotune(strategy, osum, ocnt, stream=sh)
+        _, ev = time_steps(lambda: q.run(strategy, osum, ocnt, stream=sh), 10, 2, stream)
+        ms = statistics.mean(ev)
+        alg = ds.fact_bytes()
+        row = {"theta": cfg.zipf_theta, "groups": G, "sele_pct": cfg.dims[0].pred_hi,
+               "ms": round(ms, 4), "rows_per_s": rows / (ms / 1e3),
+               "GBps": round(alg / (ms / 1e3) / 1e9, 1),
+               "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 3),
+               "passing_rows": int(ocnt.sum().item()),
+               "launch": [shape[1], shape[2]]}
+        log(f"sweep {cfg.name}: {row}")
+        out.append(row)
+        q.close()
+        del ds, osum, ocnt
+    return {"rows": rows, "strategy": strategy, "note": "every dimension at the same selectivity; "
+            "G split over date/customer/part/supplier; foreign keys Zipf(theta) over each "
+            "dimension's rows", "configs": out}
+
+
 def bench_batched(ctx, ds, stream, nq=64, seed=7):
     """The batched form on the plain workload's fact table: nq seeded mixed
     queries in one fused scan (autotuned, then captured and replayed as a
@@ -251,6 +293,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other strategies / configs")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the skew/groups/selectivity sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
     ap.add_argument("--serve-rates", default="50,500,1000,2000,5000")
@@ -324,6 +367,8 @@ def main():
 
     if not args.no_batch:
         extra["batched"] = bench_batched(ctx, ds, stream)
+    if not args.no_sweep:
+        extra["sweep"] = bench_sweep(ctx, stream, cfg.rows, args.strategy)
     if not args.no_serve:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         extra["served"] = bench_served(ctx, ds, rates, args.serve_seconds)

@@ -88,18 +88,31 @@ Layout plan_layout(const olap_query_s *q, bool with_steps, bool with_table, bool
     }
     uint32_t cap = kLdsBytes - (with_list ? kLdsListBytes : 0u) - 16u - reserve;
     uint32_t cur = 0;  // bytes
+    const uint64_t G = (uint64_t)q->plan.group_nums;
+    // With many matches (>= 1 in 256 rows) a group table that fits LDS on its
+    // own takes precedence over the dimension vectors: each match would
+    // otherwise cost a pair of global atomics, while a vector left in HBM
+    // costs an L2 gather only for the rows that reach its step.
+    uint32_t step_cap = cap;
+    if (with_steps && with_table) {
+        uint32_t steps_bytes = 0;
+        for (int s = 0; s < nd; s++)
+            if (only_step < 0 || s == only_step) steps_bytes += align_up(q->steps[s].nwords * 4u, 16u);
+        const uint64_t tb = align_up((uint32_t)std::min<uint64_t>(G * 12u, 1u << 30), 16u);
+        if (steps_bytes + tb > cap && tb <= cap && q->est_events * 256.0 >= (double)q->cols.rows)
+            step_cap = cap - (uint32_t)tb;
+    }
     if (with_steps) {
         for (int s = 0; s < nd; s++) {
             if (only_step >= 0 && s != only_step) continue;
             const uint32_t bytes = align_up(q->steps[s].nwords * 4u, 16u);
-            if (cur + bytes <= cap) {
+            if (cur + bytes <= step_cap) {
                 L.st[s].in_lds = 1;
                 L.st[s].lds_off = cur / 4u;
                 cur += bytes;
             }
         }
     }
-    const uint64_t G = (uint64_t)q->plan.group_nums;
     if (with_table) {
         const uint64_t tb = G * 12u;
         const uint32_t off = align_up(cur, 16u);

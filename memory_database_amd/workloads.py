@@ -82,6 +82,29 @@ STAR_4DIM_600M = StarConfig("star_4dim_600M", 600_000_000, STAR_DIMS, 1000, 1_00
 
 CONFIGS = {c.name: c for c in (STAR_1DIM_6M, STAR_4DIM_60M, STAR_4DIM_600M)}
 
+# Sweep of the 4-dimension star: Zipf skew of every foreign key, total group
+# count G (split over date / customer / part / supplier) and the selectivity
+# of every dimension predicate.
+SWEEP_THETA = (0.0, 0.5, 1.0)
+SWEEP_GROUPS = {100: (4, 25, 1, 1), 10_000: (10, 100, 1, 10), 1_000_000: (10, 100, 10, 100)}
+SWEEP_SELE = (1, 10, 50)   # percent, every dimension
+
+
+def sweep_configs(rows: int = 600_000_000, thetas=SWEEP_THETA, groups=tuple(SWEEP_GROUPS),
+                  seles=SWEEP_SELE):
+    """Star configurations of the sweep, grouped so that configurations with
+    the same theta are adjacent (they share the fact columns)."""
+    out = []
+    for th in thetas:
+        for G in groups:
+            gd, gc, gp, gs = SWEEP_GROUPS[G]
+            for sel in seles:
+                dims = [Dim(3, 2_556, sel, gd), Dim(0, 30_000, sel, gc), Dim(2, 200_000, sel, gp),
+                        Dim(1, 2_000, sel, gs)]
+                name = f"sweep_4dim_{rows // 1_000_000}M_theta{th:g}_G{G}_sel{sel}"
+                out.append(StarConfig(name, rows, dims, 1000, 1_000_000, zipf_theta=th))
+    return out
+
 
 def scaled(cfg: StarConfig, rows: int, name: Optional[str] = None, **kw) -> StarConfig:
     d = dict(cfg.__dict__)
@@ -95,7 +118,9 @@ class DeviceStar:
     """A StarConfig materialised in HBM (torch tensors on the ROCm device)."""
 
     def __init__(self, ctx, cfg: StarConfig, device: str = "cuda", stream=None,
-                 code_bytes: int = 1):
+                 code_bytes: int = 1, fact: Optional["DeviceStar"] = None):
+        """fact: a DeviceStar of the same rows, seed, dimension sizes and skew
+        whose foreign keys and measures are reused instead of regenerated."""
         import torch
 
         from . import olapcore as oc
@@ -118,6 +143,9 @@ class DeviceStar:
             ctx.build_dimvec(f, g, d.pred_lo, d.pred_hi, d.bitmap, dv,
                              group_mod=d.groups if code_bytes == 1 else 0, stream=stream)
             self.filters[d.slot], self.groups[d.slot], self.dimvecs[d.slot] = f, g, dv
+            if fact is not None:
+                self.fks[d.slot] = fact.fks[d.slot]
+                continue
             fk = torch.empty(cfg.rows, dtype=i32, device=device)
             if cfg.zipf_theta > 0.0:
                 thr = torch.from_numpy(oc.zipf_table(d.rows, cfg.zipf_theta).view("int64")).to(device)
@@ -126,12 +154,16 @@ class DeviceStar:
             else:
                 ctx.gen_uniform(fk, cfg.seed, 300 + d.slot, 0, d.rows, stream=stream)
             self.fks[d.slot] = fk
-        self.m1 = torch.empty(cfg.rows, dtype=i32, device=device)
-        ctx.gen_uniform(self.m1, cfg.seed, 400, cfg.m_lo, cfg.m_hi + 1, stream=stream)
+        if fact is not None:
+            self.m1 = fact.m1
+        else:
+            self.m1 = torch.empty(cfg.rows, dtype=i32, device=device)
+            ctx.gen_uniform(self.m1, cfg.seed, 400, cfg.m_lo, cfg.m_hi + 1, stream=stream)
         self.m2 = None
         if cfg.two_measures:
             self.m2 = torch.empty(cfg.rows, dtype=i32, device=device)
             ctx.gen_uniform(self.m2, cfg.seed, 401, cfg.m_lo, cfg.m_hi + 1, stream=stream)
+        self._zipf = zipf_thr  # generator tables stay alive with the columns
         self.plan = oc.make_plan(cfg.groups_by_slot(), cfg.bitmap_by_slot(), cfg.sele_by_slot())
         self.cols = oc.Context.columns(self.dimvecs, self.fks, self.m1, self.m2, rows=cfg.rows)
 
