@@ -66,6 +66,14 @@ struct BatchArgs {
     unsigned long long *gcnt;
     uint32_t *err;
     StageArgs stage;
+    // event-grouped kernel: LDS word offsets of the per-query counters
+    // (then cursors, starts and the total), the staged attributes [4][R],
+    // the staged measures [R] (m1, then m2) and the event list of ev_cap
+    uint32_t cnt_lds;
+    uint32_t row_lds;
+    uint32_t m_lds;
+    uint32_t ev_lds;
+    uint32_t ev_cap;
 };
 
 struct MaskPred {
@@ -80,6 +88,9 @@ struct MaskPred {
 typedef __attribute__((address_space(1))) unsigned long long g_u64;
 typedef __attribute__((address_space(3))) unsigned long long l_u64;
 typedef __attribute__((address_space(3))) uint32_t l_u32;
+// descriptors read with a wave-uniform query index go through the constant
+// address space, so they become scalar loads
+typedef __attribute__((address_space(4))) const struct BQDesc c_desc;
 
 __device__ __forceinline__ void b_atomic_dev(unsigned long long *p, unsigned long long v)
 {
@@ -280,33 +291,314 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
     }
 }
 
+// ---------------------------------------------------------- grouped events
+// Event-grouped form of the scan. The per-lane loop above leaves most lanes
+// idle (a wave waits for its busiest lane) and every lane reads a different
+// query's descriptor. Here a tile of R = 2 * BS rows goes through five steps:
+//  1. rows: each lane loads the foreign keys of its two rows, gathers the
+//     per-table query masks and attributes, and stages the attributes and
+//     measures of its rows in LDS;
+//  2. count: one LDS atomic per (row, query) match on the query's counter;
+//  3. offsets: exclusive sums over the queries;
+//  4. list: a cursor atomic per match places it in an LDS event list grouped
+//     by query;
+//  5. events: every query's segment is padded to whole waves, so each wave
+//     takes 64 events of one query: its descriptor is uniform, and a query
+//     of one aggregation partition takes one counter/cursor atomic per wave.
+// A tile with more matches than the list holds runs steps 4-5 in rounds, each
+// over a run of whole query segments.
+
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// the first set bit of w (non-zero) at or after bit r, wrapping around
+__device__ __forceinline__ uint32_t next_bit_from(unsigned long long w, uint32_t r)
+{
+    const unsigned long long rot = (w >> r) | (w << ((64u - r) & 63u));
+    return ((uint32_t)__ffsll(rot) - 1u + r) & 63u;
+}
+
+typedef int v2i __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int2 bload2(const int32_t *__restrict__ p, uint64_t r, uint64_t rows)
+{
+    if (r + 2 <= rows) {
+        const v2i v = __builtin_nontemporal_load((const v2i *)(p + r));
+        return make_int2(v.x, v.y);
+    }
+    int2 v = make_int2(0, 0);
+    if (r < rows) v.x = p[r];
+    return v;
+}
+
+template <int BS, int MODE>
+__global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
+{
+    constexpr uint32_t R = 2u * BS;
+    extern __shared__ __align__(16) uint32_t lds[];
+    stage_begin<MODE>(a.stage, lds);
+    for (int t = 0; t < 4; t++) {
+        if (a.mask_lds[t] != kNoLds) {
+            const uint32_t *src = (const uint32_t *)a.gmask[t];
+            for (uint32_t i = threadIdx.x; i < 2u * a.dim_rows[t]; i += BS) lds[a.mask_lds[t] + i] = src[i];
+        }
+        if (a.attr_lds[t] != kNoLds) {
+            const uint32_t *src = (const uint32_t *)a.gattr[t];
+            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = src[i];
+        }
+    }
+    {
+        const uint32_t *src = (const uint32_t *)a.desc;
+        const uint32_t n = a.nq * kDescWords;
+        for (uint32_t i = threadIdx.x; i < n; i += BS) lds[a.desc_lds + i] = src[i];
+    }
+    for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
+    if (threadIdx.x < 64u) lds[a.cnt_lds + threadIdx.x] = 0u;
+    __syncthreads();
+    l_u32 *L = (l_u32 *)lds;
+    l_u32 *qcnt = L + a.cnt_lds;          // [64] matches per query in the tile
+    l_u32 *qcur = qcnt + 64u;             // [64] list cursors per query
+    l_u32 *qreal = qcur + 64u;            // [64] matches per query (kept for the padding)
+    l_u32 *seg = qreal + 64u;             // [64] padded query starts, [64] the padded total
+    l_u32 *rattr = L + a.row_lds;         // [4][R]
+    l_u32 *rm = L + a.m_lds;              // [R] m1, then [R] m2
+    l_u32 *ev = L + a.ev_lds;             // [ev_cap]
+    stage_l_u32 *scur = (stage_l_u32 *)lds + a.stage.cur_lds;
+    const stage_l_u64 *sstart = (const stage_l_u64 *)((stage_l_u32 *)lds + a.stage.start_lds);
+    const BQDesc *D = a.desc;
+    const uint64_t rows = a.rows;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t r0 = tile * R + 2u * tid;
+        // 1. rows
+        unsigned long long al0 = r0 < rows ? ~0ull : 0ull;
+        unsigned long long al1 = r0 + 1 < rows ? ~0ull : 0ull;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            uint32_t at0 = 0u, at1 = 0u;
+            if ((a.used >> t) & 1u) {
+                const int2 k = bload2(a.fk[t], r0, rows);
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint32_t key = (uint32_t)(j ? k.y : k.x);
+                    unsigned long long &al = j ? al1 : al0;
+                    uint32_t &at = j ? at1 : at0;
+                    if (al) {
+                        if (key < a.dim_rows[t]) {
+                            if (a.mask_lds[t] != kNoLds) {
+                                const l_u64 *w = (const l_u64 *)(L + a.mask_lds[t]);
+                                al &= w[key];
+                                at = L[a.attr_lds[t] + key];
+                            } else {
+                                const uint4 rec = a.grec[t][key];
+                                al &= (unsigned long long)rec.x | ((unsigned long long)rec.y << 32);
+                                at = rec.z;
+                            }
+                        } else {
+                            al &= a.nouse[t];
+                        }
+                    }
+                }
+                rattr[(uint32_t)t * R + 2u * tid] = at0;
+                rattr[(uint32_t)t * R + 2u * tid + 1u] = at1;
+            }
+        }
+        if (al0 | al1) {
+            const int2 m = bload2(a.m1, r0, rows);
+            rm[2u * tid] = (uint32_t)m.x;
+            rm[2u * tid + 1u] = (uint32_t)m.y;
+            if (a.m2) {
+                const int2 m2 = bload2(a.m2, r0, rows);
+                rm[R + 2u * tid] = (uint32_t)m2.x;
+                rm[R + 2u * tid + 1u] = (uint32_t)m2.y;
+            }
+        }
+        // 2. count: one LDS atomic per match on its query's counter; each lane
+        // walks its bits from its own lane number on, so the lanes of one
+        // atomic instruction mostly hit different counters
+        for (int j = 0; j < 2; j++) {
+            for (unsigned long long w = j ? al1 : al0; w;) {
+                const uint32_t q = next_bit_from(w, lane);
+                w &= ~(1ull << q);
+                __hip_atomic_fetch_add(qcnt + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        __syncthreads();
+        // 3. offsets: every query's segment is padded to whole waves
+        if (tid < 64u) {
+            const uint32_t c = qcnt[tid];
+            const uint32_t padded = (c + 63u) & ~63u;
+            uint32_t incl = padded;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+                if (tid >= (uint32_t)d) incl += v;
+            }
+            seg[tid] = incl - padded;
+            qcur[tid] = incl - padded;
+            qreal[tid] = c;
+            qcnt[tid] = 0u;  // ready for the next tile (used again after two barriers)
+            if (tid == 63u) seg[64] = incl;
+        }
+        __syncthreads();
+        const uint32_t total = seg[64];
+        // rounds over runs of whole query segments that fit the list (one
+        // query's padded segment, at most R events, always fits)
+        for (uint32_t qa = 0; qa < 64u && seg[qa] < total;) {
+            uint32_t qb = qa + 1u;
+            while (qb < 64u && seg[qb + 1u] - seg[qa] <= a.ev_cap) qb++;
+            const uint32_t lo = seg[qa];
+            const uint32_t n = seg[qb] - lo;  // a multiple of 64, at most ev_cap
+            const unsigned long long qmask =
+                (qb == 64u ? ~0ull : ((1ull << qb) - 1ull)) & ~((1ull << qa) - 1ull);
+            // 4. list: a cursor atomic per match gives its slot in the
+            // grouped order (row in bits 0..15, query in bits 16..21); the
+            // padding slots of a segment hold row 0xffff
+            for (int j = 0; j < 2; j++) {
+                for (unsigned long long w = (j ? al1 : al0) & qmask; w;) {
+                    const uint32_t q = next_bit_from(w, lane);
+                    w &= ~(1ull << q);
+                    const uint32_t pos = __hip_atomic_fetch_add(qcur + q, 1u, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_WORKGROUP) - lo;
+                    ev[pos] = (2u * tid + (uint32_t)j) | (q << 16);
+                }
+            }
+            if (tid >= qa && tid < qb) {
+                const uint32_t end = seg[tid] + ((qreal[tid] + 63u) & ~63u);
+                for (uint32_t p = seg[tid] + qreal[tid]; p < end; p++) ev[p - lo] = 0xffffu | (tid << 16);
+            }
+            __syncthreads();
+            // 5. events: each wave takes whole 64-event chunks, all of one query
+            const uint32_t chunks = n >> 6;
+            for (uint32_t ch = tid >> 6; ch < chunks; ch += BS / 64u) {
+                const uint32_t e = ev[ch * 64u + lane];
+                const uint32_t q = __builtin_amdgcn_readfirstlane(e >> 16);
+                const bool valid = (e & 0xffffu) != 0xffffu;
+                const uint32_t row = valid ? (e & 0xffffu) : 0u;
+                const c_desc &d = ((const c_desc *)a.desc)[q];
+                const uint32_t G = d.G;
+                const uint32_t ns = d.nsteps;
+                // the step attributes are read together, then combined
+                uint32_t av[4];
+#pragma unroll
+                for (uint32_t st = 0; st < 4u; st++) av[st] = st < ns ? rattr[d.tab[st] * R + row] : 0u;
+                uint32_t gid = 0u;
+#pragma unroll
+                for (uint32_t st = 0; st < 4u; st++)
+                    if (st < ns)
+                        gid += (uint32_t)__umul64hi(d.M[st] * (uint64_t)av[st], (uint64_t)d.g[st]) * d.fac[st];
+                const bool ok = valid && gid < G;
+                if (valid && gid >= G)
+                    __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (d.lds_tab != kNoLds) {
+                    if (MODE != AGG_SCATTER && ok) {
+                        const unsigned long long mv = (unsigned long long)(
+                            (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
+                        b_atomic_lds(lds + d.lds_tab, G, gid, mv);
+                    }
+                } else if (MODE == AGG_DIRECT) {
+                    if (ok) {
+                        const unsigned long long mv = (unsigned long long)(
+                            (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
+                        b_atomic_dev(&a.gsum[d.out_off + gid], mv);
+                        b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+                    }
+                } else {
+                    const uint32_t part = d.pbase + (gid >> kPartBits);
+                    uint32_t pos = 0u;
+                    if (G <= kPartSlots) {
+                        // one partition: one counter / cursor atomic for the wave
+                        const unsigned long long act = __ballot(ok);
+                        if (act) {
+                            const int leader = __ffsll(act) - 1;
+                            uint32_t at = 0u;
+                            if ((int)lane == leader)
+                                at = __hip_atomic_fetch_add(scur + d.pbase, (uint32_t)__popcll(act),
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            pos = (uint32_t)__shfl((int)at, leader) + lane_rank(act);
+                        }
+                    } else if (ok) {
+                        pos = __hip_atomic_fetch_add(scur + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (MODE == AGG_SCATTER && ok) {
+                        const unsigned long long idx = sstart[part] + pos;
+                        if (idx >= a.stage.cap) {
+                            __hip_atomic_fetch_add(a.stage.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else if (a.stage.rec4) {
+                            a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), rm[row], a.m2 ? rm[R + row] : 0u, 0u);
+                        } else {
+                            a.stage.rec2[idx] = make_uint2(gid & (kPartSlots - 1u), rm[row]);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            qa = qb;
+        }
+    }
+    __syncthreads();
+    if (MODE == AGG_COUNT) stage_end<MODE>(a.stage, lds);
+    if (MODE == AGG_SCATTER) return;
+    for (uint32_t q = 0; q < a.nq; q++) {
+        const BQDesc &d = D[q];
+        if (d.lds_tab == kNoLds) continue;
+        const l_u64 *ls = (const l_u64 *)((l_u32 *)lds + d.lds_tab);
+        const l_u32 *lc = (l_u32 *)lds + d.lds_tab + 2u * d.G;
+        for (uint32_t g = tid; g < d.G; g += BS) {
+            const uint32_t c = lc[g];
+            if (c) {
+                b_atomic_dev(&a.gcnt[d.out_off + g], (unsigned long long)c);
+                b_atomic_dev(&a.gsum[d.out_off + g], ls[g]);
+            }
+        }
+    }
+}
+
 // (threads, 128-bit chunks per thread per tile, LDS KiB for query tables,
-// staged aggregation of the queries without an LDS table)
+// staged aggregation of the queries without an LDS table, event-grouped
+// kernel with two rows per thread, copies of small tables in LDS)
 struct BatchCfg {
     int bs;
     int unroll;
     int lds_kib;
     int staged;
+    int grouped;
+    int small;
+    int evk;   // grouped: event-list capacity in events per tile row (at most)
 };
-const BatchCfg kBatchCfgs[] = {{512, 1, 48, 0},  {512, 1, 96, 0},  {256, 1, 48, 0},
-                               {256, 1, 48, 1},  {256, 1, 96, 1},  {512, 1, 0, 1},
-                               {512, 1, 48, 1},  {512, 1, 96, 1},  {512, 2, 48, 1},
-                               {1024, 1, 48, 1}};
+const BatchCfg kBatchCfgs[] = {
+    {512, 1, 48, 0, 0, 1, 0},   {512, 1, 96, 0, 0, 1, 0},   {256, 1, 48, 0, 0, 1, 0},
+    {256, 1, 48, 1, 0, 1, 0},   {256, 1, 96, 1, 0, 1, 0},   {512, 1, 0, 1, 0, 1, 0},
+    {512, 1, 48, 1, 0, 1, 0},   {512, 1, 96, 1, 0, 1, 0},   {512, 2, 48, 1, 0, 1, 0},
+    {1024, 1, 48, 1, 0, 1, 0},  {1024, 1, 16, 1, 1, 1, 16}, {1024, 1, 16, 1, 1, 0, 16},
+    {1024, 1, 0, 1, 1, 0, 4},   {1024, 1, 16, 0, 1, 1, 16}, {512, 1, 0, 1, 1, 0, 4},
+    {512, 1, 0, 1, 1, 0, 2},    {512, 1, 16, 1, 1, 1, 16},  {512, 1, 16, 1, 1, 0, 4},
+    {512, 1, 8, 1, 1, 0, 4}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
 template <int MODE>
-static const void *batch_kernel(int bs, int unroll)
+static const void *batch_kernel(const BatchCfg &c)
 {
-    if (bs == 256) return unroll == 1 ? (const void *)k_batch<256, 1, MODE> : (const void *)k_batch<256, 2, MODE>;
-    if (bs == 512) return unroll == 1 ? (const void *)k_batch<512, 1, MODE> : (const void *)k_batch<512, 2, MODE>;
+    if (c.grouped) return c.bs == 512 ? (const void *)k_batch_ev<512, MODE> : (const void *)k_batch_ev<1024, MODE>;
+    if (c.bs == 256) return c.unroll == 1 ? (const void *)k_batch<256, 1, MODE> : (const void *)k_batch<256, 2, MODE>;
+    if (c.bs == 512) return c.unroll == 1 ? (const void *)k_batch<512, 1, MODE> : (const void *)k_batch<512, 2, MODE>;
     return (const void *)k_batch<1024, 1, MODE>;
 }
 
-static const void *batch_kernel_mode(int mode, int bs, int unroll)
+static const void *batch_kernel_mode(int mode, const BatchCfg &c)
 {
-    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(bs, unroll);
-    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(bs, unroll);
-    return batch_kernel<AGG_DIRECT>(bs, unroll);
+    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c);
+    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c);
+    return batch_kernel<AGG_DIRECT>(c);
+}
+
+// fact rows per tile of a configuration
+static uint64_t batch_tile_rows(const BatchCfg &c)
+{
+    return c.grouped ? 2ull * (uint64_t)c.bs : (uint64_t)c.bs * 4u * (uint64_t)c.unroll;
 }
 
 // grid of a configuration: persistent, limited by the least occupant of the
@@ -317,7 +609,7 @@ static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, 
     const int modes[2] = {c.staged ? AGG_COUNT : AGG_DIRECT, c.staged ? AGG_SCATTER : AGG_DIRECT};
     int best = 1 << 30;
     for (int m : modes) {
-        const void *k = batch_kernel_mode(m, c.bs, c.unroll);
+        const void *k = batch_kernel_mode(m, c);
         if (dyn > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
             if (e != hipSuccess) return e;
@@ -328,7 +620,7 @@ static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, 
         best = std::min(best, per_cu);
     }
     if (best < 1) return hipErrorInvalidConfiguration;
-    const uint64_t tile = (uint64_t)c.bs * 4u * c.unroll;
+    const uint64_t tile = batch_tile_rows(c);
     const uint64_t tiles = (rows + tile - 1) / tile;
     *grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best * num_cus, tiles));
     return hipSuccess;
@@ -338,12 +630,11 @@ static hipError_t batch_launch(int cfg, int mode, BatchArgs &a, uint32_t dyn, in
                                hipStream_t s)
 {
     const BatchCfg c = kBatchCfgs[cfg];
-    const uint64_t tile = (uint64_t)c.bs * 4u * c.unroll;
+    const uint64_t tile = batch_tile_rows(c);
     a.ntiles = (a.rows + tile - 1) / tile;
     if (a.ntiles == 0) return hipSuccess;
     void *args[] = {&a};
-    return hipLaunchKernel(batch_kernel_mode(mode, c.bs, c.unroll), dim3(grid), dim3(c.bs), args,
-                           dyn, s);
+    return hipLaunchKernel(batch_kernel_mode(mode, c), dim3(grid), dim3(c.bs), args, dyn, s);
 }
 
 }  // namespace olap
@@ -462,7 +753,22 @@ olap_status plan_config(olap_batch b, int c)
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
     // staged configurations keep a u32 counter and a u64 range start per partition
     const uint32_t stage_bytes = cfg.staged ? up16(b->nparts * 4u) + b->nparts * 8u : 0u;
-    const uint32_t cap = kLdsBytes - 1024u - stage_bytes;
+    // the event-grouped kernel stages per-tile rows and at least one event
+    // per row of the tile
+    const uint32_t R = (uint32_t)batch_tile_rows(cfg);
+    const uint32_t ev_min = cfg.grouped ? 4u * (R + 64u) : 0u;
+    if (cfg.grouped) {
+        a.cnt_lds = cur / 4u;
+        cur += up16(4u * (64u + 64u + 64u + 65u));
+        a.row_lds = cur / 4u;
+        cur += 16u * R;
+        a.m_lds = cur / 4u;
+        cur += (b->star.m2 ? 8u : 4u) * R;
+    }
+    if ((uint64_t)cur + stage_bytes + ev_min + 1024u > kLdsBytes)
+        return bfail(ctx, OLAP_E_CAPACITY, "batch config " + std::to_string(c) + ": LDS too small for " +
+                                               std::to_string(b->nparts) + " partitions");
+    const uint32_t cap = kLdsBytes - 1024u - stage_bytes - ev_min;
     // masks + attributes of probed tables, smallest first, within cap - table budget
     std::vector<int> order;
     for (int t = 0; t < 4; t++)
@@ -475,7 +781,7 @@ olap_status plan_config(olap_batch b, int c)
     }
     for (int t : order) {
         const uint32_t need = up16(8u * (uint32_t)b->star.dim_rows[t]) + up16(4u * (uint32_t)b->star.dim_rows[t]);
-        if (cur + need + table_budget <= cap) {
+        if (cfg.small && cur + need + table_budget <= cap) {
             a.mask_lds[t] = cur / 4u;
             cur += up16(8u * (uint32_t)b->star.dim_rows[t]);
             a.attr_lds[t] = cur / 4u;
@@ -493,7 +799,8 @@ olap_status plan_config(olap_batch b, int c)
     for (int q : qs) desc[q].lds_tab = kNoLds;
     for (int q : qs) {
         const uint64_t need = up16((uint32_t)std::min<int64_t>(b->queries[q].group_nums * 12, 1u << 30));
-        if (b->queries[q].group_nums * 12 <= (int64_t)table_budget && tab_used + need <= table_budget) {
+        if (b->queries[q].group_nums * 12 <= (int64_t)table_budget && tab_used + need <= table_budget &&
+            cur + tab_used + need <= cap) {
             desc[q].lds_tab = (cur + tab_used) / 4u;
             tab_used += (uint32_t)need;
         }
@@ -502,6 +809,13 @@ olap_status plan_config(olap_batch b, int c)
     a.tab_end = cur / 4u;
     b->stage_lds[c] = cur / 4u;
     cur += stage_bytes;
+    if (cfg.grouped) {
+        // the rest of LDS holds the event list (up to 16 events per row)
+        a.ev_lds = cur / 4u;
+        a.ev_cap = std::min<uint32_t>(std::max<uint32_t>((uint32_t)cfg.evk * R, R + 64u),
+                                      (kLdsBytes - 1024u - cur) / 4u) & ~63u;
+        cur += 4u * a.ev_cap;
+    }
     uint32_t pbase = 0;
     for (int q = 0; q < b->nq; q++) {
         const olap_batch_query &Q = b->queries[q];
@@ -747,7 +1061,7 @@ olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_
 {
     if (cfg < 0 || cfg >= kNumBatchCfgs) return OLAP_E_INVALID;
     if (threads) *threads = kBatchCfgs[cfg].bs;
-    if (rows_per_thread) *rows_per_thread = 4 * kBatchCfgs[cfg].unroll;
+    if (rows_per_thread) *rows_per_thread = kBatchCfgs[cfg].grouped ? 2 : 4 * kBatchCfgs[cfg].unroll;
     if (lds_table_kib) *lds_table_kib = kBatchCfgs[cfg].lds_kib;
     if (staged) *staged = kBatchCfgs[cfg].staged;
     return OLAP_OK;
@@ -785,6 +1099,10 @@ olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void
     olap_status st = OLAP_OK;
     for (int c = 0; c < kNumBatchCfgs && st == OLAP_OK; c++) {
         st = olap_batch_set_config(b, c);
+        if (st == OLAP_E_CAPACITY) {  // this layout does not fit the batch: skip it
+            st = OLAP_OK;
+            continue;
+        }
         if (st != OLAP_OK) break;
         st = olap_batch_run(b, sum, count, stream);
         std::vector<double> ts;

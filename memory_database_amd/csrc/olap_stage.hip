@@ -106,8 +106,11 @@ typedef __attribute__((address_space(1))) unsigned long long g_u64;
 
 // work items: chunks of kReduceChunk records of one partition, claimed with
 // one global atomic each; the partition of an item is found by binary search
+constexpr int kReduceThreads = 1024;
+constexpr int kReduceUnroll = 4;   // independent record loads in flight per thread
+
 template <bool WIDE>
-__global__ __launch_bounds__(512) void k_stage_reduce(
+__global__ __launch_bounds__(kReduceThreads) void k_stage_reduce(
     const unsigned long long *part_begin, const uint32_t *item_begin, uint32_t nparts,
     const void *rec, const unsigned long long *part_out, const uint32_t *part_slots,
     uint32_t *work, unsigned long long *gsum, unsigned long long *gcnt)
@@ -144,22 +147,34 @@ __global__ __launch_bounds__(512) void k_stage_reduce(
             lcnt[s] = 0u;
         }
         __syncthreads();
-        for (unsigned long long i = b + threadIdx.x; i < e; i += blockDim.x) {
-            uint32_t g;
-            long long m;
-            if (WIDE) {
-                const uint4 r = ((const uint4 *)rec)[i];
-                g = r.x;
-                m = (long long)(int32_t)r.y + (long long)(int32_t)r.z;
-            } else {
-                const uint2 r = ((const uint2 *)rec)[i];
-                g = r.x;
-                m = (long long)(int32_t)r.y;
+        constexpr unsigned long long step = (unsigned long long)kReduceThreads * kReduceUnroll;
+        for (unsigned long long i0 = b + threadIdx.x; i0 < e; i0 += step) {
+            uint32_t g[kReduceUnroll];
+            long long m[kReduceUnroll];
+#pragma unroll
+            for (int u = 0; u < kReduceUnroll; u++) {
+                const unsigned long long i = i0 + (unsigned long long)u * kReduceThreads;
+                g[u] = kPartSlots;  // no record
+                m[u] = 0;
+                if (i < e) {
+                    if (WIDE) {
+                        const uint4 r = ((const uint4 *)rec)[i];
+                        g[u] = r.x;
+                        m[u] = (long long)(int32_t)r.y + (long long)(int32_t)r.z;
+                    } else {
+                        const uint2 r = ((const uint2 *)rec)[i];
+                        g[u] = r.x;
+                        m[u] = (long long)(int32_t)r.y;
+                    }
+                }
             }
-            if (g < slots) {
-                __hip_atomic_fetch_add(&lsum[g], (unsigned long long)m, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&lcnt[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+            for (int u = 0; u < kReduceUnroll; u++) {
+                if (g[u] < slots) {
+                    __hip_atomic_fetch_add(&lsum[g[u]], (unsigned long long)m[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&lcnt[g[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
         }
         __syncthreads();
@@ -279,11 +294,11 @@ hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, in
     if (e != hipSuccess) return e;
     // 96 KB of LDS per workgroup: one per CU
     if (rec_wide)
-        k_stage_reduce<true><<<num_cus, 512, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
-                                                     part_slots, work, gsum, gcnt);
+        k_stage_reduce<true><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
+                                                                part_slots, work, gsum, gcnt);
     else
-        k_stage_reduce<false><<<num_cus, 512, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
-                                                      part_slots, work, gsum, gcnt);
+        k_stage_reduce<false><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
+                                                                 part_slots, work, gsum, gcnt);
     return hipGetLastError();
 }
 
