@@ -74,6 +74,11 @@ struct BatchArgs {
     uint32_t m_lds;
     uint32_t ev_lds;
     uint32_t ev_cap;
+    // one-pass staging: per-partition chunk state in LDS (u64 base, t1b, t2b,
+    // then u32 rem, id, cnt, t1n, nparts each) and a second event word array
+    ChunkArgs chunk;
+    uint32_t pc_lds;
+    uint32_t evr_lds;
 };
 
 struct MaskPred {
@@ -313,6 +318,19 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Workgroup barrier for LDS hand-offs: waits for this wave's LDS operations
+// (lgkmcnt 0), not for its global stores, which no thread of the workgroup
+// reads back inside the kernel (records, chunk tables and atomics are read by
+// later kernels). The signal fences keep the compiler from moving memory
+// operations across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt 63, expcnt 7, lgkmcnt 0
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // the first set bit of w (non-zero) at or after bit r, wrapping around
 __device__ __forceinline__ uint32_t next_bit_from(unsigned long long w, uint32_t r)
 {
@@ -356,6 +374,21 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     }
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
     if (threadIdx.x < 64u) lds[a.cnt_lds + threadIdx.x] = 0u;
+    const uint32_t np = a.chunk.nparts;
+    stage_l_u64 *pc_base = (stage_l_u64 *)((stage_l_u32 *)lds + a.pc_lds);
+    stage_l_u64 *pc_t1b = pc_base + np;
+    stage_l_u64 *pc_t2b = pc_t1b + np;
+    stage_l_u32 *pc_rem = (stage_l_u32 *)(pc_t2b + np);
+    stage_l_u32 *pc_id = pc_rem + np;
+    stage_l_u32 *pc_cnt = pc_id + np;
+    stage_l_u32 *pc_t1n = pc_cnt + np;
+    if (MODE == AGG_CHUNK) {
+        for (uint32_t p = threadIdx.x; p < np; p += BS) {
+            pc_rem[p] = 0u;
+            pc_id[p] = 0xffffffffu;
+            pc_cnt[p] = 0u;
+        }
+    }
     __syncthreads();
     l_u32 *L = (l_u32 *)lds;
     l_u32 *qcnt = L + a.cnt_lds;          // [64] matches per query in the tile
@@ -365,6 +398,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     l_u32 *rattr = L + a.row_lds;         // [4][R]
     l_u32 *rm = L + a.m_lds;              // [R] m1, then [R] m2
     l_u32 *ev = L + a.ev_lds;             // [ev_cap]
+    l_u32 *evr = L + a.evr_lds;           // [ev_cap] one-pass: gid within partition | row << 13
     stage_l_u32 *scur = (stage_l_u32 *)lds + a.stage.cur_lds;
     const stage_l_u64 *sstart = (const stage_l_u64 *)((stage_l_u32 *)lds + a.stage.start_lds);
     const BQDesc *D = a.desc;
@@ -426,7 +460,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 __hip_atomic_fetch_add(qcnt + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        __syncthreads();
+        lds_barrier();
         // 3. offsets: every query's segment is padded to whole waves
         if (tid < 64u) {
             const uint32_t c = qcnt[tid];
@@ -443,7 +477,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             qcnt[tid] = 0u;  // ready for the next tile (used again after two barriers)
             if (tid == 63u) seg[64] = incl;
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t total = seg[64];
         // rounds over runs of whole query segments that fit the list (one
         // query's padded segment, at most R events, always fits)
@@ -470,7 +504,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t end = seg[tid] + ((qreal[tid] + 63u) & ~63u);
                 for (uint32_t p = seg[tid] + qreal[tid]; p < end; p++) ev[p - lo] = 0xffffu | (tid << 16);
             }
-            __syncthreads();
+            lds_barrier();
             // 5. events: each wave takes whole 64-event chunks, all of one query
             const uint32_t chunks = n >> 6;
             for (uint32_t ch = tid >> 6; ch < chunks; ch += BS / 64u) {
@@ -478,6 +512,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t q = __builtin_amdgcn_readfirstlane(e >> 16);
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
+                if (MODE == AGG_CHUNK) ev[ch * 64u + lane] = 0xffffffffu;  // no record unless staged below
                 const c_desc &d = ((const c_desc *)a.desc)[q];
                 const uint32_t G = d.G;
                 const uint32_t ns = d.nsteps;
@@ -508,6 +543,9 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     }
                 } else {
                     const uint32_t part = d.pbase + (gid >> kPartBits);
+                    // counters: the COUNT pass's per-partition counts, the
+                    // SCATTER pass's cursors, or the one-pass round counts
+                    stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_cnt : scur;
                     uint32_t pos = 0u;
                     if (G <= kPartSlots) {
                         // one partition: one counter / cursor atomic for the wave
@@ -516,12 +554,16 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                             const int leader = __ffsll(act) - 1;
                             uint32_t at = 0u;
                             if ((int)lane == leader)
-                                at = __hip_atomic_fetch_add(scur + d.pbase, (uint32_t)__popcll(act),
+                                at = __hip_atomic_fetch_add(ctr + d.pbase, (uint32_t)__popcll(act),
                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             pos = (uint32_t)__shfl((int)at, leader) + lane_rank(act);
                         }
                     } else if (ok) {
-                        pos = __hip_atomic_fetch_add(scur + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (MODE == AGG_CHUNK && ok) {
+                        ev[ch * 64u + lane] = (part << 11) | pos;
+                        evr[ch * 64u + lane] = (gid & (kPartSlots - 1u)) | (row << 13);
                     }
                     if (MODE == AGG_SCATTER && ok) {
                         const unsigned long long idx = sstart[part] + pos;
@@ -535,9 +577,81 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     }
                 }
             }
-            __syncthreads();
+            lds_barrier();
+            if (MODE == AGG_CHUNK) {
+                // 5b. slots for this round's matches: the rest of each
+                // partition's current chunk, then (one global atomic) a new
+                // chunk; a chunk holds R records, a round's matches of one
+                // partition at most R
+                const uint32_t CH = 1u << a.chunk.ch_bits;
+                for (uint32_t p = tid; p < np; p += BS) {
+                    const uint32_t c = pc_cnt[p];
+                    if (!c) continue;
+                    pc_cnt[p] = 0u;
+                    const uint32_t rem = pc_rem[p];
+                    pc_t1b[p] = pc_base[p];
+                    if (c <= rem) {
+                        pc_t1n[p] = c;
+                        pc_base[p] += c;
+                        pc_rem[p] = rem - c;
+                    } else {
+                        pc_t1n[p] = rem;
+                        if (pc_id[p] != 0xffffffffu) a.chunk.chunk_fill[pc_id[p]] = CH;
+                        const unsigned long long nc = __hip_atomic_fetch_add(
+                            a.chunk.cursor, (unsigned long long)CH, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (nc + CH <= a.chunk.cap) {
+                            const uint32_t id = (uint32_t)(nc >> a.chunk.ch_bits);
+                            a.chunk.chunk_part[id] = p;
+                            pc_t2b[p] = nc;
+                            pc_base[p] = nc + (c - rem);
+                            pc_rem[p] = CH - (c - rem);
+                            pc_id[p] = id;
+                        } else {
+                            pc_t2b[p] = ~0ull;  // pool exhausted: global atomics below
+                            pc_rem[p] = 0u;
+                            pc_id[p] = 0xffffffffu;
+                        }
+                    }
+                }
+                lds_barrier();
+                // 5c. write the records
+                for (uint32_t i = tid; i < n; i += BS) {
+                    const uint32_t e = ev[i];
+                    if (e == 0xffffffffu) continue;
+                    const uint32_t p = e >> 11, k = e & 2047u;
+                    const uint32_t r = evr[i];
+                    const uint32_t glow = r & (kPartSlots - 1u), row = r >> 13;
+                    const uint32_t t1n = pc_t1n[p];
+                    unsigned long long idx;
+                    if (k < t1n) {
+                        idx = pc_t1b[p] + k;
+                    } else {
+                        const unsigned long long t2 = pc_t2b[p];
+                        if (t2 == ~0ull) {
+                            const unsigned long long mv = (unsigned long long)(
+                                (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
+                            const unsigned long long o = a.chunk.part_out[p] + glow;
+                            b_atomic_dev(&a.gsum[o], mv);
+                            b_atomic_dev(&a.gcnt[o], 1ull);
+                            __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            continue;
+                        }
+                        idx = t2 + (k - t1n);
+                    }
+                    if (a.m2)
+                        ((uint4 *)a.chunk.pool)[idx] = make_uint4(glow, rm[row], rm[R + row], 0u);
+                    else
+                        ((uint2 *)a.chunk.pool)[idx] = make_uint2(glow, rm[row]);
+                }
+                lds_barrier();
+            }
             qa = qb;
         }
+    }
+    if (MODE == AGG_CHUNK) {
+        // fills of the chunks still open
+        for (uint32_t p = tid; p < np; p += BS)
+            if (pc_id[p] != 0xffffffffu) a.chunk.chunk_fill[pc_id[p]] = (1u << a.chunk.ch_bits) - pc_rem[p];
     }
     __syncthreads();
     if (MODE == AGG_COUNT) stage_end<MODE>(a.stage, lds);
@@ -564,7 +678,7 @@ struct BatchCfg {
     int bs;
     int unroll;
     int lds_kib;
-    int staged;
+    int staged;   // 0 atomics, 1 two-pass staging (count, scatter), 2 one-pass staging (chunks)
     int grouped;
     int small;
     int evk;   // grouped: event-list capacity in events per tile row (at most)
@@ -576,13 +690,15 @@ const BatchCfg kBatchCfgs[] = {
     {1024, 1, 48, 1, 0, 1, 0},  {1024, 1, 16, 1, 1, 1, 16}, {1024, 1, 16, 1, 1, 0, 16},
     {1024, 1, 0, 1, 1, 0, 4},   {1024, 1, 16, 0, 1, 1, 16}, {512, 1, 0, 1, 1, 0, 4},
     {512, 1, 0, 1, 1, 0, 2},    {512, 1, 16, 1, 1, 1, 16},  {512, 1, 16, 1, 1, 0, 4},
-    {512, 1, 8, 1, 1, 0, 4}};
+    {512, 1, 8, 1, 1, 0, 4},    {512, 1, 8, 2, 1, 0, 2},    {512, 1, 0, 2, 1, 0, 2},
+    {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
 template <int MODE>
 static const void *batch_kernel(const BatchCfg &c)
 {
     if (c.grouped) return c.bs == 512 ? (const void *)k_batch_ev<512, MODE> : (const void *)k_batch_ev<1024, MODE>;
+    if (MODE == AGG_CHUNK) return nullptr;  // one-pass staging exists in the grouped kernel only
     if (c.bs == 256) return c.unroll == 1 ? (const void *)k_batch<256, 1, MODE> : (const void *)k_batch<256, 2, MODE>;
     if (c.bs == 512) return c.unroll == 1 ? (const void *)k_batch<512, 1, MODE> : (const void *)k_batch<512, 2, MODE>;
     return (const void *)k_batch<1024, 1, MODE>;
@@ -590,6 +706,7 @@ static const void *batch_kernel(const BatchCfg &c)
 
 static const void *batch_kernel_mode(int mode, const BatchCfg &c)
 {
+    if (mode == AGG_CHUNK) return batch_kernel<AGG_CHUNK>(c);
     if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c);
     if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c);
     return batch_kernel<AGG_DIRECT>(c);
@@ -606,7 +723,8 @@ static uint64_t batch_tile_rows(const BatchCfg &c)
 static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, int *grid)
 {
     const BatchCfg c = kBatchCfgs[cfg];
-    const int modes[2] = {c.staged ? AGG_COUNT : AGG_DIRECT, c.staged ? AGG_SCATTER : AGG_DIRECT};
+    const int modes[2] = {c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_COUNT : AGG_DIRECT,
+                          c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_SCATTER : AGG_DIRECT};
     int best = 1 << 30;
     for (int m : modes) {
         const void *k = batch_kernel_mode(m, c);
@@ -752,11 +870,13 @@ olap_status plan_config(olap_batch b, int c)
     cur += up16((uint32_t)(sizeof(BQDesc) * b->nq));
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
     // staged configurations keep a u32 counter and a u64 range start per partition
-    const uint32_t stage_bytes = cfg.staged ? up16(b->nparts * 4u) + b->nparts * 8u : 0u;
+    const uint32_t stage_bytes = cfg.staged == 1 ? up16(b->nparts * 4u) + b->nparts * 8u
+                                 : cfg.staged == 2 ? up16(b->nparts * 40u) : 0u;
     // the event-grouped kernel stages per-tile rows and at least one event
     // per row of the tile
     const uint32_t R = (uint32_t)batch_tile_rows(cfg);
-    const uint32_t ev_min = cfg.grouped ? 4u * (R + 64u) : 0u;
+    const uint32_t ev_bytes = cfg.staged == 2 ? 8u : 4u;  // one-pass keeps a second event word
+    const uint32_t ev_min = cfg.grouped ? ev_bytes * (R + 64u) : 0u;
     if (cfg.grouped) {
         a.cnt_lds = cur / 4u;
         cur += up16(4u * (64u + 64u + 64u + 65u));
@@ -809,12 +929,15 @@ olap_status plan_config(olap_batch b, int c)
     a.tab_end = cur / 4u;
     b->stage_lds[c] = cur / 4u;
     cur += stage_bytes;
+    if (cfg.staged == 2) a.pc_lds = b->stage_lds[c];
     if (cfg.grouped) {
-        // the rest of LDS holds the event list (up to 16 events per row)
-        a.ev_lds = cur / 4u;
+        // the rest of LDS holds the event list (at most evk events per row)
         a.ev_cap = std::min<uint32_t>(std::max<uint32_t>((uint32_t)cfg.evk * R, R + 64u),
-                                      (kLdsBytes - 1024u - cur) / 4u) & ~63u;
+                                      (kLdsBytes - 1024u - cur) / ev_bytes) & ~63u;
+        a.ev_lds = cur / 4u;
         cur += 4u * a.ev_cap;
+        a.evr_lds = cur / 4u;
+        if (cfg.staged == 2) cur += 4u * a.ev_cap;
     }
     uint32_t pbase = 0;
     for (int q = 0; q < b->nq; q++) {
@@ -890,6 +1013,23 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
     olap_status st = stager_for(b, c, s);
     if (st != OLAP_OK) return st;
     Stager &S = b->ws->stager;
+    if (kBatchCfgs[c].staged == 2) {
+        // one pass: the pool is sized from the expected matches (it only
+        // grows, so a captured replay never reallocates); matches beyond it
+        // go to global atomics inside the scan
+        const uint32_t ch_bits = (uint32_t)__builtin_ctz((uint32_t)batch_tile_rows(kBatchCfgs[c]));
+        if (!sized) {
+            const double expect = std::max(0.0, b->events) * (double)b->star.rows;
+            e = S.chunk_reserve((unsigned long long)expect, (uint32_t)b->grid[c], ch_bits);
+            if (e != hipSuccess) return bhip(ctx, e, "batch chunk pool");
+        }
+        e = S.chunk_clear(s);
+        a.chunk = S.chunk_args();
+        if (e == hipSuccess) e = batch_launch(c, AGG_CHUNK, a, b->dyn[c], b->grid[c], s);
+        if (e == hipSuccess) e = S.chunk_reduce((unsigned long long *)sum, (unsigned long long *)count,
+                                                ctx->num_cus, s);
+        return e == hipSuccess ? OLAP_OK : bhip(ctx, e, "batch one-pass scan/reduce");
+    }
     const uint32_t cur = b->stage_lds[c];
     const uint32_t start = cur + up16(b->nparts * 4u) / 4u;
     a.stage = S.args(cur, start);

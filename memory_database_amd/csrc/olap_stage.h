@@ -27,7 +27,27 @@ constexpr uint32_t kPartSlots = 1u << kPartBits;       // 8192 gids per partitio
 constexpr uint32_t kReduceChunk = 1u << 21;             // records per reduce work item
 constexpr uint32_t kMaxParts = 1u << 16;
 
-enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2 };
+enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK = 3 };
+
+// One-pass staging (AGG_CHUNK): instead of a COUNT pass that sizes exact
+// ranges, a workgroup hands out record slots from its current chunk of
+// kChunkRecs-aligned slots of a shared pool, one chunk per partition, and
+// claims a new chunk with one global atomic when a chunk runs out. Each chunk
+// records its partition and fill; after the scan the chunks are grouped by
+// partition and reduced like the ranges of the two-pass form. A pool that runs
+// out (it is sized from the expected matches) never loses a match: the scan
+// adds such matches to the outputs with global atomics.
+struct ChunkArgs {
+    void *pool;                       // uint2 or uint4 records
+    unsigned long long cap;           // records the pool holds
+    unsigned long long *cursor;       // records handed out (whole chunks)
+    uint32_t *chunk_part;             // [cap / chunk] partition of each chunk
+    uint32_t *chunk_fill;             // [cap / chunk] records written
+    const unsigned long long *part_out;  // first output slot of each partition
+    uint32_t *overflow;               // matches that went to global atomics
+    uint32_t ch_bits;                 // log2 of the chunk size
+    uint32_t nparts;
+};
 
 struct StageArgs {
     uint32_t nparts;                  // partitions of this scan
@@ -134,15 +154,38 @@ struct Stager {
     uint32_t lds_bytes() const;
     // records dropped because the buffers were too small (synchronises)
     hipError_t overflowed(uint32_t *out, hipStream_t s);
+
+    // one-pass form: size the pool for `records` expected matches plus the
+    // partial chunks `nwg` workgroups can leave, with chunks of 2^ch_bits
+    hipError_t chunk_reserve(unsigned long long records, uint32_t nwg, uint32_t ch_bits);
+    // clear the pool cursor and counters before a scan (kernels only)
+    hipError_t chunk_clear(hipStream_t s);
+    ChunkArgs chunk_args() const;
+    // group the chunks by partition and reduce them into the outputs
+    hipError_t chunk_reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus,
+                            hipStream_t s);
     void release();
     Stager() = default;
     Stager(const Stager &) = delete;
     Stager &operator=(const Stager &) = delete;
     ~Stager() { release(); }
 
+    // one-pass scratch
+    void *pool = nullptr;
+    unsigned long long pool_cap = 0;      // records
+    uint32_t pool_ch_bits = 0;
+    unsigned long long *pool_cursor = nullptr;
+    uint32_t *chunk_part = nullptr;
+    uint32_t *chunk_fill = nullptr;
+    unsigned long long *chunk_order = nullptr;  // (fill << 32) | chunk id, grouped by partition
+    uint32_t *part_chunks = nullptr;      // [nparts + 1] chunk counts, then starts
+    uint32_t *part_ccur = nullptr;        // [nparts] scatter cursors
+    uint32_t *citem_begin = nullptr;      // [nparts + 1] reduce items per partition
+
   private:
     std::vector<uint32_t> qbase_;
     size_t cap_[8] = {};   // element capacities of the eight scratch arrays
+    size_t ccap_[3] = {};  // capacities of part_chunks, part_ccur, citem_begin
 };
 
 }  // namespace olap

@@ -192,6 +192,204 @@ __global__ __launch_bounds__(kReduceThreads) void k_stage_reduce(
     }
 }
 
+// ---------------------------------------------------------------- one-pass
+// chunks handed out by a scan: ids below min(cursor, cap) / chunk size
+
+__device__ __forceinline__ uint32_t chunks_used(const unsigned long long *cursor, unsigned long long cap,
+                                                uint32_t ch_bits)
+{
+    return (uint32_t)(min(*cursor, cap) >> ch_bits);
+}
+
+// chunk counts per partition: a per-workgroup LDS histogram (partitions
+// beyond it count with global atomics directly), then one atomic per bin
+constexpr uint32_t kHistBins = 8192;
+
+__global__ __launch_bounds__(256) void k_chunk_count(const unsigned long long *cursor, unsigned long long cap,
+                                                     uint32_t ch_bits, const uint32_t *chunk_part,
+                                                     uint32_t *part_chunks, uint32_t nparts)
+{
+    __shared__ uint32_t hist[kHistBins];
+    const uint32_t nb = min(nparts, kHistBins);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0u;
+    __syncthreads();
+    const uint32_t n = chunks_used(cursor, cap, ch_bits);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const uint32_t p = chunk_part[c];
+        if (p < nb) atomicAdd(&hist[p], 1u);
+        else atomicAdd(&part_chunks[p], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (hist[i]) atomicAdd(&part_chunks[i], hist[i]);
+}
+
+// exclusive starts of every partition's chunks (in place, total at
+// [nparts]), scatter cursors, and reduce work items of cpi chunks
+__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t *part_chunks, uint32_t *part_ccur,
+                                                     uint32_t *item_begin, uint32_t nparts, uint32_t cpi)
+{
+    __shared__ uint32_t tot[1024];
+    __shared__ uint32_t items[1024];
+    const uint32_t per = (nparts + 1023) / 1024;
+    const uint32_t b = min(threadIdx.x * per, nparts);
+    const uint32_t e = min(b + per, nparts);
+    uint32_t acc = 0, it = 0;
+    for (uint32_t p = b; p < e; p++) {
+        acc += part_chunks[p];
+        it += (part_chunks[p] + cpi - 1) / cpi;
+    }
+    tot[threadIdx.x] = acc;
+    items[threadIdx.x] = it;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0, ri = 0;
+        for (int t = 0; t < 1024; t++) {
+            const uint32_t v = tot[t], vi = items[t];
+            tot[t] = r;
+            items[t] = ri;
+            r += v;
+            ri += vi;
+        }
+        part_chunks[nparts] = r;
+        item_begin[nparts] = ri;
+    }
+    __syncthreads();
+    uint32_t r = tot[threadIdx.x], ri = items[threadIdx.x];
+    for (uint32_t p = b; p < e; p++) {
+        const uint32_t c = part_chunks[p];
+        part_chunks[p] = r;
+        part_ccur[p] = r;
+        item_begin[p] = ri;
+        r += c;
+        ri += (c + cpi - 1) / cpi;
+    }
+}
+
+// chunk ids grouped by partition, each with its fill: (fill << 32) | id;
+// the same per-workgroup LDS pre-count reserves each bin's slots at once
+__global__ __launch_bounds__(256) void k_chunk_scatter(const unsigned long long *cursor, unsigned long long cap,
+                                                       uint32_t ch_bits, const uint32_t *chunk_part,
+                                                       const uint32_t *chunk_fill, uint32_t *part_ccur,
+                                                       unsigned long long *order, uint32_t nparts)
+{
+    __shared__ uint32_t hist[kHistBins];
+    const uint32_t nb = min(nparts, kHistBins);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0u;
+    __syncthreads();
+    const uint32_t n = chunks_used(cursor, cap, ch_bits);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // pass 1: count this workgroup's chunks per bin
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+        const uint32_t p = chunk_part[c];
+        if (p < nb) atomicAdd(&hist[p], 1u);
+    }
+    __syncthreads();
+    // reserve each bin's slots with one global atomic
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (hist[i]) hist[i] = atomicAdd(&part_ccur[i], hist[i]);
+    __syncthreads();
+    // pass 2: place the chunks (LDS cursors inside the reserved ranges)
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+        const uint32_t p = chunk_part[c];
+        const uint32_t pos = p < nb ? atomicAdd(&hist[p], 1u) : atomicAdd(&part_ccur[p], 1u);
+        order[pos] = ((unsigned long long)chunk_fill[c] << 32) | c;
+    }
+}
+
+// work items: up to cpi chunks of one partition, aggregated in an LDS table
+template <bool WIDE>
+__global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
+    const uint32_t *part_chunks, const uint32_t *item_begin, uint32_t nparts, uint32_t cpi,
+    const unsigned long long *order, const void *pool, uint32_t ch_bits,
+    const unsigned long long *part_out, const uint32_t *part_slots, uint32_t *work,
+    unsigned long long *gsum, unsigned long long *gcnt)
+{
+    __shared__ unsigned long long lsum[kPartSlots];
+    __shared__ uint32_t lcnt[kPartSlots];
+    __shared__ uint32_t s_item, s_part;
+    const uint32_t total_items = item_begin[nparts];
+    const uint32_t ch = 1u << ch_bits;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t item = __hip_atomic_fetch_add(work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t lo = 0;
+            if (item < total_items) {
+                uint32_t hi = nparts;  // last p with item_begin[p] <= item
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (item_begin[mid] <= item) lo = mid; else hi = mid;
+                }
+            }
+            s_item = item;
+            s_part = lo;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        const uint32_t p = s_part;
+        if (item >= total_items) break;
+        const uint32_t slots = part_slots[p];
+        const uint32_t cb = part_chunks[p] + (item - item_begin[p]) * cpi;
+        const uint32_t ce = min(cb + cpi, part_chunks[p + 1]);
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            lsum[s] = 0ull;
+            lcnt[s] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t k0 = cb; k0 < ce; k0 += 4u) {
+            uint32_t g[8];
+            long long m[8];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t c = 0, fill = 0;
+                if (k0 + u < ce) {
+                    const unsigned long long o = order[k0 + u];
+                    c = (uint32_t)o;
+                    fill = (uint32_t)(o >> 32);
+                }
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint32_t r = threadIdx.x + (uint32_t)j * kReduceThreads;
+                    g[2 * u + j] = kPartSlots;
+                    m[2 * u + j] = 0;
+                    if (r < fill && r < ch) {
+                        const size_t i = ((size_t)c << ch_bits) + r;
+                        if (WIDE) {
+                            const uint4 rc = ((const uint4 *)pool)[i];
+                            g[2 * u + j] = rc.x;
+                            m[2 * u + j] = (long long)(int32_t)rc.y + (long long)(int32_t)rc.z;
+                        } else {
+                            const uint2 rc = ((const uint2 *)pool)[i];
+                            g[2 * u + j] = rc.x;
+                            m[2 * u + j] = (long long)(int32_t)rc.y;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (g[u] < slots) {
+                    __hip_atomic_fetch_add(&lsum[g[u]], (unsigned long long)m[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&lcnt[g[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned long long out = part_out[p];
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            const uint32_t c = lcnt[s];
+            if (c) {
+                __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // grow-only device array: reallocated only when n exceeds its capacity
 template <typename T>
 hipError_t grow_dev(T *&p, size_t &cap, size_t n)
@@ -302,6 +500,98 @@ hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, in
     return hipGetLastError();
 }
 
+hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint32_t ch_bits)
+{
+    const unsigned long long ch = 1ull << ch_bits;
+    // expected matches + 25%, plus the partial chunk every workgroup can
+    // leave in every partition
+    unsigned long long want = records + records / 4 + (unsigned long long)std::max<uint32_t>(nparts, 1) * nwg_ * ch + ch;
+    want = (want + ch - 1) & ~(ch - 1);
+    hipError_t e = hipSuccess;
+    if (!pool_cursor) e = hipMalloc(&pool_cursor, sizeof(unsigned long long));
+    if (e == hipSuccess && (want > pool_cap || pool_ch_bits != ch_bits || rec_wide != two_measures || !pool)) {
+        if (pool) (void)hipFree(pool);
+        if (chunk_part) (void)hipFree(chunk_part);
+        if (chunk_fill) (void)hipFree(chunk_fill);
+        if (chunk_order) (void)hipFree(chunk_order);
+        pool = nullptr;
+        chunk_part = chunk_fill = nullptr;
+        chunk_order = nullptr;
+        pool_cap = 0;
+        const size_t nch = (size_t)(want >> ch_bits);
+        e = hipMalloc(&pool, (size_t)want * (two_measures ? 16u : 8u));
+        if (e == hipSuccess) e = hipMalloc(&chunk_part, nch * 4);
+        if (e == hipSuccess) e = hipMalloc(&chunk_fill, nch * 4);
+        if (e == hipSuccess) e = hipMalloc(&chunk_order, nch * 8);
+        if (e == hipSuccess) {
+            pool_cap = want;
+            pool_ch_bits = ch_bits;
+            rec_wide = two_measures;
+        }
+    }
+    const size_t np = std::max<uint32_t>(nparts, 1);
+    if (e == hipSuccess) e = grow_dev(part_chunks, ccap_[0], np + 1);
+    if (e == hipSuccess) e = grow_dev(part_ccur, ccap_[1], np);
+    if (e == hipSuccess) e = grow_dev(citem_begin, ccap_[2], np + 1);
+    return e;
+}
+
+hipError_t Stager::chunk_clear(hipStream_t s)
+{
+    hipError_t e = launch_zero_u64(pool_cursor, 1, s);
+    if (e == hipSuccess) e = launch_zero_u32(work, 4, s);
+    return e;
+}
+
+ChunkArgs Stager::chunk_args() const
+{
+    ChunkArgs c;
+    c.pool = pool;
+    c.cap = pool_cap;
+    c.cursor = pool_cursor;
+    c.chunk_part = chunk_part;
+    c.chunk_fill = chunk_fill;
+    c.part_out = part_out;
+    c.overflow = work + 1;
+    c.ch_bits = pool_ch_bits;
+    c.nparts = nparts;
+    return c;
+}
+
+hipError_t Stager::chunk_reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus, hipStream_t s)
+{
+    if (!nparts) return hipSuccess;
+    const unsigned long long nch = pool_cap >> pool_ch_bits;
+    const unsigned grid = (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>((nch + 255) / 256, 4096));
+    hipError_t e = launch_zero_u32(part_chunks, (size_t)nparts + 1, s);
+    if (e == hipSuccess) {
+        k_chunk_count<<<grid, 256, 0, s>>>(pool_cursor, pool_cap, pool_ch_bits, chunk_part, part_chunks, nparts);
+        e = hipGetLastError();
+    }
+    // reduce items of about kReduceChunk records
+    const uint32_t cpi = std::max<uint32_t>(1u, (uint32_t)(kReduceChunk >> pool_ch_bits));
+    if (e == hipSuccess) {
+        k_chunk_scan<<<1, 1024, 0, s>>>(part_chunks, part_ccur, citem_begin, nparts, cpi);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_chunk_scatter<<<grid, 256, 0, s>>>(pool_cursor, pool_cap, pool_ch_bits, chunk_part, chunk_fill, part_ccur,
+                                             chunk_order, nparts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = launch_zero_u32(work, 1, s);
+    if (e != hipSuccess) return e;
+    if (rec_wide)
+        k_chunk_reduce<true><<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order,
+                                                                pool, pool_ch_bits, part_out, part_slots, work, gsum,
+                                                                gcnt);
+    else
+        k_chunk_reduce<false><<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order,
+                                                                 pool, pool_ch_bits, part_out, part_slots, work, gsum,
+                                                                 gcnt);
+    return hipGetLastError();
+}
+
 StageArgs Stager::args(uint32_t cur_lds, uint32_t start_lds) const
 {
     StageArgs a;
@@ -351,6 +641,16 @@ void Stager::release()
     cap = 0;
     nparts = 0;
     for (size_t &c : cap_) c = 0;
+    void *cptrs[] = {pool, pool_cursor, chunk_part, chunk_fill, chunk_order, part_chunks, part_ccur, citem_begin};
+    for (void *p : cptrs)
+        if (p) (void)hipFree(p);
+    pool = nullptr;
+    pool_cursor = nullptr;
+    chunk_part = chunk_fill = part_chunks = part_ccur = citem_begin = nullptr;
+    chunk_order = nullptr;
+    pool_cap = 0;
+    pool_ch_bits = 0;
+    for (size_t &c : ccap_) c = 0;
 }
 
 }  // namespace olap
