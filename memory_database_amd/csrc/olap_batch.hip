@@ -691,7 +691,8 @@ const BatchCfg kBatchCfgs[] = {
     {1024, 1, 0, 1, 1, 0, 4},   {1024, 1, 16, 0, 1, 1, 16}, {512, 1, 0, 1, 1, 0, 4},
     {512, 1, 0, 1, 1, 0, 2},    {512, 1, 16, 1, 1, 1, 16},  {512, 1, 16, 1, 1, 0, 4},
     {512, 1, 8, 1, 1, 0, 4},    {512, 1, 8, 2, 1, 0, 2},    {512, 1, 0, 2, 1, 0, 2},
-    {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4}};
+    {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4},    {512, 1, 0, 2, 1, 0, 4},
+    {512, 1, 0, 2, 1, 0, 3}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
 template <int MODE>

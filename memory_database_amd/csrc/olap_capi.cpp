@@ -13,6 +13,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "olap_internal.h"
@@ -147,6 +149,7 @@ void fill_args(ScanArgs &a, const olap_query_s *q, const Layout &L, int64_t *sum
     a.list_off = L.list_off;
     a.count_only = q->cols.m1 == nullptr ? 1u : 0u;
     a.gvec_null = OLAP_GID_NULL;
+    a.eager1 = q->eager1;
 }
 
 bool columns_vec_ok(const olap_query_s *q)
@@ -712,11 +715,28 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
         return hip_fail(ctx, e, "query_create: packed vectors");
     }
     q->packed_words = total_words;
+    // Probe order: the most selective dimension first. A row passes only if
+    // every step passes and its gid is the sum of code * factor over the
+    // steps, so the order changes no result; it decides how many later
+    // foreign-key lines the early exit still has to fetch.
+    int perm[OLAP_MAX_DIMS];
+    double frac[OLAP_MAX_DIMS];
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        perm[s] = s;
+        frac[s] = cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
+    }
+    std::stable_sort(perm, perm + nd, [&](int x, int y) { return frac[x] < frac[y]; });
+    // step 1 is fetched with step 0 when most of its 128-byte lines (32 keys)
+    // hold a row that passed step 0: a streamed column then costs less than
+    // the dependent partial fetches
+    q->eager1 = nd > 1 && 1.0 - std::pow(1.0 - frac[perm[0]], 32.0) > 0.5 ? 1u : 0u;
     size_t off = 0;
     for (int s = 0; s < OLAP_MAX_DIMS; s++) memset(&q->steps[s], 0, sizeof(StepDesc));
-    for (int s = 0; s < nd && e == hipSuccess; s++) {
+    for (int i = 0; i < nd && e == hipSuccess; i++) {
+        const int s = perm[i];
         const int t = plan->orders[s];
-        StepDesc &st = q->steps[s];
+        StepDesc &st = q->steps[i];
         st.fk = cols->fk[t];
         st.words = q->packed + off;
         st.dim_rows = (uint32_t)cols->dim_rows[t];

@@ -55,6 +55,8 @@ struct ScanArgs {
     uint32_t gt_off;            // word offset of the LDS group table (8-byte aligned)
     uint32_t list_off;          // word offset of the LDS tile list (compacting kernels)
     uint32_t count_only;        // no measure: only counts
+    uint32_t eager1;            // row-wise: load step 1's keys with step 0's (most of its lines are live);
+                                // selects the kernel variant, not read by the kernels
     // column-wise state
     int32_t *gvec;              // per-row gid vector (static-vector column-wise)
     int32_t gvec_null;          // NULL value written into gvec
@@ -102,6 +104,7 @@ struct olap_query_s {
     // -1 automatic (from est_events), 0 global atomics, 1 staged
     int agg_layout = -1;
     double est_events = 0.0;           // expected matches (uniform keys assumed)
+    uint32_t eager1 = 0;               // row-wise: stream step 1's keys with step 0's
     olap::Stager *stager = nullptr;
     uint32_t stage_grid = 0;
 };

@@ -422,7 +422,7 @@ __device__ __forceinline__ void emit_staged(const ScanArgs &a, uint32_t *lds, ui
 // MODE: AGG_DIRECT aggregates into the LDS table (GLDS) or with global
 // atomics; AGG_COUNT / AGG_SCATTER are the two scan passes of the staged
 // aggregation (the COUNT pass never reads the measures)
-template <int NDIM, bool VEC, bool GLDS, int BS, int U, int MODE = AGG_DIRECT>
+template <int NDIM, bool VEC, bool GLDS, int BS, int U, int MODE = AGG_DIRECT, bool EAGER = false>
 __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
 {
     constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
@@ -434,30 +434,43 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
     const uint32_t tid = threadIdx.x;
     // software pipeline: the first key column of the next tile is in flight
     // while the dependent probe chain of the current tile runs
-    v4i nf[U];
+    v4i nf[U], nf1[U];
     uint32_t np[U];
     uint64_t tile = blockIdx.x;
+    constexpr bool eager = NDIM > 1 && EAGER;
     if (tile < a.ntiles) {
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            nf[u] = load4<VEC>(a.st[0].fk, tile * TILE + (uint64_t)(u * BS + tid) * 4u, rows, np[u]);
+        for (int u = 0; u < U; u++) {
+            const uint64_t r = tile * TILE + (uint64_t)(u * BS + tid) * 4u;
+            nf[u] = load4<VEC>(a.st[0].fk, r, rows, np[u]);
+            if (eager) {
+                uint32_t dummy;
+                nf1[u] = load4<VEC>(a.st[NDIM > 1 ? 1 : 0].fk, r, rows, dummy);
+            }
+        }
     }
     for (; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile * TILE;
-        v4i f[U];
+        v4i f[U], f1[U];
         uint32_t alive[U];
         uint32_t gid[U][4];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             f[u] = nf[u];
+            f1[u] = nf1[u];
             alive[u] = np[u];
         }
         const uint64_t next = tile + gridDim.x;
         if (next < a.ntiles) {
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                nf[u] = load4<VEC>(a.st[0].fk, next * TILE + (uint64_t)(u * BS + tid) * 4u, rows,
-                                   np[u]);
+            for (int u = 0; u < U; u++) {
+                const uint64_t r = next * TILE + (uint64_t)(u * BS + tid) * 4u;
+                nf[u] = load4<VEC>(a.st[0].fk, r, rows, np[u]);
+                if (eager) {
+                    uint32_t dummy;
+                    nf1[u] = load4<VEC>(a.st[NDIM > 1 ? 1 : 0].fk, r, rows, dummy);
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
@@ -465,7 +478,9 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
         for (int s = 1; s < NDIM; s++) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (alive[u]) {
+                if (s == 1 && eager) {
+                    f[u] = f1[u];
+                } else if (alive[u]) {
                     uint32_t dummy;
                     const uint64_t r = base + (uint64_t)(u * BS + tid) * 4u;
                     f[u] = load4<VEC>(a.st[s].fk, r, rows, dummy);
@@ -922,6 +937,17 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
     const ScanShape sh = kRowShapes[shape];
     a.ntiles = (a.rows + (uint64_t)sh.bs * 4u * sh.unroll - 1) / ((uint64_t)sh.bs * 4u * sh.unroll);
     if (a.ntiles == 0) return hipSuccess;
+    if (VEC && NDIM > 1 && a.eager1) {
+        // step 1's keys streamed with step 0's
+        switch (shape) {
+        case 0: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 4, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+        case 1: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 2, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+        case 2: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 1024, 2, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 1024);
+        case 3: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 1024, 4, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 1024);
+        case 4: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 256, 8, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 256);
+        default: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 8, AGG_DIRECT, true>, a, num_cus, dyn_lds, a.ntiles, s, 512);
+        }
+    }
     switch (shape) {
     case 0: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 4>, a, num_cus, dyn_lds, a.ntiles, s, 512);
     case 1: return launch_persistent(k_rowwise<NDIM, VEC, GLDS, 512, 2>, a, num_cus, dyn_lds, a.ntiles, s, 512);
