@@ -222,7 +222,7 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
         "speedup": plain_ms / batch_ms,
         "matches_per_row": events,
         "config": {"id": cfg, "threads": threads, "rows_per_thread": rpt, "lds_table_kib": kib,
-                   "staged": bool(staged)},
+                   "aggregation": ["atomics", "two-pass staged", "one-pass staged"][staged]},
         "union_bytes": union_bytes, "union_GBps": union_bytes / (batch_ms / 1e3) / 1e9,
         "effective_GBps": per_query_bytes / (batch_ms / 1e3) / 1e9,
         "timing": "hipGraph replay, HIP events, mean of 5 after 1 warmup; plain = sum of the "

@@ -46,13 +46,13 @@ struct Request {
 // expected matches above which a batch uses the staged aggregation
 // (count / scatter / LDS reduce) rather than LDS tables plus global atomics;
 // measured crossover on 60M and 600M fact rows: between 1e7 and 3e7 matches
+// (The one-pass event-grouped scan is faster only for the heaviest batches,
+// 73 against 84 ms at 64 queries over 600M rows, and a worker using both
+// would keep both a two-pass record buffer and a one-pass pool of that size;
+// workers stay with two-pass staging.)
 constexpr double kStagedMatches = 2.0e7;
-// above this many matches the one-pass event-grouped scan wins (measured at
-// 64 queries over 600M rows: 73 ms against 84 ms)
-constexpr double kOnePassMatches = 3.0e9;
 constexpr int kCfgAtomics = 0;   // 512 threads, 48 KiB of LDS tables
 constexpr int kCfgStaged = 9;    // 1024 threads, 48 KiB of LDS tables, two-pass staging
-constexpr int kCfgOnePass = 22;  // event-grouped, 512 threads, 8 KiB of LDS tables, one-pass staging
 
 }  // namespace
 
@@ -200,7 +200,7 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
     int cfg = -1;
     if (st == OLAP_OK) {
         const double matches = olap_batch_expected_events(b) * (double)star.rows;
-        cfg = matches > kOnePassMatches ? kCfgOnePass : matches > kStagedMatches ? kCfgStaged : kCfgAtomics;
+        cfg = matches > kStagedMatches ? kCfgStaged : kCfgAtomics;
         st = olap_batch_set_config(b, cfg);
     }
     const size_t total = (size_t)off.back();

@@ -270,7 +270,8 @@ def legacy_count(sums) -> int:
 
 
 def config_info(cfg: int):
-    """(threads, rows per thread, LDS KiB for query tables, staged) of a batch config."""
+    """(threads, rows per thread, LDS KiB for query tables, aggregation: 0 atomics,
+    1 two-pass staged, 2 one-pass staged) of a batch config."""
     th, rp, kib, st = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     load().olap_batch_config_info(cfg, C.byref(th), C.byref(rp), C.byref(kib), C.byref(st))
     return th.value, rp.value, kib.value, st.value
