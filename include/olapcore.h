@@ -320,6 +320,8 @@ olap_status olap_server_pause(olap_server s, int32_t paused);
 olap_status olap_server_cancel(olap_server s, uint64_t ticket);
 olap_status olap_server_get_stats(olap_server s, olap_server_stats *out);
 olap_status olap_server_reset_stats(olap_server s);
+/* message of the last failure on the server; valid until the next failing
+ * call on it */
 const char *olap_server_last_error(olap_server s);
 /* Stops the workers after their running batches; requests still pending are
  * dropped, so call it once every wait has returned. */
