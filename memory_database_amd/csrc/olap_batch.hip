@@ -568,7 +568,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     if (MODE == AGG_SCATTER && ok) {
                         const unsigned long long idx = sstart[part] + pos;
                         if (idx >= a.stage.cap) {
-                            __hip_atomic_fetch_add(a.stage.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            stage_overflow(a.stage, part, gid & (kPartSlots - 1u),
+                                           (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
                         } else if (a.stage.rec4) {
                             a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), rm[row], a.m2 ? rm[R + row] : 0u, 0u);
                         } else {
@@ -1033,7 +1034,7 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
     }
     const uint32_t cur = b->stage_lds[c];
     const uint32_t start = cur + up16(b->nparts * 4u) / 4u;
-    a.stage = S.args(cur, start);
+    a.stage = S.args(cur, start, a.gsum, a.gcnt);
     e = batch_launch(c, AGG_COUNT, a, b->dyn[c], b->grid[c], s);
     if (e == hipSuccess) e = S.prefix(s);
     if (e != hipSuccess) return bhip(ctx, e, "batch count pass");
@@ -1042,7 +1043,7 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
         e = S.total(&total, s);
         if (e == hipSuccess) e = S.reserve(total);
         if (e != hipSuccess) return bhip(ctx, e, "batch staging buffers");
-        a.stage = S.args(cur, start);
+        a.stage = S.args(cur, start, a.gsum, a.gcnt);
     }
     e = batch_launch(c, AGG_SCATTER, a, b->dyn[c], b->grid[c], s);
     if (e == hipSuccess) e = S.reduce((unsigned long long *)sum, (unsigned long long *)count,

@@ -313,13 +313,13 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     }
     ScanArgs a;
     fill_args(a, q, L, sum, count);
-    a.stage = S.args(cur_lds, start_lds);
+    a.stage = S.args(cur_lds, start_lds, (unsigned long long *)sum, (unsigned long long *)count);
     HIPC(ctx, run_rowwise_staged(a, nd, dyn, AGG_COUNT, grid, s));
     HIPC(ctx, S.prefix(s));
     unsigned long long total = 0;
     HIPC(ctx, S.total(&total, s));
     HIPC(ctx, S.reserve(total));
-    a.stage = S.args(cur_lds, start_lds);
+    a.stage = S.args(cur_lds, start_lds, (unsigned long long *)sum, (unsigned long long *)count);
     HIPC(ctx, run_rowwise_staged(a, nd, dyn, AGG_SCATTER, grid, s));
     HIPC(ctx, S.reduce((unsigned long long *)sum, (unsigned long long *)count, ctx->num_cus, s));
     return OLAP_OK;

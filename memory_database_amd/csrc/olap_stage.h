@@ -25,6 +25,7 @@ namespace olap {
 constexpr uint32_t kPartBits = 13;
 constexpr uint32_t kPartSlots = 1u << kPartBits;       // 8192 gids per partition (96 KB LDS table)
 constexpr uint32_t kReduceChunk = 1u << 21;             // records per reduce work item
+constexpr uint32_t kReduceItemChunks = 2048;            // one-pass: at most this many chunks per work item
 constexpr uint32_t kMaxParts = 1u << 16;
 
 enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK = 3 };
@@ -61,7 +62,12 @@ struct StageArgs {
     uint2 *rec2;
     uint4 *rec4;
     unsigned long long cap;           // records the buffer holds
-    uint32_t *overflow;               // set when a range would exceed cap
+    uint32_t *overflow;               // matches beyond cap (added with global atomics)
+    // outputs, for matches beyond cap (a buffer sized by an earlier scan of
+    // columns that have since changed): never lost, added directly
+    const unsigned long long *part_out;  // first output slot of each partition
+    unsigned long long *gsum;
+    unsigned long long *gcnt;
 };
 
 #if defined(__HIPCC__)
@@ -78,6 +84,16 @@ __device__ __forceinline__ void stage_count(const StageArgs &s, uint32_t *lds, u
     __hip_atomic_fetch_add(c + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// a match with no room in the record buffer goes to the outputs directly
+__device__ __forceinline__ void stage_overflow(const StageArgs &s, uint32_t p, uint32_t gid_low, long long m)
+{
+    typedef __attribute__((address_space(1))) unsigned long long s_g_u64;
+    const unsigned long long o = s.part_out[p] + gid_low;
+    __hip_atomic_fetch_add((s_g_u64 *)&s.gsum[o], (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add((s_g_u64 *)&s.gcnt[o], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(s.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // write one record of partition p
 __device__ __forceinline__ void stage_scatter(const StageArgs &s, uint32_t *lds, uint32_t p,
                                               uint32_t gid_low, int32_t m1, int32_t m2)
@@ -87,7 +103,7 @@ __device__ __forceinline__ void stage_scatter(const StageArgs &s, uint32_t *lds,
     const stage_l_u64 *st = (const stage_l_u64 *)((stage_l_u32 *)lds + s.start_lds);
     const unsigned long long i = st[p] + pos;
     if (i >= s.cap) {
-        __hip_atomic_fetch_add(s.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stage_overflow(s, p, gid_low, (long long)m1 + (long long)m2);
         return;
     }
     if (s.rec4) s.rec4[i] = make_uint4(gid_low, (uint32_t)m1, (uint32_t)m2, 0u);
@@ -149,7 +165,7 @@ struct Stager {
     uint32_t part_base(int q) const { return qbase_.empty() ? 0 : qbase_[(size_t)q]; }
     hipError_t reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus,
                       hipStream_t s);
-    StageArgs args(uint32_t cur_lds, uint32_t start_lds) const;
+    StageArgs args(uint32_t cur_lds, uint32_t start_lds, unsigned long long *gsum, unsigned long long *gcnt) const;
     // LDS bytes of the counters/cursors plus the u64 range starts
     uint32_t lds_bytes() const;
     // records dropped because the buffers were too small (synchronises)

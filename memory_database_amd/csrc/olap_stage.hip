@@ -105,89 +105,111 @@ __global__ void k_stage_finalize(unsigned long long *starts, const unsigned long
 typedef __attribute__((address_space(1))) unsigned long long g_u64;
 
 // work items: chunks of kReduceChunk records of one partition, claimed with
-// one global atomic each; the partition of an item is found by binary search
+// one global atomic each; the partition of an item is found by binary search.
+// Records are read 16 bytes per load (two 8-byte records or one 16-byte
+// record), kReduceLoads independent loads per thread in flight.
 constexpr int kReduceThreads = 1024;
-constexpr int kReduceUnroll = 4;   // independent record loads in flight per thread
+constexpr int kReduceLoads = 8;
 
+// claim the next work item; thread 0 finds its partition
+__device__ __forceinline__ void reduce_claim(uint32_t *work, const uint32_t *item_begin, uint32_t nparts,
+                                             uint32_t &s_item, uint32_t &s_part)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t item = __hip_atomic_fetch_add(work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t lo = 0;
+        if (item < item_begin[nparts]) {
+            uint32_t hi = nparts;  // last p with item_begin[p] <= item
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (item_begin[mid] <= item) lo = mid; else hi = mid;
+            }
+        }
+        s_item = item;
+        s_part = lo;
+    }
+}
+
+// one record into the LDS table (g >= slots: no record)
+__device__ __forceinline__ void reduce_add(unsigned long long *lsum, uint32_t *lcnt, uint32_t slots, uint32_t g,
+                                           long long m)
+{
+    if (g < slots) {
+        __hip_atomic_fetch_add(&lsum[g], (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&lcnt[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// the non-empty slots of the LDS table into the outputs
+__device__ __forceinline__ void reduce_flush(const unsigned long long *lsum, const uint32_t *lcnt, uint32_t slots,
+                                             unsigned long long out, unsigned long long *gsum,
+                                             unsigned long long *gcnt)
+{
+    for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+        const uint32_t c = lcnt[s];
+        if (c) {
+            __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Partition ranges of the two-pass form: records [part_begin[p],
+// part_begin[p + 1]) of the buffer, clipped to its capacity (matches beyond
+// it were added to the outputs by the scan).
 template <bool WIDE>
 __global__ __launch_bounds__(kReduceThreads) void k_stage_reduce(
     const unsigned long long *part_begin, const uint32_t *item_begin, uint32_t nparts,
-    const void *rec, const unsigned long long *part_out, const uint32_t *part_slots,
+    const void *rec, unsigned long long cap, const unsigned long long *part_out, const uint32_t *part_slots,
     uint32_t *work, unsigned long long *gsum, unsigned long long *gcnt)
 {
     __shared__ unsigned long long lsum[kPartSlots];
     __shared__ uint32_t lcnt[kPartSlots];
     __shared__ uint32_t s_item, s_part;
-    const uint32_t total_items = item_begin[nparts];
+    const uint4 *v4 = (const uint4 *)rec;
     for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t item =
-                __hip_atomic_fetch_add(work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t lo = 0;
-            if (item < total_items) {
-                uint32_t hi = nparts;  // last p with item_begin[p] <= item
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (item_begin[mid] <= item) lo = mid; else hi = mid;
-                }
-            }
-            s_item = item;
-            s_part = lo;
-        }
+        reduce_claim(work, item_begin, nparts, s_item, s_part);
         __syncthreads();
         const uint32_t item = s_item;
         const uint32_t p = s_part;
-        if (item >= total_items) break;
+        if (item >= item_begin[nparts]) break;
         const uint32_t slots = part_slots[p];
-        const unsigned long long b =
-            part_begin[p] + (unsigned long long)(item - item_begin[p]) * kReduceChunk;
-        const unsigned long long e = min(b + kReduceChunk, part_begin[p + 1]);
+        const unsigned long long b = min(part_begin[p] + (unsigned long long)(item - item_begin[p]) * kReduceChunk, cap);
+        const unsigned long long e = min(min(b + kReduceChunk, part_begin[p + 1]), cap);
         for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
             lsum[s] = 0ull;
             lcnt[s] = 0u;
         }
         __syncthreads();
-        constexpr unsigned long long step = (unsigned long long)kReduceThreads * kReduceUnroll;
-        for (unsigned long long i0 = b + threadIdx.x; i0 < e; i0 += step) {
-            uint32_t g[kReduceUnroll];
-            long long m[kReduceUnroll];
+        // 16-byte units [u0, u1): record pairs from the even record at or
+        // below b (narrow), or single records (wide); cap is even, so the
+        // pair holding record e - 1 ends within the buffer
+        const unsigned long long u0 = WIDE ? b : (b >> 1);
+        const unsigned long long u1 = WIDE ? e : ((e + 1) >> 1);
+        constexpr unsigned long long step = (unsigned long long)kReduceThreads * kReduceLoads;
+        for (unsigned long long i0 = u0 + threadIdx.x; i0 < u1; i0 += step) {
+            uint4 v[kReduceLoads];
 #pragma unroll
-            for (int u = 0; u < kReduceUnroll; u++) {
+            for (int u = 0; u < kReduceLoads; u++) {
                 const unsigned long long i = i0 + (unsigned long long)u * kReduceThreads;
-                g[u] = kPartSlots;  // no record
-                m[u] = 0;
-                if (i < e) {
-                    if (WIDE) {
-                        const uint4 r = ((const uint4 *)rec)[i];
-                        g[u] = r.x;
-                        m[u] = (long long)(int32_t)r.y + (long long)(int32_t)r.z;
-                    } else {
-                        const uint2 r = ((const uint2 *)rec)[i];
-                        g[u] = r.x;
-                        m[u] = (long long)(int32_t)r.y;
-                    }
-                }
+                v[u] = i < u1 ? v4[i] : make_uint4(kPartSlots, 0u, kPartSlots, 0u);
             }
 #pragma unroll
-            for (int u = 0; u < kReduceUnroll; u++) {
-                if (g[u] < slots) {
-                    __hip_atomic_fetch_add(&lsum[g[u]], (unsigned long long)m[u], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(&lcnt[g[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int u = 0; u < kReduceLoads; u++) {
+                const unsigned long long i = i0 + (unsigned long long)u * kReduceThreads;
+                if (WIDE) {
+                    reduce_add(lsum, lcnt, slots, v[u].x, (long long)(int32_t)v[u].y + (long long)(int32_t)v[u].z);
+                } else {
+                    const unsigned long long r = 2ull * i;  // record index of v[u].x
+                    reduce_add(lsum, lcnt, slots, r >= b && r < e ? v[u].x : kPartSlots, (long long)(int32_t)v[u].y);
+                    reduce_add(lsum, lcnt, slots, r + 1 >= b && r + 1 < e ? v[u].z : kPartSlots,
+                               (long long)(int32_t)v[u].w);
                 }
             }
         }
         __syncthreads();
-        const unsigned long long out = part_out[p];
-        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
-            const uint32_t c = lcnt[s];
-            if (c) {
-                __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+        reduce_flush(lsum, lcnt, slots, part_out[p], gsum, gcnt);
         __syncthreads();
     }
 }
@@ -307,85 +329,58 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
 {
     __shared__ unsigned long long lsum[kPartSlots];
     __shared__ uint32_t lcnt[kPartSlots];
+    __shared__ unsigned long long lord[kReduceItemChunks];  // the item's (fill << 32 | chunk id)
     __shared__ uint32_t s_item, s_part;
-    const uint32_t total_items = item_begin[nparts];
-    const uint32_t ch = 1u << ch_bits;
+    const uint4 *v4 = (const uint4 *)pool;
+    // 16-byte units per chunk: record pairs (narrow) or records (wide)
+    const uint32_t ubits = WIDE ? ch_bits : ch_bits - 1u;
     for (;;) {
-        if (threadIdx.x == 0) {
-            const uint32_t item = __hip_atomic_fetch_add(work, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t lo = 0;
-            if (item < total_items) {
-                uint32_t hi = nparts;  // last p with item_begin[p] <= item
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (item_begin[mid] <= item) lo = mid; else hi = mid;
-                }
-            }
-            s_item = item;
-            s_part = lo;
-        }
+        reduce_claim(work, item_begin, nparts, s_item, s_part);
         __syncthreads();
         const uint32_t item = s_item;
         const uint32_t p = s_part;
-        if (item >= total_items) break;
+        if (item >= item_begin[nparts]) break;
         const uint32_t slots = part_slots[p];
         const uint32_t cb = part_chunks[p] + (item - item_begin[p]) * cpi;
-        const uint32_t ce = min(cb + cpi, part_chunks[p + 1]);
+        const uint32_t n = min(cb + cpi, part_chunks[p + 1]) - cb;  // at most cpi <= kReduceItemChunks
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lord[i] = order[cb + i];
         for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
             lsum[s] = 0ull;
             lcnt[s] = 0u;
         }
         __syncthreads();
-        for (uint32_t k0 = cb; k0 < ce; k0 += 4u) {
-            uint32_t g[8];
-            long long m[8];
+        const uint32_t units = n << ubits;
+        for (uint32_t i0 = threadIdx.x; i0 < units; i0 += (uint32_t)kReduceThreads * kReduceLoads) {
+            uint4 v[kReduceLoads];
+            uint32_t r[kReduceLoads], fill[kReduceLoads];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                uint32_t c = 0, fill = 0;
-                if (k0 + u < ce) {
-                    const unsigned long long o = order[k0 + u];
-                    c = (uint32_t)o;
-                    fill = (uint32_t)(o >> 32);
-                }
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const uint32_t r = threadIdx.x + (uint32_t)j * kReduceThreads;
-                    g[2 * u + j] = kPartSlots;
-                    m[2 * u + j] = 0;
-                    if (r < fill && r < ch) {
-                        const size_t i = ((size_t)c << ch_bits) + r;
-                        if (WIDE) {
-                            const uint4 rc = ((const uint4 *)pool)[i];
-                            g[2 * u + j] = rc.x;
-                            m[2 * u + j] = (long long)(int32_t)rc.y + (long long)(int32_t)rc.z;
-                        } else {
-                            const uint2 rc = ((const uint2 *)pool)[i];
-                            g[2 * u + j] = rc.x;
-                            m[2 * u + j] = (long long)(int32_t)rc.y;
-                        }
-                    }
+            for (int u = 0; u < kReduceLoads; u++) {
+                const uint32_t i = i0 + (uint32_t)u * kReduceThreads;
+                v[u] = make_uint4(kPartSlots, 0u, kPartSlots, 0u);
+                r[u] = 0u;
+                fill[u] = 0u;
+                if (i < units) {
+                    const unsigned long long o = lord[i >> ubits];
+                    const uint32_t w = i & ((1u << ubits) - 1u);
+                    r[u] = WIDE ? w : 2u * w;  // record index within the chunk
+                    fill[u] = (uint32_t)(o >> 32);
+                    if (r[u] < fill[u]) v[u] = v4[((size_t)(uint32_t)o << ubits) + w];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (g[u] < slots) {
-                    __hip_atomic_fetch_add(&lsum[g[u]], (unsigned long long)m[u], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(&lcnt[g[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int u = 0; u < kReduceLoads; u++) {
+                if (WIDE) {
+                    reduce_add(lsum, lcnt, slots, r[u] < fill[u] ? v[u].x : kPartSlots,
+                               (long long)(int32_t)v[u].y + (long long)(int32_t)v[u].z);
+                } else {
+                    reduce_add(lsum, lcnt, slots, r[u] < fill[u] ? v[u].x : kPartSlots, (long long)(int32_t)v[u].y);
+                    reduce_add(lsum, lcnt, slots, r[u] + 1u < fill[u] ? v[u].z : kPartSlots,
+                               (long long)(int32_t)v[u].w);
                 }
             }
         }
         __syncthreads();
-        const unsigned long long out = part_out[p];
-        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
-            const uint32_t c = lcnt[s];
-            if (c) {
-                __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+        reduce_flush(lsum, lcnt, slots, part_out[p], gsum, gcnt);
         __syncthreads();
     }
 }
@@ -472,7 +467,8 @@ hipError_t Stager::total(unsigned long long *out, hipStream_t s)
 hipError_t Stager::reserve(unsigned long long records)
 {
     if (records <= cap && rec && rec_wide == two_measures) return hipSuccess;
-    const unsigned long long want = records + records / 4 + 4096;
+    // even, so the reduce's 16-byte record pairs never read past it
+    const unsigned long long want = (records + records / 4 + 4096) & ~1ull;
     if (rec) (void)hipFree(rec);
     rec = nullptr;
     cap = 0;
@@ -492,11 +488,11 @@ hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, in
     if (e != hipSuccess) return e;
     // 96 KB of LDS per workgroup: one per CU
     if (rec_wide)
-        k_stage_reduce<true><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
-                                                                part_slots, work, gsum, gcnt);
+        k_stage_reduce<true><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, cap,
+                                                                part_out, part_slots, work, gsum, gcnt);
     else
-        k_stage_reduce<false><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, part_out,
-                                                                 part_slots, work, gsum, gcnt);
+        k_stage_reduce<false><<<num_cus, kReduceThreads, 0, s>>>(part_begin, item_begin, nparts, rec, cap,
+                                                                 part_out, part_slots, work, gsum, gcnt);
     return hipGetLastError();
 }
 
@@ -569,7 +565,7 @@ hipError_t Stager::chunk_reduce(unsigned long long *gsum, unsigned long long *gc
         e = hipGetLastError();
     }
     // reduce items of about kReduceChunk records
-    const uint32_t cpi = std::max<uint32_t>(1u, (uint32_t)(kReduceChunk >> pool_ch_bits));
+    const uint32_t cpi = std::min<uint32_t>(kReduceItemChunks, std::max<uint32_t>(1u, (uint32_t)(kReduceChunk >> pool_ch_bits)));
     if (e == hipSuccess) {
         k_chunk_scan<<<1, 1024, 0, s>>>(part_chunks, part_ccur, citem_begin, nparts, cpi);
         e = hipGetLastError();
@@ -592,7 +588,8 @@ hipError_t Stager::chunk_reduce(unsigned long long *gsum, unsigned long long *gc
     return hipGetLastError();
 }
 
-StageArgs Stager::args(uint32_t cur_lds, uint32_t start_lds) const
+StageArgs Stager::args(uint32_t cur_lds, uint32_t start_lds, unsigned long long *gsum,
+                        unsigned long long *gcnt) const
 {
     StageArgs a;
     a.nparts = nparts;
@@ -605,6 +602,9 @@ StageArgs Stager::args(uint32_t cur_lds, uint32_t start_lds) const
     a.rec4 = rec_wide ? (uint4 *)rec : nullptr;
     a.cap = cap;
     a.overflow = work + 1;
+    a.part_out = part_out;
+    a.gsum = gsum;
+    a.gcnt = gcnt;
     return a;
 }
 
