@@ -405,8 +405,24 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
+    // the columns of a tile are loaded one tile ahead: the next tile's keys
+    // and measures are issued after this tile's rows step and arrive while
+    // its counts, lists and events run (the barriers wait for LDS only)
+    int2 nk[4], nm1, nm2;
+    auto load_tile = [&](uint64_t tl) {
+        const uint64_t r = tl * R + 2u * tid;
+#pragma unroll
+        for (int t = 0; t < 4; t++) nk[t] = (a.used >> t) & 1u ? bload2(a.fk[t], r, rows) : make_int2(0, 0);
+        nm1 = bload2(a.m1, r, rows);
+        nm2 = a.m2 ? bload2(a.m2, r, rows) : make_int2(0, 0);
+    };
+    if (blockIdx.x < a.ntiles) load_tile(blockIdx.x);
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * R + 2u * tid;
+        int2 ck[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) ck[t] = nk[t];
+        const int2 cm1 = nm1, cm2 = nm2;
         // 1. rows
         unsigned long long al0 = r0 < rows ? ~0ull : 0ull;
         unsigned long long al1 = r0 + 1 < rows ? ~0ull : 0ull;
@@ -414,7 +430,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         for (int t = 0; t < 4; t++) {
             uint32_t at0 = 0u, at1 = 0u;
             if ((a.used >> t) & 1u) {
-                const int2 k = bload2(a.fk[t], r0, rows);
+                const int2 k = ck[t];
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
                     const uint32_t key = (uint32_t)(j ? k.y : k.x);
@@ -441,15 +457,14 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             }
         }
         if (al0 | al1) {
-            const int2 m = bload2(a.m1, r0, rows);
-            rm[2u * tid] = (uint32_t)m.x;
-            rm[2u * tid + 1u] = (uint32_t)m.y;
+            rm[2u * tid] = (uint32_t)cm1.x;
+            rm[2u * tid + 1u] = (uint32_t)cm1.y;
             if (a.m2) {
-                const int2 m2 = bload2(a.m2, r0, rows);
-                rm[R + 2u * tid] = (uint32_t)m2.x;
-                rm[R + 2u * tid + 1u] = (uint32_t)m2.y;
+                rm[R + 2u * tid] = (uint32_t)cm2.x;
+                rm[R + 2u * tid + 1u] = (uint32_t)cm2.y;
             }
         }
+        if (tile + gridDim.x < a.ntiles) load_tile(tile + gridDim.x);
         // 2. count: one LDS atomic per match on its query's counter; each lane
         // walks its bits from its own lane number on, so the lanes of one
         // atomic instruction mostly hit different counters

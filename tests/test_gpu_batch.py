@@ -164,3 +164,53 @@ def test_batch_rejects_bad_queries(ctx):
         ctx.batch(ds.star, [bad])
     with pytest.raises(oc.OlapError):
         ctx.batch(ds.star, [oc.BatchQuery.make([(1, 0, 10, 2)], [1], 2)] * 65)
+
+
+@pytest.mark.parametrize("aggregation", [1, 2])
+def test_batch_graph_replay_after_columns_grow_matches(ctx, aggregation):
+    """A graph captured with record buffers sized for few matches, replayed
+    after the fact columns change so that every row matches: the matches that
+    no longer fit the buffers go to the outputs directly, and the replay
+    equals a fresh batch over the new columns and the closed form."""
+    import torch
+
+    rows, D, G = 1_000_000, 50_000, 20_000
+    filt = torch.arange(D, dtype=torch.int32, device="cuda") % 100        # filter 0: 1 row in 100
+    attr = torch.arange(D, dtype=torch.int32, device="cuda") * 7 % 65536
+    fk = torch.full((rows,), 99, dtype=torch.int32, device="cuda")         # row 99 fails [0, 1)
+    fk[:1000] = torch.arange(1000, dtype=torch.int32, device="cuda") * 100 % D
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    m1 = torch.randint(-1000, 1000, (rows,), dtype=torch.int32, device="cuda", generator=gen)
+    star = oc.Context.star([filt, None, None, None], [attr, None, None, None],
+                           [fk, None, None, None], m1, None, rows=rows)
+    q = oc.BatchQuery.make([(0, 0, 1, G)], [1], G)
+    cfg = next(k for k in range(ctx.lib.olap_batch_num_configs())
+               if oc.config_info(k)[3] == aggregation and oc.config_info(k)[2] == 8)
+    _say(f"config {cfg} {oc.config_info(cfg)}")
+    b = ctx.batch(star, [q])
+    b.set_config(cfg)
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    b.run(osum, ocnt)
+    torch.cuda.synchronize()
+    assert int(ocnt.sum()) == 1000
+    stream = torch.cuda.Stream()
+    b.capture(osum, ocnt, stream.cuda_stream)
+    # every row now probes a passing dimension row
+    fk.copy_(torch.randint(0, D // 100, (rows,), dtype=torch.int32, device="cuda", generator=gen) * 100)
+    torch.cuda.synchronize()
+    b.launch(stream.cuda_stream)
+    torch.cuda.synchronize()
+    gid = (attr[fk.long()].long() % G)
+    es = torch.zeros(G, dtype=torch.int64, device="cuda").index_add_(0, gid, m1.long())
+    ec = torch.bincount(gid, minlength=G)
+    assert int(ocnt.sum()) == rows
+    assert torch.equal(ocnt, ec) and torch.equal(osum, es), "replay after the columns changed"
+    b2 = ctx.batch(star, [q])
+    b2.set_config(cfg)
+    fsum, fcnt = torch.empty_like(osum), torch.empty_like(ocnt)
+    b2.run(fsum, fcnt)
+    torch.cuda.synchronize()
+    assert torch.equal(fsum, es) and torch.equal(fcnt, ec), "fresh batch"
+    b2.close()
+    b.close()
