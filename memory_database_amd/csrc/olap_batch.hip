@@ -367,11 +367,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = src[i];
         }
     }
-    {
-        const uint32_t *src = (const uint32_t *)a.desc;
-        const uint32_t n = a.nq * kDescWords;
-        for (uint32_t i = threadIdx.x; i < n; i += BS) lds[a.desc_lds + i] = src[i];
-    }
+    // (descriptors are read from global memory with scalar loads: no LDS copy)
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
     if (threadIdx.x < 64u) lds[a.cnt_lds + threadIdx.x] = 0u;
     const uint32_t np = a.chunk.nparts;
@@ -407,8 +403,14 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     const uint32_t lane = tid & 63u;
     // the columns of a tile are loaded one tile ahead: the next tile's keys
     // and measures are issued after this tile's rows step and arrive while
-    // its counts, lists and events run (the barriers wait for LDS only)
+    // its counts, lists and events run (the barriers wait for LDS only). The
+    // one-pass form, whose LDS already limits it to two workgroups per CU,
+    // also gathers the next tile's records of dimension rows outside LDS
+    // before this tile's events (32 more registers per lane, which would cost
+    // the other forms occupancy)
+    constexpr bool kGatherAhead = MODE == AGG_CHUNK;
     int2 nk[4], nm1, nm2;
+    uint4 ng[4][2];
     auto load_tile = [&](uint64_t tl) {
         const uint64_t r = tl * R + 2u * tid;
 #pragma unroll
@@ -416,12 +418,34 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         nm1 = bload2(a.m1, r, rows);
         nm2 = a.m2 ? bload2(a.m2, r, rows) : make_int2(0, 0);
     };
-    if (blockIdx.x < a.ntiles) load_tile(blockIdx.x);
+    auto gather_tile = [&]() {
+        if (!kGatherAhead) return;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const uint32_t key = (uint32_t)(j ? nk[t].y : nk[t].x);
+                ng[t][j] = make_uint4(0u, 0u, 0u, 0u);
+                if (((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds && key < a.dim_rows[t]) ng[t][j] = a.grec[t][key];
+            }
+        }
+    };
+    if (blockIdx.x < a.ntiles) {
+        load_tile(blockIdx.x);
+        gather_tile();
+    }
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * R + 2u * tid;
+        const bool has_next = tile + gridDim.x < a.ntiles;
+        bool gathered = false;
         int2 ck[4];
+        uint4 cg[4][2];
 #pragma unroll
-        for (int t = 0; t < 4; t++) ck[t] = nk[t];
+        for (int t = 0; t < 4; t++) {
+            ck[t] = nk[t];
+            cg[t][0] = ng[t][0];
+            cg[t][1] = ng[t][1];
+        }
         const int2 cm1 = nm1, cm2 = nm2;
         // 1. rows
         unsigned long long al0 = r0 < rows ? ~0ull : 0ull;
@@ -443,7 +467,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                                 al &= w[key];
                                 at = L[a.attr_lds[t] + key];
                             } else {
-                                const uint4 rec = a.grec[t][key];
+                                const uint4 rec = kGatherAhead ? cg[t][j] : a.grec[t][key];
                                 al &= (unsigned long long)rec.x | ((unsigned long long)rec.y << 32);
                                 at = rec.z;
                             }
@@ -464,7 +488,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 rm[R + 2u * tid + 1u] = (uint32_t)cm2.y;
             }
         }
-        if (tile + gridDim.x < a.ntiles) load_tile(tile + gridDim.x);
+        if (has_next) load_tile(tile + gridDim.x);
         // 2. count: one LDS atomic per match on its query's counter; each lane
         // walks its bits from its own lane number on, so the lanes of one
         // atomic instruction mostly hit different counters
@@ -519,6 +543,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t end = seg[tid] + ((qreal[tid] + 63u) & ~63u);
                 for (uint32_t p = seg[tid] + qreal[tid]; p < end; p++) ev[p - lo] = 0xffffu | (tid << 16);
             }
+            if (has_next && !gathered) gather_tile();
+            gathered = true;
             lds_barrier();
             // 5. events: each wave takes whole 64-event chunks, all of one query
             const uint32_t chunks = n >> 6;
@@ -663,6 +689,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             }
             qa = qb;
         }
+        if (has_next && !gathered) gather_tile();
     }
     if (MODE == AGG_CHUNK) {
         // fills of the chunks still open
@@ -884,7 +911,9 @@ olap_status plan_config(olap_batch b, int c)
     std::vector<BQDesc> desc((size_t)b->nq);
     uint32_t cur = 0;
     a.desc_lds = 0;
-    cur += up16((uint32_t)(sizeof(BQDesc) * b->nq));
+    // the row-wise batch kernel reads the descriptors from LDS, the
+    // event-grouped one from global memory (wave-uniform scalar loads)
+    if (!cfg.grouped) cur += up16((uint32_t)(sizeof(BQDesc) * b->nq));
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
     // staged configurations keep a u32 counter and a u64 range start per partition
     const uint32_t stage_bytes = cfg.staged == 1 ? up16(b->nparts * 4u) + b->nparts * 8u
