@@ -52,6 +52,7 @@ struct BatchArgs {
     const unsigned long long *gmask[4];
     const int32_t *gattr[4];
     const uint4 *grec[4];   // per dimension row: mask (2 words), attribute, 0
+    const uint4 *grec_any;  // the records of some probed table (a valid address)
     uint32_t dim_rows[4];
     uint32_t mask_lds[4];   // word offset of the LDS copy or kNoLds
     uint32_t attr_lds[4];
@@ -331,14 +332,14 @@ __device__ __forceinline__ void lds_barrier()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// the first set bit of w (non-zero) at or after bit r, wrapping around
-__device__ __forceinline__ uint32_t next_bit_from(unsigned long long w, uint32_t r)
+// w rotated right by r (0..63)
+__device__ __forceinline__ unsigned long long rotr64(unsigned long long w, uint32_t r)
 {
-    const unsigned long long rot = (w >> r) | (w << ((64u - r) & 63u));
-    return ((uint32_t)__ffsll(rot) - 1u + r) & 63u;
+    return (w >> r) | (w << ((64u - r) & 63u));
 }
 
 typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v2i_a4 __attribute__((ext_vector_type(2), aligned(4)));  // a pair at any 4-byte offset
 
 __device__ __forceinline__ int2 bload2(const int32_t *__restrict__ p, uint64_t r, uint64_t rows)
 {
@@ -408,26 +409,45 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     // also gathers the next tile's records of dimension rows outside LDS
     // before this tile's events (32 more registers per lane, which would cost
     // the other forms occupancy)
+    //
+    // The loads are unconditional, so the loaded registers need no copies
+    // (a copy would wait for the load at once): a pair that reaches past the
+    // end reads the last pair in bounds, rows - 2 .. rows - 1 (rows >= 2; a
+    // 4-byte-aligned load), and takes its second element for its first row;
+    // rows past the end are masked out by al0/al1. The last tile loads
+    // itself again, unused columns read m1, and a gather off a table kept in
+    // LDS reads record 0 of the table grec_any.
     constexpr bool kGatherAhead = MODE == AGG_CHUNK;
+    const uint64_t rlast = rows - 2u;
+    const int32_t *colp[6];
+#pragma unroll
+    for (int t = 0; t < 4; t++) colp[t] = (a.used >> t) & 1u ? a.fk[t] : a.m1;
+    colp[4] = a.m1;
+    colp[5] = a.m2 ? a.m2 : a.m1;
     int2 nk[4], nm1, nm2;
     uint4 ng[4][2];
     auto load_tile = [&](uint64_t tl) {
         const uint64_t r = tl * R + 2u * tid;
+        const uint64_t rl = min(r, rlast);
+        const bool shifted = r != rl;
+        auto ld2 = [&](const int32_t *p) {
+            const v2i_a4 v = __builtin_nontemporal_load((const v2i_a4 *)(p + rl));
+            return make_int2(shifted ? v.y : v.x, v.y);
+        };
 #pragma unroll
-        for (int t = 0; t < 4; t++) nk[t] = (a.used >> t) & 1u ? bload2(a.fk[t], r, rows) : make_int2(0, 0);
-        nm1 = bload2(a.m1, r, rows);
-        nm2 = a.m2 ? bload2(a.m2, r, rows) : make_int2(0, 0);
+        for (int t = 0; t < 4; t++) nk[t] = ld2(colp[t]);
+        nm1 = ld2(colp[4]);
+        nm2 = ld2(colp[5]);
     };
     auto gather_tile = [&]() {
         if (!kGatherAhead) return;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
+            const bool hbm = ((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds;
+            const uint4 *g = hbm ? a.grec[t] : a.grec_any;
+            const uint32_t kmax = hbm ? max(a.dim_rows[t], 1u) - 1u : 0u;
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const uint32_t key = (uint32_t)(j ? nk[t].y : nk[t].x);
-                ng[t][j] = make_uint4(0u, 0u, 0u, 0u);
-                if (((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds && key < a.dim_rows[t]) ng[t][j] = a.grec[t][key];
-            }
+            for (int j = 0; j < 2; j++) ng[t][j] = g[min((uint32_t)(j ? nk[t].y : nk[t].x), kmax)];
         }
     };
     if (blockIdx.x < a.ntiles) {
@@ -436,8 +456,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     }
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * R + 2u * tid;
-        const bool has_next = tile + gridDim.x < a.ntiles;
-        bool gathered = false;
+        const uint64_t next = tile + gridDim.x < a.ntiles ? tile + gridDim.x : tile;
         int2 ck[4];
         uint4 cg[4][2];
 #pragma unroll
@@ -488,14 +507,15 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 rm[R + 2u * tid + 1u] = (uint32_t)cm2.y;
             }
         }
-        if (has_next) load_tile(tile + gridDim.x);
+        load_tile(next);
         // 2. count: one LDS atomic per match on its query's counter; each lane
         // walks its bits from its own lane number on, so the lanes of one
         // atomic instruction mostly hit different counters
         for (int j = 0; j < 2; j++) {
-            for (unsigned long long w = j ? al1 : al0; w;) {
-                const uint32_t q = next_bit_from(w, lane);
-                w &= ~(1ull << q);
+            // the mask rotated right by the lane number: bit b is query
+            // (b + lane) mod 64, and w & (w - 1) drops the bit just taken
+            for (unsigned long long w = rotr64(j ? al1 : al0, lane); w; w &= w - 1ull) {
+                const uint32_t q = ((uint32_t)__builtin_ctzll(w) + lane) & 63u;
                 __hip_atomic_fetch_add(qcnt + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
@@ -517,6 +537,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             if (tid == 63u) seg[64] = incl;
         }
         lds_barrier();
+        gather_tile();  // the next tile's keys have had the count and offset steps to arrive
         const uint32_t total = seg[64];
         // rounds over runs of whole query segments that fit the list (one
         // query's padded segment, at most R events, always fits)
@@ -531,9 +552,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             // grouped order (row in bits 0..15, query in bits 16..21); the
             // padding slots of a segment hold row 0xffff
             for (int j = 0; j < 2; j++) {
-                for (unsigned long long w = (j ? al1 : al0) & qmask; w;) {
-                    const uint32_t q = next_bit_from(w, lane);
-                    w &= ~(1ull << q);
+                for (unsigned long long w = rotr64((j ? al1 : al0) & qmask, lane); w; w &= w - 1ull) {
+                    const uint32_t q = ((uint32_t)__builtin_ctzll(w) + lane) & 63u;
                     const uint32_t pos = __hip_atomic_fetch_add(qcur + q, 1u, __ATOMIC_RELAXED,
                                                                 __HIP_MEMORY_SCOPE_WORKGROUP) - lo;
                     ev[pos] = (2u * tid + (uint32_t)j) | (q << 16);
@@ -543,8 +563,6 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t end = seg[tid] + ((qreal[tid] + 63u) & ~63u);
                 for (uint32_t p = seg[tid] + qreal[tid]; p < end; p++) ev[p - lo] = 0xffffu | (tid << 16);
             }
-            if (has_next && !gathered) gather_tile();
-            gathered = true;
             lds_barrier();
             // 5. events: each wave takes whole 64-event chunks, all of one query
             const uint32_t chunks = n >> 6;
@@ -556,16 +574,16 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 if (MODE == AGG_CHUNK) ev[ch * 64u + lane] = 0xffffffffu;  // no record unless staged below
                 const c_desc &d = ((const c_desc *)a.desc)[q];
                 const uint32_t G = d.G;
-                const uint32_t ns = d.nsteps;
-                // the step attributes are read together, then combined
+                // all four steps, without branches: a step past nsteps (or of
+                // a filter-only dimension) has group_mod 0 and multiplier 0,
+                // so it adds 0; the descriptor's scalar loads issue together
                 uint32_t av[4];
 #pragma unroll
-                for (uint32_t st = 0; st < 4u; st++) av[st] = st < ns ? rattr[d.tab[st] * R + row] : 0u;
+                for (uint32_t st = 0; st < 4u; st++) av[st] = rattr[d.tab[st] * R + row];
                 uint32_t gid = 0u;
 #pragma unroll
                 for (uint32_t st = 0; st < 4u; st++)
-                    if (st < ns)
-                        gid += (uint32_t)__umul64hi(d.M[st] * (uint64_t)av[st], (uint64_t)d.g[st]) * d.fac[st];
+                    gid += (uint32_t)__umul64hi(d.M[st] * (uint64_t)av[st], (uint64_t)d.g[st]) * d.fac[st];
                 const bool ok = valid && gid < G;
                 if (valid && gid >= G)
                     __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -689,7 +707,6 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             }
             qa = qb;
         }
-        if (has_next && !gathered) gather_tile();
     }
     if (MODE == AGG_CHUNK) {
         // fills of the chunks still open
@@ -931,6 +948,10 @@ olap_status plan_config(olap_batch b, int c)
         a.m_lds = cur / 4u;
         cur += (b->star.m2 ? 8u : 4u) * R;
     }
+    // the event-grouped kernel reads the fact columns in row pairs with no
+    // tail branch (the last pair in bounds stands in for pairs past the end)
+    if (cfg.grouped && b->star.rows < 2)
+        return bfail(ctx, OLAP_E_CAPACITY, "batch config " + std::to_string(c) + ": needs at least 2 fact rows");
     if ((uint64_t)cur + stage_bytes + ev_min + 1024u > kLdsBytes)
         return bfail(ctx, OLAP_E_CAPACITY, "batch config " + std::to_string(c) + ": LDS too small for " +
                                                std::to_string(b->nparts) + " partitions");
@@ -1014,6 +1035,8 @@ olap_status plan_config(olap_batch b, int c)
         a.dim_rows[t] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
         a.nouse[t] = b->nouse[t];
     }
+    for (int t = 3; t >= 0; t--)
+        if ((b->used >> t) & 1u) a.grec_any = b->ws->rec[t];
     a.m1 = b->star.m1;
     a.m2 = b->star.m2;
     a.rows = (uint64_t)b->star.rows;
