@@ -540,12 +540,16 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         gather_tile();  // the next tile's keys have had the count and offset steps to arrive
         const uint32_t total = seg[64];
         // rounds over runs of whole query segments that fit the list (one
-        // query's padded segment, at most R events, always fits)
-        for (uint32_t qa = 0; qa < 64u && seg[qa] < total;) {
-            uint32_t qb = qa + 1u;
-            while (qb < 64u && seg[qb + 1u] - seg[qa] <= a.ev_cap) qb++;
-            const uint32_t lo = seg[qa];
-            const uint32_t n = seg[qb] - lo;  // a multiple of 64, at most ev_cap
+        // query's padded segment, at most R events, always fits). Lane q
+        // holds the end of query q's segment; the ends do not decrease, so
+        // the queries from qa on that end within ev_cap of the round's start
+        // are a run, counted with one ballot
+        const uint32_t seg_end = seg[lane + 1u];
+        for (uint32_t qa = 0, lo = 0; qa < 64u && lo < total;) {
+            const unsigned long long fit = __ballot(lane >= qa && seg_end - lo <= a.ev_cap);
+            const uint32_t qb = max(qa + 1u, qa + (uint32_t)__popcll(fit));
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)seg_end, (int)(qb - 1u));
+            const uint32_t n = hi - lo;  // a multiple of 64, at most ev_cap
             const unsigned long long qmask =
                 (qb == 64u ? ~0ull : ((1ull << qb) - 1ull)) & ~((1ull << qa) - 1ull);
             // 4. list: a cursor atomic per match gives its slot in the
@@ -706,6 +710,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 lds_barrier();
             }
             qa = qb;
+            lo = hi;
         }
     }
     if (MODE == AGG_CHUNK) {
