@@ -75,11 +75,10 @@ struct BatchArgs {
     uint32_t m_lds;
     uint32_t ev_lds;
     uint32_t ev_cap;
-    // one-pass staging: per-partition chunk state in LDS (u64 base, t1b, t2b,
-    // then u32 rem, id, cnt, t1n, nparts each) and a second event word array
+    // one-pass staging: per-partition chunk state in LDS (u64 current and
+    // spare chunk, then u32 fill, nparts each)
     ChunkArgs chunk;
     uint32_t pc_lds;
-    uint32_t evr_lds;
 };
 
 struct MaskPred {
@@ -332,6 +331,20 @@ __device__ __forceinline__ void lds_barrier()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+constexpr unsigned long long kNoChunk = ~0ull;
+
+// claim a chunk of the pool for partition p: its first record index, or
+// kNoChunk when the pool is used up
+__device__ __forceinline__ unsigned long long chunk_claim(const ChunkArgs &c, uint32_t p)
+{
+    const unsigned long long ch = 1ull << c.ch_bits;
+    const unsigned long long nc =
+        __hip_atomic_fetch_add(c.cursor, ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nc + ch > c.cap) return kNoChunk;
+    c.chunk_part[nc >> c.ch_bits] = p;
+    return nc;
+}
+
 // w rotated right by r (0..63)
 __device__ __forceinline__ unsigned long long rotr64(unsigned long long w, uint32_t r)
 {
@@ -371,19 +384,19 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     // (descriptors are read from global memory with scalar loads: no LDS copy)
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
     if (threadIdx.x < 64u) lds[a.cnt_lds + threadIdx.x] = 0u;
+    // one-pass staging: per partition a current chunk and a spare (pool
+    // record indexes, kNoChunk when the pool ran out) and the records handed
+    // out in the current chunk (positions past its end go to the spare)
     const uint32_t np = a.chunk.nparts;
-    stage_l_u64 *pc_base = (stage_l_u64 *)((stage_l_u32 *)lds + a.pc_lds);
-    stage_l_u64 *pc_t1b = pc_base + np;
-    stage_l_u64 *pc_t2b = pc_t1b + np;
-    stage_l_u32 *pc_rem = (stage_l_u32 *)(pc_t2b + np);
-    stage_l_u32 *pc_id = pc_rem + np;
-    stage_l_u32 *pc_cnt = pc_id + np;
-    stage_l_u32 *pc_t1n = pc_cnt + np;
+    const uint32_t CH = 1u << a.chunk.ch_bits;
+    stage_l_u64 *pc_cur = (stage_l_u64 *)((stage_l_u32 *)lds + a.pc_lds);
+    stage_l_u64 *pc_spare = pc_cur + np;
+    stage_l_u32 *pc_fill = (stage_l_u32 *)(pc_spare + np);
     if (MODE == AGG_CHUNK) {
         for (uint32_t p = threadIdx.x; p < np; p += BS) {
-            pc_rem[p] = 0u;
-            pc_id[p] = 0xffffffffu;
-            pc_cnt[p] = 0u;
+            pc_cur[p] = chunk_claim(a.chunk, p);
+            pc_spare[p] = chunk_claim(a.chunk, p);
+            pc_fill[p] = 0u;
         }
     }
     __syncthreads();
@@ -395,7 +408,6 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     l_u32 *rattr = L + a.row_lds;         // [4][R]
     l_u32 *rm = L + a.m_lds;              // [R] m1, then [R] m2
     l_u32 *ev = L + a.ev_lds;             // [ev_cap]
-    l_u32 *evr = L + a.evr_lds;           // [ev_cap] one-pass: gid within partition | row << 13
     stage_l_u32 *scur = (stage_l_u32 *)lds + a.stage.cur_lds;
     const stage_l_u64 *sstart = (const stage_l_u64 *)((stage_l_u32 *)lds + a.stage.start_lds);
     const BQDesc *D = a.desc;
@@ -575,7 +587,6 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t q = __builtin_amdgcn_readfirstlane(e >> 16);
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
-                if (MODE == AGG_CHUNK) ev[ch * 64u + lane] = 0xffffffffu;  // no record unless staged below
                 const c_desc &d = ((const c_desc *)a.desc)[q];
                 const uint32_t G = d.G;
                 // all four steps, without branches: a step past nsteps (or of
@@ -607,8 +618,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 } else {
                     const uint32_t part = d.pbase + (gid >> kPartBits);
                     // counters: the COUNT pass's per-partition counts, the
-                    // SCATTER pass's cursors, or the one-pass round counts
-                    stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_cnt : scur;
+                    // SCATTER pass's cursors, or the one-pass chunk fills
+                    stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_fill : scur;
                     uint32_t pos = 0u;
                     if (G <= kPartSlots) {
                         // one partition: one counter / cursor atomic for the wave
@@ -625,8 +636,22 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     if (MODE == AGG_CHUNK && ok) {
-                        ev[ch * 64u + lane] = (part << 11) | pos;
-                        evr[ch * 64u + lane] = (gid & (kPartSlots - 1u)) | (row << 13);
+                        // a round adds at most R = CH records to a partition
+                        // whose fill is below CH: pos < 2 CH
+                        const unsigned long long base = pos < CH ? pc_cur[part] : pc_spare[part];
+                        const unsigned long long idx = base + (pos < CH ? pos : pos - CH);
+                        const uint32_t glow = gid & (kPartSlots - 1u);
+                        if (base == kNoChunk) {
+                            const unsigned long long o = a.chunk.part_out[part] + glow;
+                            b_atomic_dev(&a.gsum[o], (unsigned long long)((long long)(int32_t)rm[row] +
+                                                                          (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll)));
+                            b_atomic_dev(&a.gcnt[o], 1ull);
+                            __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else if (a.m2) {
+                            ((uint4 *)a.chunk.pool)[idx] = make_uint4(glow, rm[row], rm[R + row], 0u);
+                        } else {
+                            ((uint2 *)a.chunk.pool)[idx] = make_uint2(glow, rm[row]);
+                        }
                     }
                     if (MODE == AGG_SCATTER && ok) {
                         const unsigned long long idx = sstart[part] + pos;
@@ -643,69 +668,17 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             }
             lds_barrier();
             if (MODE == AGG_CHUNK) {
-                // 5b. slots for this round's matches: the rest of each
-                // partition's current chunk, then (one global atomic) a new
-                // chunk; a chunk holds R records, a round's matches of one
-                // partition at most R
-                const uint32_t CH = 1u << a.chunk.ch_bits;
+                // 6. full chunks retire (fill CH), the spare becomes the
+                // current chunk and a new spare is claimed
                 for (uint32_t p = tid; p < np; p += BS) {
-                    const uint32_t c = pc_cnt[p];
-                    if (!c) continue;
-                    pc_cnt[p] = 0u;
-                    const uint32_t rem = pc_rem[p];
-                    pc_t1b[p] = pc_base[p];
-                    if (c <= rem) {
-                        pc_t1n[p] = c;
-                        pc_base[p] += c;
-                        pc_rem[p] = rem - c;
-                    } else {
-                        pc_t1n[p] = rem;
-                        if (pc_id[p] != 0xffffffffu) a.chunk.chunk_fill[pc_id[p]] = CH;
-                        const unsigned long long nc = __hip_atomic_fetch_add(
-                            a.chunk.cursor, (unsigned long long)CH, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (nc + CH <= a.chunk.cap) {
-                            const uint32_t id = (uint32_t)(nc >> a.chunk.ch_bits);
-                            a.chunk.chunk_part[id] = p;
-                            pc_t2b[p] = nc;
-                            pc_base[p] = nc + (c - rem);
-                            pc_rem[p] = CH - (c - rem);
-                            pc_id[p] = id;
-                        } else {
-                            pc_t2b[p] = ~0ull;  // pool exhausted: global atomics below
-                            pc_rem[p] = 0u;
-                            pc_id[p] = 0xffffffffu;
-                        }
+                    const uint32_t f = pc_fill[p];
+                    if (f >= CH) {
+                        const unsigned long long cur = pc_cur[p];
+                        if (cur != kNoChunk) a.chunk.chunk_fill[cur >> a.chunk.ch_bits] = CH;
+                        pc_cur[p] = pc_spare[p];
+                        pc_fill[p] = f - CH;
+                        pc_spare[p] = chunk_claim(a.chunk, p);
                     }
-                }
-                lds_barrier();
-                // 5c. write the records
-                for (uint32_t i = tid; i < n; i += BS) {
-                    const uint32_t e = ev[i];
-                    if (e == 0xffffffffu) continue;
-                    const uint32_t p = e >> 11, k = e & 2047u;
-                    const uint32_t r = evr[i];
-                    const uint32_t glow = r & (kPartSlots - 1u), row = r >> 13;
-                    const uint32_t t1n = pc_t1n[p];
-                    unsigned long long idx;
-                    if (k < t1n) {
-                        idx = pc_t1b[p] + k;
-                    } else {
-                        const unsigned long long t2 = pc_t2b[p];
-                        if (t2 == ~0ull) {
-                            const unsigned long long mv = (unsigned long long)(
-                                (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
-                            const unsigned long long o = a.chunk.part_out[p] + glow;
-                            b_atomic_dev(&a.gsum[o], mv);
-                            b_atomic_dev(&a.gcnt[o], 1ull);
-                            __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            continue;
-                        }
-                        idx = t2 + (k - t1n);
-                    }
-                    if (a.m2)
-                        ((uint4 *)a.chunk.pool)[idx] = make_uint4(glow, rm[row], rm[R + row], 0u);
-                    else
-                        ((uint2 *)a.chunk.pool)[idx] = make_uint2(glow, rm[row]);
                 }
                 lds_barrier();
             }
@@ -714,9 +687,11 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         }
     }
     if (MODE == AGG_CHUNK) {
-        // fills of the chunks still open
-        for (uint32_t p = tid; p < np; p += BS)
-            if (pc_id[p] != 0xffffffffu) a.chunk.chunk_fill[pc_id[p]] = (1u << a.chunk.ch_bits) - pc_rem[p];
+        // fills of the chunks still open (fill < CH after the last round)
+        for (uint32_t p = tid; p < np; p += BS) {
+            if (pc_cur[p] != kNoChunk) a.chunk.chunk_fill[pc_cur[p] >> a.chunk.ch_bits] = pc_fill[p];
+            if (pc_spare[p] != kNoChunk) a.chunk.chunk_fill[pc_spare[p] >> a.chunk.ch_bits] = 0u;
+        }
     }
     __syncthreads();
     if (MODE == AGG_COUNT) stage_end<MODE>(a.stage, lds);
@@ -757,7 +732,8 @@ const BatchCfg kBatchCfgs[] = {
     {512, 1, 0, 1, 1, 0, 2},    {512, 1, 16, 1, 1, 1, 16},  {512, 1, 16, 1, 1, 0, 4},
     {512, 1, 8, 1, 1, 0, 4},    {512, 1, 8, 2, 1, 0, 2},    {512, 1, 0, 2, 1, 0, 2},
     {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4},    {512, 1, 0, 2, 1, 0, 4},
-    {512, 1, 0, 2, 1, 0, 3}};
+    {512, 1, 0, 2, 1, 0, 3},    {512, 1, 8, 2, 1, 0, 6},    {512, 1, 8, 2, 1, 0, 8},
+    {1024, 1, 8, 2, 1, 0, 4},   {512, 1, 8, 1, 1, 0, 8},    {512, 1, 16, 2, 1, 0, 4}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
 template <int MODE>
@@ -939,11 +915,11 @@ olap_status plan_config(olap_batch b, int c)
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
     // staged configurations keep a u32 counter and a u64 range start per partition
     const uint32_t stage_bytes = cfg.staged == 1 ? up16(b->nparts * 4u) + b->nparts * 8u
-                                 : cfg.staged == 2 ? up16(b->nparts * 40u) : 0u;
+                                 : cfg.staged == 2 ? up16(b->nparts * 20u) : 0u;
     // the event-grouped kernel stages per-tile rows and at least one event
     // per row of the tile
     const uint32_t R = (uint32_t)batch_tile_rows(cfg);
-    const uint32_t ev_bytes = cfg.staged == 2 ? 8u : 4u;  // one-pass keeps a second event word
+    const uint32_t ev_bytes = 4u;
     const uint32_t ev_min = cfg.grouped ? ev_bytes * (R + 64u) : 0u;
     if (cfg.grouped) {
         a.cnt_lds = cur / 4u;
@@ -1008,8 +984,6 @@ olap_status plan_config(olap_batch b, int c)
                                       (kLdsBytes - 1024u - cur) / ev_bytes) & ~63u;
         a.ev_lds = cur / 4u;
         cur += 4u * a.ev_cap;
-        a.evr_lds = cur / 4u;
-        if (cfg.staged == 2) cur += 4u * a.ev_cap;
     }
     uint32_t pbase = 0;
     for (int q = 0; q < b->nq; q++) {

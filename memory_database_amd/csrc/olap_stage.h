@@ -31,13 +31,15 @@ constexpr uint32_t kMaxParts = 1u << 16;
 enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK = 3 };
 
 // One-pass staging (AGG_CHUNK): instead of a COUNT pass that sizes exact
-// ranges, a workgroup hands out record slots from its current chunk of
-// kChunkRecs-aligned slots of a shared pool, one chunk per partition, and
-// claims a new chunk with one global atomic when a chunk runs out. Each chunk
-// records its partition and fill; after the scan the chunks are grouped by
-// partition and reduced like the ranges of the two-pass form. A pool that runs
-// out (it is sized from the expected matches) never loses a match: the scan
-// adds such matches to the outputs with global atomics.
+// ranges, a workgroup keeps per partition a current chunk of a shared pool
+// and a spare one, and hands out record slots with an LDS counter: positions
+// past the current chunk's end fall into the spare, so every match writes its
+// record at once. Between rounds of matches, full chunks retire, the spare
+// becomes current and a new spare is claimed with one global atomic. Each
+// chunk records its partition and fill; after the scan the chunks are grouped
+// by partition and reduced like the ranges of the two-pass form. A pool that
+// runs out (it is sized from the expected matches) never loses a match: the
+// scan adds such matches to the outputs with global atomics.
 struct ChunkArgs {
     void *pool;                       // uint2 or uint4 records
     unsigned long long cap;           // records the pool holds

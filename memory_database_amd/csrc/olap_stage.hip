@@ -499,9 +499,10 @@ hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, in
 hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint32_t ch_bits)
 {
     const unsigned long long ch = 1ull << ch_bits;
-    // expected matches + 25%, plus the partial chunk every workgroup can
-    // leave in every partition
-    unsigned long long want = records + records / 4 + (unsigned long long)std::max<uint32_t>(nparts, 1) * nwg_ * ch + ch;
+    // expected matches + 25%, plus the two chunks every workgroup holds open
+    // in every partition (a partial current chunk and a spare)
+    unsigned long long want =
+        records + records / 4 + 2ull * (unsigned long long)std::max<uint32_t>(nparts, 1) * nwg_ * ch + ch;
     want = (want + ch - 1) & ~(ch - 1);
     hipError_t e = hipSuccess;
     if (!pool_cursor) e = hipMalloc(&pool_cursor, sizeof(unsigned long long));
