@@ -650,7 +650,12 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         } else if (a.m2) {
                             ((uint4 *)a.chunk.pool)[idx] = make_uint4(glow, rm[row], rm[R + row], 0u);
                         } else {
-                            ((uint2 *)a.chunk.pool)[idx] = make_uint2(glow, rm[row]);
+                            // one measure: a chunk holds CH u32 measures,
+                            // then CH u16 slots (6 bytes a record)
+                            char *cp = (char *)a.chunk.pool + 6ull * base;
+                            const uint32_t off = (uint32_t)(idx - base);
+                            ((uint32_t *)cp)[off] = rm[row];
+                            ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
                         }
                     }
                     if (MODE == AGG_SCATTER && ok) {

@@ -41,7 +41,8 @@ enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK =
 // runs out (it is sized from the expected matches) never loses a match: the
 // scan adds such matches to the outputs with global atomics.
 struct ChunkArgs {
-    void *pool;                       // uint2 or uint4 records
+    void *pool;                       // chunks of CH records: CH u32 measures then CH u16
+                                      // slots (one measure), or CH uint4 records (two)
     unsigned long long cap;           // records the pool holds
     unsigned long long *cursor;       // records handed out (whole chunks)
     uint32_t *chunk_part;             // [cap / chunk] partition of each chunk

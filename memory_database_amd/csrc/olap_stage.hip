@@ -332,8 +332,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
     __shared__ unsigned long long lord[kReduceItemChunks];  // the item's (fill << 32 | chunk id)
     __shared__ uint32_t s_item, s_part;
     const uint4 *v4 = (const uint4 *)pool;
-    // 16-byte units per chunk: record pairs (narrow) or records (wide)
-    const uint32_t ubits = WIDE ? ch_bits : ch_bits - 1u;
+    // units of a chunk: single 16-byte records (wide), or four records of a
+    // one-measure chunk (CH u32 measures, then CH u16 slots): 16 bytes of
+    // measures and 8 bytes of slots
+    const uint32_t ubits = WIDE ? ch_bits : ch_bits - 2u;
+    const unsigned long long ch = 1ull << ch_bits;
     for (;;) {
         reduce_claim(work, item_begin, nparts, s_item, s_part);
         __syncthreads();
@@ -352,19 +355,30 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
         const uint32_t units = n << ubits;
         for (uint32_t i0 = threadIdx.x; i0 < units; i0 += (uint32_t)kReduceThreads * kReduceLoads) {
             uint4 v[kReduceLoads];
+            uint2 gs[kReduceLoads];
             uint32_t r[kReduceLoads], fill[kReduceLoads];
 #pragma unroll
             for (int u = 0; u < kReduceLoads; u++) {
                 const uint32_t i = i0 + (uint32_t)u * kReduceThreads;
-                v[u] = make_uint4(kPartSlots, 0u, kPartSlots, 0u);
+                v[u] = make_uint4(0u, 0u, 0u, 0u);
+                gs[u] = make_uint2(0u, 0u);
                 r[u] = 0u;
                 fill[u] = 0u;
                 if (i < units) {
                     const unsigned long long o = lord[i >> ubits];
                     const uint32_t w = i & ((1u << ubits) - 1u);
-                    r[u] = WIDE ? w : 2u * w;  // record index within the chunk
+                    const unsigned long long cbase = (unsigned long long)(uint32_t)o << ch_bits;  // first record
+                    r[u] = WIDE ? w : 4u * w;  // first record index of the unit within the chunk
                     fill[u] = (uint32_t)(o >> 32);
-                    if (r[u] < fill[u]) v[u] = v4[((size_t)(uint32_t)o << ubits) + w];
+                    if (r[u] < fill[u]) {
+                        if (WIDE) {
+                            v[u] = v4[cbase + w];
+                        } else {
+                            const char *cp = (const char *)pool + 6ull * cbase;
+                            v[u] = ((const uint4 *)cp)[w];
+                            gs[u] = ((const uint2 *)(cp + 4ull * ch))[w];
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -373,9 +387,12 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
                     reduce_add(lsum, lcnt, slots, r[u] < fill[u] ? v[u].x : kPartSlots,
                                (long long)(int32_t)v[u].y + (long long)(int32_t)v[u].z);
                 } else {
-                    reduce_add(lsum, lcnt, slots, r[u] < fill[u] ? v[u].x : kPartSlots, (long long)(int32_t)v[u].y);
-                    reduce_add(lsum, lcnt, slots, r[u] + 1u < fill[u] ? v[u].z : kPartSlots,
-                               (long long)(int32_t)v[u].w);
+                    const uint32_t m[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                    const uint32_t g[4] = {gs[u].x & 0xffffu, gs[u].x >> 16, gs[u].y & 0xffffu, gs[u].y >> 16};
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        reduce_add(lsum, lcnt, slots, r[u] + (uint32_t)k < fill[u] ? g[k] : kPartSlots,
+                                   (long long)(int32_t)m[k]);
                 }
             }
         }
@@ -516,7 +533,7 @@ hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint
         chunk_order = nullptr;
         pool_cap = 0;
         const size_t nch = (size_t)(want >> ch_bits);
-        e = hipMalloc(&pool, (size_t)want * (two_measures ? 16u : 8u));
+        e = hipMalloc(&pool, (size_t)want * (two_measures ? 16u : 6u));
         if (e == hipSuccess) e = hipMalloc(&chunk_part, nch * 4);
         if (e == hipSuccess) e = hipMalloc(&chunk_fill, nch * 4);
         if (e == hipSuccess) e = hipMalloc(&chunk_order, nch * 8);

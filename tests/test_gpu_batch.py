@@ -85,6 +85,50 @@ def test_batch_matches_oracle_every_config(ctx, ncpu):
     b.close()
 
 
+def test_batch_two_measures_every_config(ctx, ncpu):
+    """A star with two measures (SUM of m1 + m2, 16-byte staged records):
+    every configuration equals the oracle, measures spanning the int32 range."""
+    import torch
+
+    rows = 300_007
+    cfg = wl.scaled(wl.STAR_4DIM_600M, rows, name="batch_m2")
+    ds = wl.DeviceBatchStar(ctx, cfg)
+    qs = wl.batch_queries(cfg, 24, 13, max_groups=30_000)
+    host = list(host_star(cfg, rows))
+    rng = np.random.default_rng(13)
+    m1 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    dm1, dm2 = torch.from_numpy(m1).cuda(), torch.from_numpy(m2).cuda()
+    star = oc.Context.star(ds.filters, ds.attrs, ds.fks, dm1, dm2, rows=rows)
+    b = ctx.batch(star, [oc.BatchQuery.make(*q) for q in qs])
+    expect = []
+    filters, attrs, fks, _ = host
+    for steps, factor, G in qs:
+        dvs = [None] * 4
+        for (t, lo, hi, g) in steps:
+            dvs[t] = po.build_dimvec(filters[t], attrs[t], lo, hi, 1 if g > 0 else 0, 4, group_mod=g)
+        p = po.plan([st[0] for st in steps], factor, G)
+        es, ec, bad = po.run("rowwise", p, po.Star(dvs, fks, m1, m2), nthreads=ncpu)
+        assert bad == 0
+        expect.append((es, ec))
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        try:
+            b.set_config(k)
+        except oc.OlapError as e:
+            _say(f"  config {k} skipped: {e}")
+            continue
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        s, c = osum.cpu().numpy(), ocnt.cpu().numpy()
+        for i, (es, ec) in enumerate(expect):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            np.testing.assert_array_equal(c[lo:hi], ec, err_msg=f"config {k} query {i} counts")
+            np.testing.assert_array_equal(s[lo:hi], es, err_msg=f"config {k} query {i} sums")
+    b.close()
+
+
 def test_batch_small_groups_and_ragged_rows(ctx, ncpu):
     """Small group counts (LDS tables for every query), a ragged row count and
     a batch of one."""
