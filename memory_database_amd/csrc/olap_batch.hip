@@ -738,7 +738,8 @@ const BatchCfg kBatchCfgs[] = {
     {512, 1, 8, 1, 1, 0, 4},    {512, 1, 8, 2, 1, 0, 2},    {512, 1, 0, 2, 1, 0, 2},
     {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4},    {512, 1, 0, 2, 1, 0, 4},
     {512, 1, 0, 2, 1, 0, 3},    {512, 1, 8, 2, 1, 0, 6},    {512, 1, 8, 2, 1, 0, 8},
-    {1024, 1, 8, 2, 1, 0, 4},   {512, 1, 8, 1, 1, 0, 8},    {512, 1, 16, 2, 1, 0, 4}};
+    {1024, 1, 8, 2, 1, 0, 4},   {512, 1, 8, 1, 1, 0, 8},    {512, 1, 16, 2, 1, 0, 4},
+    {1024, 1, 0, 2, 1, 1, 6},   {1024, 1, 0, 2, 1, 1, 8},   {1024, 1, 4, 2, 1, 1, 6}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 
 template <int MODE>
