@@ -249,6 +249,13 @@ def bench_served(ctx, ds, rates, seconds, streams=2):
     qs = [oc.BatchQuery.make(*t) for t in serving.templates(256, seed=3)]
     srv = ctx.server(max_batch=64, streams=streams, memory_budget=32 << 30)
     ids = [srv.add_dataset(s.star) for s in stars]
+    # warm-up: full batches on every dataset for every worker, so the
+    # workers' grow-only staging has reached its size before any timing
+    srv.pause(True)
+    warm = [srv.submit(d, qs[i % len(qs)]) for d in ids for i in range(64 * streams)]
+    srv.pause(False)
+    for t in warm:
+        srv.wait(t)
     serving.run_open_loop(srv, ids, qs, serving.arrivals(200.0, 60, len(ids), len(qs), seed=99))
     out = []
     for r in rates:
@@ -296,7 +303,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the skew/groups/selectivity sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
-    ap.add_argument("--serve-rates", default="50,500,1000,2000,5000")
+    ap.add_argument("--serve-rates", default="50,500,1000,2000,3000,5000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (HBM-counter calibration)")

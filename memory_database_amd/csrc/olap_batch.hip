@@ -589,16 +589,22 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
                 const c_desc &d = ((const c_desc *)a.desc)[q];
                 const uint32_t G = d.G;
-                // all four steps, without branches: a step past nsteps (or of
-                // a filter-only dimension) has group_mod 0 and multiplier 0,
-                // so it adds 0; the descriptor's scalar loads issue together
-                uint32_t av[4];
-#pragma unroll
-                for (uint32_t st = 0; st < 4u; st++) av[st] = rattr[d.tab[st] * R + row];
-                uint32_t gid = 0u;
-#pragma unroll
-                for (uint32_t st = 0; st < 4u; st++)
-                    gid += (uint32_t)__umul64hi(d.M[st] * (uint64_t)av[st], (uint64_t)d.g[st]) * d.fac[st];
+                // the steps of the query (a wave-uniform branch on nsteps;
+                // a filter-only step has group_mod 0 and multiplier 0 and
+                // adds 0)
+                const uint32_t ns = d.nsteps;
+                auto step = [&](uint32_t st) {
+                    return (uint32_t)__umul64hi(d.M[st] * (uint64_t)rattr[d.tab[st] * R + row], (uint64_t)d.g[st]) *
+                           d.fac[st];
+                };
+                uint32_t gid = step(0);
+                if (ns > 1u) {
+                    gid += step(1);
+                    if (ns > 2u) {
+                        gid += step(2);
+                        if (ns > 3u) gid += step(3);
+                    }
+                }
                 const bool ok = valid && gid < G;
                 if (valid && gid >= G)
                     __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -43,16 +43,21 @@ struct Request {
     bool done = false;
 };
 
-// expected matches above which a batch uses the staged aggregation
-// (count / scatter / LDS reduce) rather than LDS tables plus global atomics;
-// measured crossover on 60M and 600M fact rows: between 1e7 and 3e7 matches
-// (The one-pass event-grouped scan is faster only for the heaviest batches,
-// 73 against 84 ms at 64 queries over 600M rows, and a worker using both
-// would keep both a two-pass record buffer and a one-pass pool of that size;
-// workers stay with two-pass staging.)
-constexpr double kStagedMatches = 2.0e7;
+// A batch with more expected matches than max(kStagedMatches, kStagedPerRow
+// x rows) uses the one-pass event-grouped scan with staged aggregation;
+// fewer go to LDS tables plus global atomics. Measured on 1-64 serving
+// templates over 6M, 60M and 600M fact rows: the one-pass scan wins from
+// about 1e6 (60M rows) and 7e6 matches (600M rows); below about 1e6 its
+// fixed cost (~0.24 ms) loses to the atomics layout. It beats two-pass
+// staging everywhere (38 against 80 ms at 64 queries over 600M rows), and
+// its record pool is smaller than a two-pass buffer, so workers keep one
+// staging workspace. Two-pass staging remains the fallback when the
+// one-pass layout does not fit (too many partitions for LDS, or one row).
+constexpr double kStagedMatches = 2.0e6;
+constexpr double kStagedPerRow = 0.015;
 constexpr int kCfgAtomics = 0;   // 512 threads, 48 KiB of LDS tables
-constexpr int kCfgStaged = 9;    // 1024 threads, 48 KiB of LDS tables, two-pass staging
+constexpr int kCfgOnePass = 30;  // 1024 threads, small dimension tables in LDS, one-pass staging
+constexpr int kCfgTwoPass = 9;   // 1024 threads, 48 KiB of LDS tables, two-pass staging
 
 }  // namespace
 
@@ -200,8 +205,9 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
     int cfg = -1;
     if (st == OLAP_OK) {
         const double matches = olap_batch_expected_events(b) * (double)star.rows;
-        cfg = matches > kStagedMatches ? kCfgStaged : kCfgAtomics;
+        cfg = matches > std::max(kStagedMatches, kStagedPerRow * (double)star.rows) ? kCfgOnePass : kCfgAtomics;
         st = olap_batch_set_config(b, cfg);
+        if (st == OLAP_E_CAPACITY && cfg == kCfgOnePass) st = olap_batch_set_config(b, cfg = kCfgTwoPass);
     }
     const size_t total = (size_t)off.back();
     if (st == OLAP_OK && !w.out.fit(std::max<size_t>(total, 1))) st = OLAP_E_NOMEM;

@@ -520,6 +520,9 @@ hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint
     // in every partition (a partial current chunk and a spare)
     unsigned long long want =
         records + records / 4 + 2ull * (unsigned long long)std::max<uint32_t>(nparts, 1) * nwg_ * ch + ch;
+    // a pool that has to grow grows by at least half, so a workload of
+    // growing batches reallocates a few times rather than at every batch
+    if (pool && want > pool_cap) want = std::max<unsigned long long>(want, pool_cap + pool_cap / 2);
     want = (want + ch - 1) & ~(ch - 1);
     hipError_t e = hipSuccess;
     if (!pool_cursor) e = hipMalloc(&pool_cursor, sizeof(unsigned long long));
