@@ -236,12 +236,14 @@ olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *str
  * the winner. Synchronises. Aggregation (the `staged` output of
  * olap_batch_config_info): 0 global atomics; 1 two-pass staging (a count
  * pass sizes exact record ranges; the run reads the record total back once,
- * a captured graph reuses the buffers sized by the run made before capture);
- * 2 one-pass staging in the event-grouped kernel (records go to chunks of a
- * pool sized from the expected matches, no host read; matches beyond the
- * pool are added with global atomics). rows_per_thread 2 marks the
- * event-grouped kernel. Configurations whose LDS layout does not fit the
- * batch (too many aggregation partitions) are skipped. */
+ * a captured graph reuses the buffers sized by the run made before capture;
+ * matches beyond those buffers, after the columns have changed, are added
+ * with global atomics); 2 one-pass staging in the event-grouped kernel
+ * (records go to chunks of a pool sized from the expected matches, no host
+ * read; matches beyond the pool are added with global atomics).
+ * rows_per_thread 2 marks the event-grouped kernel. Configurations whose LDS
+ * layout does not fit the batch (too many aggregation partitions), and
+ * event-grouped configurations on fewer than 2 fact rows, are skipped. */
 int32_t olap_batch_num_configs(void);
 olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
                                    int32_t *lds_table_kib, int32_t *staged);
