@@ -293,31 +293,54 @@ __device__ __forceinline__ void store4(int32_t *__restrict__ p, uint64_t r, uint
 }
 
 // probe one key: returns false when the key finds no row or a NULL code
+// IN_LDS selects typed pointers: an LDS-resident vector is read with
+// ds_read, which waits for LDS only (a generic pointer would make it a flat
+// load, and a flat load's wait also waits for the next tile's prefetched
+// columns)
+template <bool IN_LDS>
 __device__ __forceinline__ bool probe1(const StepDesc &st, const uint32_t *lds, int32_t key,
                                        uint32_t &code)
 {
     const uint32_t idx = (uint32_t)key;
     const bool inb = idx < st.dim_rows;
     const uint32_t wi = inb ? (idx >> (5u - st.lw)) : 0u;
-    const uint32_t word = st.in_lds ? lds[st.lds_off + wi] : st.words[wi];
+    const uint32_t word = IN_LDS ? ((const __attribute__((address_space(3))) uint32_t *)lds)[st.lds_off + wi]
+                                 : ((const __attribute__((address_space(1))) uint32_t *)st.words)[wi];
     code = (word >> ((idx & ((32u >> st.lw) - 1u)) << st.lw)) & st.mask;
     return inb && code != st.mask;
 }
 
-// probe 4 rows of a chunk; rows whose bit is clear in alive are skipped
-__device__ __forceinline__ void probe4(const StepDesc &st, const uint32_t *lds, v4i f,
-                                       uint32_t (&gid)[4], uint32_t &alive, bool first)
+__device__ __forceinline__ bool probe1_any(const StepDesc &st, const uint32_t *lds, int32_t key, uint32_t &code)
+{
+    return st.in_lds ? probe1<true>(st, lds, key, code) : probe1<false>(st, lds, key, code);
+}
+
+template <bool IN_LDS>
+__device__ __forceinline__ void probe4_in(const StepDesc &st, const uint32_t *lds, v4i f,
+                                          uint32_t (&gid)[4], uint32_t &alive, bool first)
 {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if ((alive >> k) & 1u) {
             uint32_t code;
-            bool ok = probe1(st, lds, f[k], code);
+            bool ok = probe1<IN_LDS>(st, lds, f[k], code);
             const uint32_t add = code * st.factor;
             gid[k] = first ? add : gid[k] + add;
             if (!ok) alive &= ~(1u << k);
         }
     }
+}
+
+// probe 4 rows of a chunk; rows whose bit is clear in alive are skipped. The
+// residency branch is wave-uniform and encloses the use of the loaded words,
+// so the two paths meet with their waits done.
+__device__ __forceinline__ void probe4(const StepDesc &st, const uint32_t *lds, v4i f,
+                                       uint32_t (&gid)[4], uint32_t &alive, bool first)
+{
+    if (st.in_lds)
+        probe4_in<true>(st, lds, f, gid, alive, first);
+    else
+        probe4_in<false>(st, lds, f, gid, alive, first);
 }
 
 // copy LDS-resident packed vectors; zero the LDS group table
@@ -567,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_vwm_sv(ScanArgs a)
                 for (int k = 0; k < 4; k++) {
                     if ((present[u] >> k) & 1u) {
                         uint32_t code;
-                        bool ok = probe1(a.st[s], lds, f[s][u][k], code);
+                        bool ok = probe1_any(a.st[s], lds, f[s][u][k], code);
                         gid[u][k] += code * a.st[s].factor;
                         if (!ok) live &= ~(1u << k);
                     }
@@ -645,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
                     egid[j] = lgid[e];
                     uint32_t code;
                     const int32_t key = a.st[s].fk[base + eoff[j]];
-                    if (probe1(a.st[s], lds, key, code)) {
+                    if (probe1_any(a.st[s], lds, key, code)) {
                         egid[j] += code * a.st[s].factor;
                         ok |= 1u << j;
                     }
@@ -834,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_cwm_dv_join(ScanArgs a)
                 eoid[j] = soid[e];
                 egid[j] = (uint32_t)sgid[e];
                 uint32_t code;
-                if (probe1(st, lds, st.fk[eoid[j]], code)) {
+                if (probe1_any(st, lds, st.fk[eoid[j]], code)) {
                     egid[j] += code * st.factor;
                     ok |= 1u << j;
                 }
