@@ -122,14 +122,17 @@ bool query_valid(const olap_star *star, const olap_batch_query *q, std::string *
 
 // One worker's output staging: sums then counts of up to `slots` groups in
 // one allocation, grown on demand and kept across batches; device memory
-// for the GPU executor (which also keeps a pinned host mirror, so a batch's
-// results come back in two DMA copies), host memory for a host executor.
+// for the GPU executor, host memory for a host executor. The GPU executor
+// also keeps two pinned host mirrors: a batch's results come back into one
+// in two DMA copies while the worker's slicer still copies the previous
+// batch's requests out of the other.
 struct Staging {
     olap_server s = nullptr;
+    hipStream_t stream = nullptr;   // GPU executor: the stream whose copies read buf
     int64_t *buf = nullptr;     // [2 * slots]: sums, then counts
     size_t slots = 0;
-    int64_t *host = nullptr;    // pinned mirror (GPU executor)
-    size_t host_slots = 0;
+    int64_t *host[2] = {nullptr, nullptr};  // pinned mirrors (GPU executor)
+    size_t host_slots[2] = {0, 0};
 
     int64_t *sum() const { return buf; }
     int64_t *count() const { return buf + slots; }
@@ -147,24 +150,32 @@ struct Staging {
         else if (!gpu_mode(s)) std::free(p);
         else (void)hipFree(p);
     }
+    // the device (or host-executor) buffer for `want` slots; a GPU buffer
+    // is replaced only after the copies still reading it have finished
     bool fit(size_t want)
     {
         if (want > slots) {
+            if (gpu_mode(s) && buf && stream && hipStreamSynchronize(stream) != hipSuccess) return false;
             dealloc(buf);
             slots = 0;
             buf = (int64_t *)alloc(16 * want);
             if (!buf) return false;
             slots = want;
         }
-        if (gpu_mode(s) && want > host_slots) {
-            if (host) (void)hipHostFree(host);
-            host_slots = 0;
-            if (hipHostMalloc((void **)&host, 16 * want, hipHostMallocDefault) != hipSuccess) {
-                host = nullptr;
-                return false;
-            }
-            host_slots = want;
+        return true;
+    }
+    // pinned mirror k for `want` slots (the caller owns mirror k: no copy and
+    // no slicer uses it)
+    bool fit_host(int k, size_t want)
+    {
+        if (want <= host_slots[k]) return true;
+        if (host[k]) (void)hipHostFree(host[k]);
+        host_slots[k] = 0;
+        if (hipHostMalloc((void **)&host[k], 16 * want, hipHostMallocDefault) != hipSuccess) {
+            host[k] = nullptr;
+            return false;
         }
+        host_slots[k] = want;
         return true;
     }
     void release()
@@ -172,10 +183,79 @@ struct Staging {
         dealloc(buf);
         buf = nullptr;
         slots = 0;
-        if (host) (void)hipHostFree(host);
-        host = nullptr;
-        host_slots = 0;
+        for (int k = 0; k < 2; k++) {
+            if (host[k]) (void)hipHostFree(host[k]);
+            host[k] = nullptr;
+            host_slots[k] = 0;
+        }
     }
+};
+
+void finish(olap_server s, const std::vector<Request *> &batch, int status);
+
+// A GPU worker's slicer thread. The worker hands it a batch whose results
+// are being copied into pinned mirror k (an event marks the end of the
+// copies) and goes on to the next batch; the slicer waits for the event,
+// copies each request's slice to its host buffers and finishes the batch.
+// Mirror k is the worker's again once the slicer is done with it.
+struct SliceJob {
+    std::vector<Request *> batch;
+    std::vector<int64_t> off;
+    int mirror = 0;
+    int worker = 0, dataset = 0, cfg = -1;
+    double setup_ms = 0.0, scan_ms = 0.0;
+    std::chrono::steady_clock::time_point t_launched;
+};
+
+class Slicer {
+  public:
+    Slicer(olap_server s, Staging *out, int device) : s_(s), out_(out), device_(device)
+    {
+        for (int k = 0; k < 2; k++)
+            if (hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming) != hipSuccess) ev_[k] = nullptr;
+        th_ = std::thread([this] { run(); });
+    }
+    ~Slicer()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+        for (int k = 0; k < 2; k++)
+            if (ev_[k]) (void)hipEventDestroy(ev_[k]);
+    }
+    bool ok() const { return ev_[0] && ev_[1]; }
+    hipEvent_t event(int k) const { return ev_[k]; }
+    // block until mirror k is free (no job of the slicer reads it)
+    void wait_free(int k)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !busy_[k]; });
+    }
+    void submit(SliceJob &&job)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            busy_[job.mirror] = true;
+            jobs_.push_back(std::move(job));
+        }
+        cv_.notify_all();
+    }
+
+  private:
+    void run();
+    olap_server s_;
+    Staging *out_;
+    int device_;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<SliceJob> jobs_;
+    bool busy_[2] = {false, false};
+    bool stop_ = false;
+    std::thread th_;
 };
 
 struct Worker {
@@ -185,6 +265,8 @@ struct Worker {
     olap::BatchWorkspace *ws = nullptr;   // kept across batches: no steady-state allocation
     Staging out;
     int setup = OLAP_OK;
+    Slicer *slicer = nullptr;             // GPU executor: copies results out, finishes requests
+    int next_mirror = 0;
 };
 
 void set_err(olap_server s, const std::string &msg)
@@ -193,14 +275,21 @@ void set_err(olap_server s, const std::string &msg)
     s->err = msg;
 }
 
-// run a batch on the GPU executor: the fused scan into the staging, then
-// each request's slice to its host buffers
+// a GPU batch handed to the worker's slicer (it finishes the requests)
+constexpr int kHandedOff = 1 << 30;
+
+// run a batch on the GPU executor: the fused scan into the staging and the
+// copies of its results into a free pinned mirror, then the batch goes to
+// the slicer; returns kHandedOff, or an error status (the batch is then
+// finished by the caller)
 int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
             const std::vector<olap_batch_query> &qs, const std::vector<int64_t> &off)
 {
     const olap_star &star = s->datasets[(size_t)batch[0]->dataset];
     const Clock::time_point t0 = Clock::now();
     olap_batch b = nullptr;
+    // the previous batch's kernels, which read the workspace this setup may
+    // grow, have finished: the worker synchronised its stream before admission
     int st = olap::batch_create_in(w.ctx, &star, qs.data(), (int32_t)qs.size(), w.ws, w.stream, &b);
     int cfg = -1;
     if (st == OLAP_OK) {
@@ -214,37 +303,83 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
     const Clock::time_point t1 = Clock::now();
     if (st == OLAP_OK) st = olap_batch_run(b, w.out.sum(), w.out.count(), w.stream);
     const Clock::time_point t2 = Clock::now();
-    // two DMA copies into the pinned mirror, then each request's slices
+    // two DMA copies into a free mirror, an event after them, then the slicer
+    const int k = w.next_mirror;
+    if (st == OLAP_OK) {
+        w.slicer->wait_free(k);
+        if (!w.out.fit_host(k, std::max<size_t>(total, 1))) st = OLAP_E_NOMEM;
+    }
     if (st == OLAP_OK &&
-        (hipMemcpyAsync(w.out.host, w.out.sum(), 8 * total, hipMemcpyDeviceToHost, w.stream) != hipSuccess ||
-         hipMemcpyAsync(w.out.host + total, w.out.count(), 8 * total, hipMemcpyDeviceToHost, w.stream) !=
+        (hipMemcpyAsync(w.out.host[k], w.out.sum(), 8 * total, hipMemcpyDeviceToHost, w.stream) != hipSuccess ||
+         hipMemcpyAsync(w.out.host[k] + total, w.out.count(), 8 * total, hipMemcpyDeviceToHost, w.stream) !=
              hipSuccess ||
-         hipStreamSynchronize(w.stream) != hipSuccess))
+         hipEventRecord(w.slicer->event(k), w.stream) != hipSuccess))
         st = OLAP_E_HIP;
-    const Clock::time_point t3 = Clock::now();
-    for (size_t i = 0; i < batch.size() && st == OLAP_OK; i++) {
-        const size_t n = (size_t)(off[i + 1] - off[i]);
-        std::memcpy(batch[i]->sum_host, w.out.host + off[i], 8 * n);
-        std::memcpy(batch[i]->count_host, w.out.host + total + off[i], 8 * n);
-    }
-    if (s->trace) {
-        auto ms = [](Clock::time_point a, Clock::time_point b) {
-            return std::chrono::duration<double, std::milli>(b - a).count();
-        };
-        std::fprintf(stderr,
-                     "olap_server: worker %d dataset %d queries %zu config %d slots %zu: setup %.2f ms, "
-                     "scan %.2f ms, results %.2f ms, slices %.2f ms, status %d\n",
-                     w.id, batch[0]->dataset, batch.size(), cfg, total, ms(t0, t1), ms(t1, t2), ms(t2, t3),
-                     ms(t3, Clock::now()), st);
-    }
+    if (b) olap_batch_destroy(b);
     if (st != OLAP_OK) {
         const char *m = olap_last_error(w.ctx);
         set_err(s, std::string("server batch: ") + (st == OLAP_E_NOMEM && (!m || !*m)
                                                         ? "output staging allocation failed"
                                                         : (m ? m : "")));
+        return st;
     }
-    if (b) olap_batch_destroy(b);
-    return st;
+    auto ms = [](Clock::time_point a, Clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    SliceJob job;
+    job.batch = batch;
+    job.off = off;
+    job.mirror = k;
+    job.worker = w.id;
+    job.dataset = batch[0]->dataset;
+    job.cfg = cfg;
+    job.setup_ms = ms(t0, t1);
+    job.scan_ms = ms(t1, t2);
+    job.t_launched = Clock::now();
+    w.slicer->submit(std::move(job));
+    w.next_mirror ^= 1;
+    return kHandedOff;
+}
+
+void Slicer::run()
+{
+    (void)hipSetDevice(device_);
+    for (;;) {
+        SliceJob job;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+            if (jobs_.empty()) return;  // stopping, nothing left
+            job = std::move(jobs_.front());
+            jobs_.pop_front();
+        }
+        int st = hipEventSynchronize(ev_[job.mirror]) == hipSuccess ? OLAP_OK : OLAP_E_HIP;
+        const Clock::time_point t3 = Clock::now();
+        const int64_t *h = out_->host[job.mirror];
+        const size_t total = (size_t)job.off.back();
+        for (size_t i = 0; i < job.batch.size() && st == OLAP_OK; i++) {
+            const size_t n = (size_t)(job.off[i + 1] - job.off[i]);
+            std::memcpy(job.batch[i]->sum_host, h + job.off[i], 8 * n);
+            std::memcpy(job.batch[i]->count_host, h + total + job.off[i], 8 * n);
+        }
+        if (st != OLAP_OK) set_err(s_, "server batch: the result copies failed");
+        if (s_->trace) {
+            auto ms = [](Clock::time_point a, Clock::time_point b) {
+                return std::chrono::duration<double, std::milli>(b - a).count();
+            };
+            std::fprintf(stderr,
+                         "olap_server: worker %d dataset %d queries %zu config %d slots %zu: setup %.2f ms, "
+                         "scan %.2f ms, results %.2f ms, slices %.2f ms, status %d\n",
+                         job.worker, job.dataset, job.batch.size(), job.cfg, total, job.setup_ms, job.scan_ms,
+                         ms(job.t_launched, t3), ms(t3, Clock::now()), st);
+        }
+        finish(s_, job.batch, st);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            busy_[job.mirror] = false;
+        }
+        cv_.notify_all();
+    }
 }
 
 // run a batch on a host executor
@@ -337,10 +472,21 @@ void worker_main(olap_server s, int id)
             hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking) != hipSuccess) {
             w.setup = OLAP_E_HIP;
             set_err(s, "server worker: device context or stream creation failed");
+        } else {
+            w.out.stream = w.stream;
+            w.slicer = new Slicer(s, &w.out, s->ctx->device);
+            if (!w.slicer->ok()) {
+                w.setup = OLAP_E_HIP;
+                set_err(s, "server worker: event creation failed");
+            }
         }
     }
     for (;;) {
         std::vector<Request *> batch;
+        // a GPU worker takes new requests only once its previous batch has
+        // left the device (its slicer may still copy the results out), so
+        // requests wait in the shared queue while both workers are busy
+        if (w.stream) (void)hipStreamSynchronize(w.stream);
         {
             std::unique_lock<std::mutex> lk(s->mu);
             while (!s->stop) {
@@ -358,8 +504,11 @@ void worker_main(olap_server s, int id)
         }
         int st = w.setup;
         if (st == OLAP_OK) st = gpu_mode(s) ? run_gpu(s, w, batch, qs, off) : run_host(s, w, batch, qs, off);
-        finish(s, batch, st);
+        if (st != kHandedOff) finish(s, batch, st);
     }
+    delete w.slicer;  // finishes the batches handed to it first
+    w.slicer = nullptr;
+    if (w.stream) (void)hipStreamSynchronize(w.stream);
     w.out.release();
     if (w.ws) olap::batch_workspace_destroy(w.ws);
     if (w.stream) (void)hipStreamDestroy(w.stream);
