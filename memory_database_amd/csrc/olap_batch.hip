@@ -819,8 +819,8 @@ struct BatchWorkspace {
     uint4 *rec[4] = {nullptr, nullptr, nullptr, nullptr};                 // [dim rows] per table
     size_t dim_cap[4] = {0, 0, 0, 0};
     BQDesc *desc[kNumBatchCfgs] = {};   // OLAP_BATCH_MAX descriptors per configuration
-    MaskPred *preds = nullptr;          // OLAP_BATCH_MAX predicates of one table
-    uint32_t *pass = nullptr;
+    MaskPred *preds = nullptr;          // up to 4 x OLAP_BATCH_MAX predicates (one per query and table)
+    uint32_t *pass = nullptr;           // their pass counts
 
     hipError_t fit_table(int t, size_t rows)
     {
@@ -841,8 +841,8 @@ struct BatchWorkspace {
         hipError_t e = hipSuccess;
         for (int c = 0; c < kNumBatchCfgs && e == hipSuccess; c++)
             if (!desc[c]) e = hipMalloc(&desc[c], sizeof(BQDesc) * OLAP_BATCH_MAX);
-        if (e == hipSuccess && !preds) e = hipMalloc(&preds, sizeof(MaskPred) * OLAP_BATCH_MAX);
-        if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !preds) e = hipMalloc(&preds, sizeof(MaskPred) * 4 * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * 4 * OLAP_BATCH_MAX);
         return e;
     }
     ~BatchWorkspace()
@@ -1175,35 +1175,44 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
             if (!uses) b->nouse[t] |= 1ull << q;
         }
     }
-    // masks and their pass statistics, on the setup stream
+    // masks and their pass statistics, on the setup stream: the predicates
+    // of every probed table in one upload, one mask kernel per table, one
+    // read-back of the pass counts and one synchronisation
     b->sel.assign((size_t)nq, 1.0);
     hipError_t e = ws->fit_fixed();
+    std::vector<MaskPred> preds;
+    std::vector<int> pq;
+    uint32_t pbeg[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < 4; t++) {
+        pbeg[t] = (uint32_t)preds.size();
+        if ((b->used >> t) & 1u)
+            for (int q = 0; q < nq; q++)
+                for (int s = 0; s < queries[q].ndims; s++)
+                    if (queries[q].table[s] == t) {
+                        preds.push_back({q, queries[q].lo[s], queries[q].hi[s], 0});
+                        pq.push_back(q);
+                    }
+        if (e == hipSuccess && ((b->used >> t) & 1u)) e = ws->fit_table(t, (size_t)star->dim_rows[t]);
+    }
+    pbeg[4] = (uint32_t)preds.size();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ws->preds, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice, strm);
+    if (e == hipSuccess) e = launch_zero_u32(ws->pass, preds.size(), strm);
     for (int t = 0; t < 4 && e == hipSuccess; t++) {
-        if (!((b->used >> t) & 1u)) continue;
-        std::vector<MaskPred> preds;
-        std::vector<int> pq;
-        for (int q = 0; q < nq; q++)
-            for (int s = 0; s < queries[q].ndims; s++)
-                if (queries[q].table[s] == t) {
-                    preds.push_back({q, queries[q].lo[s], queries[q].hi[s], 0});
-                    pq.push_back(q);
-                }
         const uint32_t D = (uint32_t)star->dim_rows[t];
-        e = ws->fit_table(t, D);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(ws->preds, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice,
-                               strm);
-        if (e == hipSuccess) e = launch_zero_u32(ws->pass, preds.size(), strm);
-        if (e == hipSuccess && D) {
-            const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
-            k_batch_masks<<<grid, 256, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds, (int)preds.size(),
-                                                  b->nouse[t], ws->mask[t], ws->rec[t], ws->pass);
-            e = hipGetLastError();
-        }
-        std::vector<uint32_t> pass(preds.size(), 0);
-        if (e == hipSuccess) e = hipMemcpyAsync(pass.data(), ws->pass, 4 * preds.size(), hipMemcpyDeviceToHost, strm);
-        if (e == hipSuccess) e = hipStreamSynchronize(strm);
-        for (size_t i = 0; i < preds.size() && e == hipSuccess; i++)
+        if (!((b->used >> t) & 1u) || !D) continue;
+        const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
+        k_batch_masks<<<grid, 256, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds + pbeg[t],
+                                              (int)(pbeg[t + 1] - pbeg[t]), b->nouse[t], ws->mask[t], ws->rec[t],
+                                              ws->pass + pbeg[t]);
+        e = hipGetLastError();
+    }
+    std::vector<uint32_t> pass(preds.size(), 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(pass.data(), ws->pass, 4 * preds.size(), hipMemcpyDeviceToHost, strm);
+    if (e == hipSuccess) e = hipStreamSynchronize(strm);
+    for (int t = 0; t < 4 && e == hipSuccess; t++) {
+        const uint32_t D = (uint32_t)star->dim_rows[t];
+        for (uint32_t i = pbeg[t]; i < pbeg[t + 1]; i++)
             b->sel[(size_t)pq[i]] *= D ? (double)pass[i] / (double)D : 0.0;
     }
     if (e != hipSuccess) {
