@@ -715,18 +715,27 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
         return hip_fail(ctx, e, "query_create: packed vectors");
     }
     q->packed_words = total_words;
-    // Probe order: the most selective dimension first. A row passes only if
-    // every step passes and its gid is the sum of code * factor over the
-    // steps, so the order changes no result; it decides how many later
-    // foreign-key lines the early exit still has to fetch.
+    // Probe order. A row passes only if every step passes and its gid is the
+    // sum of code * factor over the steps, so the order changes no result; it
+    // decides how many later foreign-key lines the early exit still fetches
+    // and how many rows reach each probe. Independent filters are best taken
+    // by falling (1 - pass fraction) / cost: a step whose packed vector is
+    // larger than LDS stays in HBM and costs an L2 gather per row that
+    // reaches it (cost 4 against 1 for an LDS probe), so a selective step in
+    // LDS can go before a slightly more selective one in HBM. With every
+    // vector in LDS this is the most selective step first.
     int perm[OLAP_MAX_DIMS];
-    double frac[OLAP_MAX_DIMS];
+    double frac[OLAP_MAX_DIMS], rank[OLAP_MAX_DIMS];
     for (int s = 0; s < nd; s++) {
         const int t = plan->orders[s];
         perm[s] = s;
         frac[s] = cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
+        const bool hbm = (uint64_t)nwords[s] * 4u > (uint64_t)kLdsBytes;
+        rank[s] = (1.0 - frac[s]) / (hbm ? 4.0 : 1.0);
     }
-    std::stable_sort(perm, perm + nd, [&](int x, int y) { return frac[x] < frac[y]; });
+    std::stable_sort(perm, perm + nd, [&](int x, int y) {
+        return rank[x] != rank[y] ? rank[x] > rank[y] : frac[x] < frac[y];
+    });
     // step 1 is fetched with step 0 when most of its 128-byte lines (32 keys)
     // hold a row that passed step 0: a streamed column then costs less than
     // the dependent partial fetches
