@@ -266,8 +266,11 @@ olap_status olap_batch_destroy(olap_batch b);
  * admits further pending requests of the same dataset in arrival order while
  * their device bytes fit the free budget (at most max_batch), and runs them
  * as one fused batched scan; requests arriving meanwhile are admitted by the
- * next idle worker, so batches overlap on the device. Results are exact:
- * every request gets the same sums and counts alone or in any mix. */
+ * next idle worker, so batches overlap on the device. A worker's results
+ * come back into one of two pinned host buffers and a per-worker thread
+ * copies each request's slice to its host buffers while the worker runs the
+ * next batch. Results are exact: every request gets the same sums and counts
+ * alone or in any mix. */
 typedef struct {
     int32_t max_batch;       /* queries per fused scan, 1..64 */
     int32_t streams;         /* concurrent batches (worker threads), 1..8 */
@@ -325,8 +328,9 @@ olap_status olap_server_reset_stats(olap_server s);
 /* message of the last failure on the server; valid until the next failing
  * call on it */
 const char *olap_server_last_error(olap_server s);
-/* Stops the workers after their running batches; requests still pending are
- * dropped, so call it once every wait has returned. */
+/* Stops the workers after their running batches (whose requests get their
+ * results); requests not yet admitted are dropped, so call it once every
+ * wait has returned. */
 olap_status olap_server_destroy(olap_server s);
 /* The admission accounting and policy, as pure host functions: the device
  * bytes a request holds (an int32 dimension vector per probed table plus its

@@ -311,7 +311,9 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
 //     takes 64 events of one query: its descriptor is uniform, and a query
 //     of one aggregation partition takes one counter/cursor atomic per wave.
 // A tile with more matches than the list holds runs steps 4-5 in rounds, each
-// over a run of whole query segments.
+// over a run of whole query segments. In the one-pass staged mode every event
+// of step 5 writes its record, and a sixth step after each round retires the
+// full record chunks.
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long m)
 {
@@ -351,19 +353,7 @@ __device__ __forceinline__ unsigned long long rotr64(unsigned long long w, uint3
     return (w >> r) | (w << ((64u - r) & 63u));
 }
 
-typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v2i_a4 __attribute__((ext_vector_type(2), aligned(4)));  // a pair at any 4-byte offset
-
-__device__ __forceinline__ int2 bload2(const int32_t *__restrict__ p, uint64_t r, uint64_t rows)
-{
-    if (r + 2 <= rows) {
-        const v2i v = __builtin_nontemporal_load((const v2i *)(p + r));
-        return make_int2(v.x, v.y);
-    }
-    int2 v = make_int2(0, 0);
-    if (r < rows) v.x = p[r];
-    return v;
-}
 
 template <int BS, int MODE>
 __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
