@@ -38,7 +38,8 @@ struct BQDesc {
     uint32_t g[4];      // group_mod (0: code 0)
     uint64_t M[4];      // fast-modulo multiplier: UINT64_MAX / g + 1
     uint32_t pbase;     // first staged-aggregation partition of the query
-    uint32_t pad[3];
+    uint32_t pbase_c;   // the same with the compact one-pass partitions (2^12 slots)
+    uint32_t pad[2];
 };
 static_assert(sizeof(BQDesc) == 112, "descriptor layout");
 constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;  // word 16: M[], word 24: pbase
@@ -133,6 +134,53 @@ __global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32
             rec[r] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), attr ? (uint32_t)attr[r] : 0u, 0u);
         }
     }
+}
+
+// min and max of a measure column: a grid-stride pass with 16-byte loads
+// (the column is 16-byte aligned), a wave reduction, one atomic pair a wave
+__global__ void k_minmax_init(int32_t *out)
+{
+    out[0] = INT32_MAX;
+    out[1] = INT32_MIN;
+}
+
+__global__ void k_minmax(const int32_t *m, uint64_t n, int32_t *out)
+{
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    const uint64_t n4 = n / 4u;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const v4i v = __builtin_nontemporal_load((const v4i *)m + i);
+        lo = min(min(lo, v.x), min(v.y, min(v.z, v.w)));
+        hi = max(max(hi, v.x), max(v.y, max(v.z, v.w)));
+    }
+    for (uint64_t i = 4u * n4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        lo = min(lo, m[i]);
+        hi = max(hi, m[i]);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d));
+        hi = max(hi, __shfl_xor(hi, d));
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicMin(&out[0], lo);
+        atomicMax(&out[1], hi);
+    }
+}
+
+// the range of n > 0 values of m into out (synchronises s)
+hipError_t measure_range(const int32_t *m, uint64_t n, int32_t *dev2, int32_t out[2], hipStream_t s)
+{
+    k_minmax_init<<<1, 1, 0, s>>>(dev2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n / 4u + 255u) / 256u, 4096u));
+    k_minmax<<<grid, 256, 0, s>>>(m, n, dev2);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev2, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
 }
 
 // ---------------------------------------------------------------- scan
@@ -333,18 +381,17 @@ __device__ __forceinline__ void lds_barrier()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-constexpr unsigned long long kNoChunk = ~0ull;
-
-// claim a chunk of the pool for partition p: its first record index, or
-// kNoChunk when the pool is used up
-__device__ __forceinline__ unsigned long long chunk_claim(const ChunkArgs &c, uint32_t p)
+// claim a chunk of the pool for partition p: its id (first record id <<
+// ch_bits), or kNoChunk when the pool is used up
+__device__ __forceinline__ uint32_t chunk_claim(const ChunkArgs &c, uint32_t p)
 {
     const unsigned long long ch = 1ull << c.ch_bits;
     const unsigned long long nc =
         __hip_atomic_fetch_add(c.cursor, ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (nc + ch > c.cap) return kNoChunk;
-    c.chunk_part[nc >> c.ch_bits] = p;
-    return nc;
+    const uint32_t id = (uint32_t)(nc >> c.ch_bits);
+    c.chunk_part[id] = p;
+    return id;
 }
 
 // w rotated right by r (0..63)
@@ -374,14 +421,15 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     // (descriptors are read from global memory with scalar loads: no LDS copy)
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
     if (threadIdx.x < 64u) lds[a.cnt_lds + threadIdx.x] = 0u;
-    // one-pass staging: per partition a current chunk and a spare (pool
-    // record indexes, kNoChunk when the pool ran out) and the records handed
-    // out in the current chunk (positions past its end go to the spare)
+    // one-pass staging: per partition a current chunk and a spare (chunk
+    // ids, kNoChunk when the pool ran out) and the records handed out in the
+    // current chunk (positions past its end go to the spare); a COMPACT pool
+    // has partitions of 2^12 slots
     const uint32_t np = a.chunk.nparts;
     const uint32_t CH = 1u << a.chunk.ch_bits;
-    stage_l_u64 *pc_cur = (stage_l_u64 *)((stage_l_u32 *)lds + a.pc_lds);
-    stage_l_u64 *pc_spare = pc_cur + np;
-    stage_l_u32 *pc_fill = (stage_l_u32 *)(pc_spare + np);
+    stage_l_u32 *pc_cur = (stage_l_u32 *)lds + a.pc_lds;
+    stage_l_u32 *pc_spare = pc_cur + np;
+    stage_l_u32 *pc_fill = pc_spare + np;
     if (MODE == AGG_CHUNK) {
         for (uint32_t p = threadIdx.x; p < np; p += BS) {
             pc_cur[p] = chunk_claim(a.chunk, p);
@@ -612,44 +660,63 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
                     }
                 } else {
-                    const uint32_t part = d.pbase + (gid >> kPartBits);
+                    const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
+                    const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
+                    const uint32_t pbase = compact ? d.pbase_c : d.pbase;
+                    const uint32_t part = pbase + (gid >> pbits);
+                    const uint32_t glow = gid & ((1u << pbits) - 1u);
+                    const long long mv =
+                        (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll);
+                    const long long moff = mv - (long long)a.chunk.mbase;
+                    // a measure outside the range 4-byte records were packed
+                    // for (the columns changed since the range was taken) is
+                    // added directly and takes no record slot
+                    bool rec = ok;
+                    if (compact && ok && (moff < 0 || moff >= (1ll << kCompactMeasureBits))) {
+                        const unsigned long long o = a.chunk.part_out[part] + glow;
+                        b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
+                        b_atomic_dev(&a.gcnt[o], 1ull);
+                        __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        rec = false;
+                    }
                     // counters: the COUNT pass's per-partition counts, the
                     // SCATTER pass's cursors, or the one-pass chunk fills
                     stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_fill : scur;
                     uint32_t pos = 0u;
-                    if (G <= kPartSlots) {
+                    if (G <= (1u << pbits)) {
                         // one partition: one counter / cursor atomic for the wave
-                        const unsigned long long act = __ballot(ok);
+                        const unsigned long long act = __ballot(rec);
                         if (act) {
                             const int leader = __ffsll(act) - 1;
                             uint32_t at = 0u;
                             if ((int)lane == leader)
-                                at = __hip_atomic_fetch_add(ctr + d.pbase, (uint32_t)__popcll(act),
+                                at = __hip_atomic_fetch_add(ctr + pbase, (uint32_t)__popcll(act),
                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             pos = (uint32_t)__shfl((int)at, leader) + lane_rank(act);
                         }
-                    } else if (ok) {
+                    } else if (rec) {
                         pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    if (MODE == AGG_CHUNK && ok) {
+                    if (MODE == AGG_CHUNK && rec) {
                         // a round adds at most R = CH records to a partition
                         // whose fill is below CH: pos < 2 CH
-                        const unsigned long long base = pos < CH ? pc_cur[part] : pc_spare[part];
-                        const unsigned long long idx = base + (pos < CH ? pos : pos - CH);
-                        const uint32_t glow = gid & (kPartSlots - 1u);
-                        if (base == kNoChunk) {
+                        const uint32_t id = pos < CH ? pc_cur[part] : pc_spare[part];
+                        const uint32_t off = pos < CH ? pos : pos - CH;
+                        const unsigned long long base = (unsigned long long)id << a.chunk.ch_bits;
+                        if (id == kNoChunk) {
+                            // no chunk (the pool is used up): added directly
                             const unsigned long long o = a.chunk.part_out[part] + glow;
-                            b_atomic_dev(&a.gsum[o], (unsigned long long)((long long)(int32_t)rm[row] +
-                                                                          (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll)));
+                            b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
                             b_atomic_dev(&a.gcnt[o], 1ull);
                             __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else if (compact) {
+                            ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | (uint32_t)moff;
                         } else if (a.m2) {
-                            ((uint4 *)a.chunk.pool)[idx] = make_uint4(glow, rm[row], rm[R + row], 0u);
+                            ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, rm[row], rm[R + row], 0u);
                         } else {
                             // one measure: a chunk holds CH u32 measures,
                             // then CH u16 slots (6 bytes a record)
                             char *cp = (char *)a.chunk.pool + 6ull * base;
-                            const uint32_t off = (uint32_t)(idx - base);
                             ((uint32_t *)cp)[off] = rm[row];
                             ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
                         }
@@ -674,8 +741,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 for (uint32_t p = tid; p < np; p += BS) {
                     const uint32_t f = pc_fill[p];
                     if (f >= CH) {
-                        const unsigned long long cur = pc_cur[p];
-                        if (cur != kNoChunk) a.chunk.chunk_fill[cur >> a.chunk.ch_bits] = CH;
+                        const uint32_t cur = pc_cur[p];
+                        if (cur != kNoChunk) a.chunk.chunk_fill[cur] = CH;
                         pc_cur[p] = pc_spare[p];
                         pc_fill[p] = f - CH;
                         pc_spare[p] = chunk_claim(a.chunk, p);
@@ -690,8 +757,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     if (MODE == AGG_CHUNK) {
         // fills of the chunks still open (fill < CH after the last round)
         for (uint32_t p = tid; p < np; p += BS) {
-            if (pc_cur[p] != kNoChunk) a.chunk.chunk_fill[pc_cur[p] >> a.chunk.ch_bits] = pc_fill[p];
-            if (pc_spare[p] != kNoChunk) a.chunk.chunk_fill[pc_spare[p] >> a.chunk.ch_bits] = 0u;
+            if (pc_cur[p] != kNoChunk) a.chunk.chunk_fill[pc_cur[p]] = pc_fill[p];
+            if (pc_spare[p] != kNoChunk) a.chunk.chunk_fill[pc_spare[p]] = 0u;
         }
     }
     __syncthreads();
@@ -811,6 +878,10 @@ struct BatchWorkspace {
     BQDesc *desc[kNumBatchCfgs] = {};   // OLAP_BATCH_MAX descriptors per configuration
     MaskPred *preds = nullptr;          // up to 4 x OLAP_BATCH_MAX predicates (one per query and table)
     uint32_t *pass = nullptr;           // their pass counts
+    int32_t *minmax = nullptr;          // [2] device scratch of measure_range
+    const int32_t *range_col = nullptr; // the measure column whose range is kept
+    int64_t range_rows = -1;
+    int32_t range[2] = {0, 0};          // its min and max
 
     hipError_t fit_table(int t, size_t rows)
     {
@@ -833,6 +904,7 @@ struct BatchWorkspace {
             if (!desc[c]) e = hipMalloc(&desc[c], sizeof(BQDesc) * OLAP_BATCH_MAX);
         if (e == hipSuccess && !preds) e = hipMalloc(&preds, sizeof(MaskPred) * 4 * OLAP_BATCH_MAX);
         if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * 4 * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !minmax) e = hipMalloc(&minmax, 8);
         return e;
     }
     ~BatchWorkspace()
@@ -845,6 +917,7 @@ struct BatchWorkspace {
             if (desc[c]) (void)hipFree(desc[c]);
         if (preds) (void)hipFree(preds);
         if (pass) (void)hipFree(pass);
+        if (minmax) (void)hipFree(minmax);
     }
 };
 
@@ -880,6 +953,9 @@ struct olap_batch_s {
     std::vector<unsigned long long> out_off;
     std::vector<uint32_t> gsz;
     uint32_t nparts = 0;
+    uint32_t nparts_c = 0;   // partitions of 2^12 slots (compact one-pass records)
+    bool compact = false;    // one measure whose range spans fewer than 2^20 values
+    int32_t mbase = 0;       // its minimum
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -917,7 +993,7 @@ olap_status plan_config(olap_batch b, int c)
     const uint32_t table_budget = (uint32_t)cfg.lds_kib * 1024u;
     // staged configurations keep a u32 counter and a u64 range start per partition
     const uint32_t stage_bytes = cfg.staged == 1 ? up16(b->nparts * 4u) + b->nparts * 8u
-                                 : cfg.staged == 2 ? up16(b->nparts * 20u) : 0u;
+                                 : cfg.staged == 2 ? up16((b->compact ? b->nparts_c : b->nparts) * 12u) : 0u;
     // the event-grouped kernel stages per-tile rows and at least one event
     // per row of the tile
     const uint32_t R = (uint32_t)batch_tile_rows(cfg);
@@ -987,7 +1063,7 @@ olap_status plan_config(olap_batch b, int c)
         a.ev_lds = cur / 4u;
         cur += 4u * a.ev_cap;
     }
-    uint32_t pbase = 0;
+    uint32_t pbase = 0, pbase_c = 0;
     for (int q = 0; q < b->nq; q++) {
         const olap_batch_query &Q = b->queries[q];
         BQDesc &d = desc[q];
@@ -996,6 +1072,8 @@ olap_status plan_config(olap_batch b, int c)
         d.out_off = (uint32_t)b->offsets[q];
         d.pbase = pbase;
         pbase += (d.G + kPartSlots - 1) / kPartSlots;
+        d.pbase_c = pbase_c;
+        pbase_c += (d.G + (1u << kCompactPartBits) - 1) >> kCompactPartBits;
         for (int s = 0; s < 4; s++) {
             const bool on = s < Q.ndims;
             d.tab[s] = on ? (uint32_t)Q.table[s] : 0u;
@@ -1035,8 +1113,9 @@ olap_status plan_config(olap_batch b, int c)
 olap_status stager_for(olap_batch b, int c, hipStream_t s)
 {
     if (b->stager_cfg == c) return OLAP_OK;
+    const uint32_t pbits = kBatchCfgs[c].staged == 2 && b->compact ? kCompactPartBits : kPartBits;
     hipError_t e = b->ws->stager.setup(b->out_off.data(), b->gsz.data(), b->nq, (uint32_t)b->grid[c],
-                                       b->star.m2 != nullptr, s);
+                                       b->star.m2 != nullptr, s, pbits);
     if (e != hipSuccess) return bhip(b->ctx, e, "stager setup");
     b->stager_cfg = c;
     return OLAP_OK;
@@ -1070,7 +1149,8 @@ olap_status batch_run(olap_batch b, int64_t *sum, int64_t *count, hipStream_t s,
         const uint32_t ch_bits = (uint32_t)__builtin_ctz((uint32_t)batch_tile_rows(kBatchCfgs[c]));
         if (!sized) {
             const double expect = std::max(0.0, b->events) * (double)b->star.rows;
-            e = S.chunk_reserve((unsigned long long)expect, (uint32_t)b->grid[c], ch_bits);
+            const uint32_t fmt = b->star.m2 ? REC_WIDE : b->compact ? REC_COMPACT : REC_NARROW;
+            e = S.chunk_reserve((unsigned long long)expect, (uint32_t)b->grid[c], ch_bits, fmt, b->mbase);
             if (e != hipSuccess) return bhip(ctx, e, "batch chunk pool");
         }
         e = S.chunk_clear(s);
@@ -1209,13 +1289,31 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
         delete b;
         return bhip(ctx, e, "batch masks");
     }
+    // one measure whose values span fewer than 2^20: the one-pass records
+    // are 4 bytes (the range is kept per workspace and column; a measure
+    // outside it later, the columns having changed, is added directly)
+    if (!star->m2 && star->rows > 0) {
+        if (ws->range_col != star->m1 || ws->range_rows != star->rows) {
+            e = measure_range(star->m1, (uint64_t)star->rows, ws->minmax, ws->range, strm);
+            if (e != hipSuccess) {
+                delete b;
+                return bhip(ctx, e, "batch measure range");
+            }
+            ws->range_col = star->m1;
+            ws->range_rows = star->rows;
+        }
+        b->compact = (int64_t)ws->range[1] - (int64_t)ws->range[0] < (1ll << kCompactMeasureBits);
+        b->mbase = ws->range[0];
+    }
     b->events = 0;
     for (double v : b->sel) b->events += v;
     for (int q = 0; q < nq; q++) {
         b->out_off.push_back((unsigned long long)b->offsets[q]);
         b->gsz.push_back((uint32_t)queries[q].group_nums);
         b->nparts += (uint32_t)((queries[q].group_nums + kPartSlots - 1) / kPartSlots);
+        b->nparts_c += (uint32_t)((queries[q].group_nums + (1 << kCompactPartBits) - 1) >> kCompactPartBits);
     }
+    if (b->nparts_c > kMaxParts) b->compact = false;
     if (b->nparts > kMaxParts) {
         delete b;
         return bfail(ctx, OLAP_E_CAPACITY, "batch_create: too many aggregation partitions");

@@ -30,6 +30,16 @@ constexpr uint32_t kMaxParts = 1u << 16;
 
 enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK = 3 };
 
+// One-pass record formats: a chunk of CH records holds CH u32 measures then
+// CH u16 slots (NARROW, one measure, 6 bytes a record), CH uint4 {slot, m1,
+// m2, 0} (WIDE, two measures), or CH u32 slot << 20 | (measure - base)
+// (COMPACT: one measure whose range spans fewer than 2^20 values, with
+// partitions of 2^12 slots; 4 bytes a record)
+enum RecFmt : uint32_t { REC_NARROW = 0, REC_WIDE = 1, REC_COMPACT = 2 };
+constexpr uint32_t kCompactPartBits = 12;
+constexpr uint32_t kCompactMeasureBits = 20;
+constexpr uint32_t kNoChunk = 0xffffffffu;   // no chunk (the pool is used up)
+
 // One-pass staging (AGG_CHUNK): instead of a COUNT pass that sizes exact
 // ranges, a workgroup keeps per partition a current chunk of a shared pool
 // and a spare one, and hands out record slots with an LDS counter: positions
@@ -41,8 +51,7 @@ enum AggMode : int { AGG_DIRECT = 0, AGG_COUNT = 1, AGG_SCATTER = 2, AGG_CHUNK =
 // runs out (it is sized from the expected matches) never loses a match: the
 // scan adds such matches to the outputs with global atomics.
 struct ChunkArgs {
-    void *pool;                       // chunks of CH records: CH u32 measures then CH u16
-                                      // slots (one measure), or CH uint4 records (two)
+    void *pool;                       // chunks of CH records in format fmt
     unsigned long long cap;           // records the pool holds
     unsigned long long *cursor;       // records handed out (whole chunks)
     uint32_t *chunk_part;             // [cap / chunk] partition of each chunk
@@ -51,6 +60,8 @@ struct ChunkArgs {
     uint32_t *overflow;               // matches that went to global atomics
     uint32_t ch_bits;                 // log2 of the chunk size
     uint32_t nparts;
+    uint32_t fmt;                     // RecFmt
+    int32_t mbase;                    // COMPACT: the measure of offset 0
 };
 
 struct StageArgs {
@@ -141,6 +152,7 @@ __device__ __forceinline__ void stage_end(const StageArgs &s, uint32_t *lds)
 struct Stager {
     uint32_t nparts = 0;
     uint32_t nwg = 0;
+    uint32_t part_bits = kPartBits;
     bool two_measures = false;
     uint32_t *wg_counts = nullptr;              // [nparts][nwg]
     unsigned long long *wg_start = nullptr;     // [nparts][nwg]
@@ -157,8 +169,9 @@ struct Stager {
     // partitions of every query: out_off[q] (first output slot) and G[q];
     // buffers only grow, so a Stager reused across scans stops allocating;
     // the uploads and the clear are ordered on stream s (synchronises s)
+    // (partitions of 2^part_bits slots)
     hipError_t setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg,
-                     bool two_measures, hipStream_t s);
+                     bool two_measures, hipStream_t s, uint32_t part_bits = kPartBits);
     // after the COUNT pass: the prefix sums (device only)
     hipError_t prefix(hipStream_t s);
     // the record total of the last COUNT pass (synchronises)
@@ -175,8 +188,10 @@ struct Stager {
     hipError_t overflowed(uint32_t *out, hipStream_t s);
 
     // one-pass form: size the pool for `records` expected matches plus the
-    // partial chunks `nwg` workgroups can leave, with chunks of 2^ch_bits
-    hipError_t chunk_reserve(unsigned long long records, uint32_t nwg, uint32_t ch_bits);
+    // chunks `nwg` workgroups can leave open, with chunks of 2^ch_bits
+    // records in format fmt (COMPACT: measures from mbase)
+    hipError_t chunk_reserve(unsigned long long records, uint32_t nwg, uint32_t ch_bits, uint32_t fmt,
+                             int32_t mbase);
     // clear the pool cursor and counters before a scan (kernels only)
     hipError_t chunk_clear(hipStream_t s);
     ChunkArgs chunk_args() const;
@@ -193,6 +208,8 @@ struct Stager {
     void *pool = nullptr;
     unsigned long long pool_cap = 0;      // records
     uint32_t pool_ch_bits = 0;
+    uint32_t pool_fmt = REC_NARROW;
+    int32_t pool_mbase = 0;
     unsigned long long *pool_cursor = nullptr;
     uint32_t *chunk_part = nullptr;
     uint32_t *chunk_fill = nullptr;

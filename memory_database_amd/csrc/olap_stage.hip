@@ -320,10 +320,10 @@ __global__ __launch_bounds__(256) void k_chunk_scatter(const unsigned long long 
 }
 
 // work items: up to cpi chunks of one partition, aggregated in an LDS table
-template <bool WIDE>
+template <uint32_t FMT>
 __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
     const uint32_t *part_chunks, const uint32_t *item_begin, uint32_t nparts, uint32_t cpi,
-    const unsigned long long *order, const void *pool, uint32_t ch_bits,
+    const unsigned long long *order, const void *pool, uint32_t ch_bits, int32_t mbase,
     const unsigned long long *part_out, const uint32_t *part_slots, uint32_t *work,
     unsigned long long *gsum, unsigned long long *gcnt)
 {
@@ -332,10 +332,10 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
     __shared__ unsigned long long lord[kReduceItemChunks];  // the item's (fill << 32 | chunk id)
     __shared__ uint32_t s_item, s_part;
     const uint4 *v4 = (const uint4 *)pool;
-    // units of a chunk: single 16-byte records (wide), or four records of a
-    // one-measure chunk (CH u32 measures, then CH u16 slots): 16 bytes of
-    // measures and 8 bytes of slots
-    const uint32_t ubits = WIDE ? ch_bits : ch_bits - 2u;
+    // 16-byte units of a chunk: one WIDE record; four COMPACT records; or
+    // four NARROW records, as 16 bytes of measures and 8 bytes of slots
+    constexpr uint32_t kPer = FMT == REC_WIDE ? 1u : 4u;
+    const uint32_t ubits = FMT == REC_WIDE ? ch_bits : ch_bits - 2u;
     const unsigned long long ch = 1ull << ch_bits;
     for (;;) {
         reduce_claim(work, item_begin, nparts, s_item, s_part);
@@ -368,11 +368,13 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
                     const unsigned long long o = lord[i >> ubits];
                     const uint32_t w = i & ((1u << ubits) - 1u);
                     const unsigned long long cbase = (unsigned long long)(uint32_t)o << ch_bits;  // first record
-                    r[u] = WIDE ? w : 4u * w;  // first record index of the unit within the chunk
+                    r[u] = kPer * w;  // first record index of the unit within the chunk
                     fill[u] = (uint32_t)(o >> 32);
                     if (r[u] < fill[u]) {
-                        if (WIDE) {
+                        if (FMT == REC_WIDE) {
                             v[u] = v4[cbase + w];
+                        } else if (FMT == REC_COMPACT) {
+                            v[u] = v4[(cbase >> 2) + w];
                         } else {
                             const char *cp = (const char *)pool + 6ull * cbase;
                             v[u] = ((const uint4 *)cp)[w];
@@ -383,16 +385,25 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
             }
 #pragma unroll
             for (int u = 0; u < kReduceLoads; u++) {
-                if (WIDE) {
+                if (FMT == REC_WIDE) {
                     reduce_add(lsum, lcnt, slots, r[u] < fill[u] ? v[u].x : kPartSlots,
                                (long long)(int32_t)v[u].y + (long long)(int32_t)v[u].z);
                 } else {
-                    const uint32_t m[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-                    const uint32_t g[4] = {gs[u].x & 0xffffu, gs[u].x >> 16, gs[u].y & 0xffffu, gs[u].y >> 16};
+                    const uint32_t x[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        reduce_add(lsum, lcnt, slots, r[u] + (uint32_t)k < fill[u] ? g[k] : kPartSlots,
-                                   (long long)(int32_t)m[k]);
+                    for (int k = 0; k < 4; k++) {
+                        uint32_t g;
+                        long long m;
+                        if (FMT == REC_COMPACT) {
+                            g = x[k] >> kCompactMeasureBits;
+                            m = (long long)mbase + (long long)(x[k] & ((1u << kCompactMeasureBits) - 1u));
+                        } else {
+                            g = k == 0 ? (gs[u].x & 0xffffu) : k == 1 ? (gs[u].x >> 16)
+                                : k == 2 ? (gs[u].y & 0xffffu) : (gs[u].y >> 16);
+                            m = (long long)(int32_t)x[k];
+                        }
+                        reduce_add(lsum, lcnt, slots, r[u] + (uint32_t)k < fill[u] ? g : kPartSlots, m);
+                    }
                 }
             }
         }
@@ -419,19 +430,21 @@ hipError_t grow_dev(T *&p, size_t &cap, size_t n)
 }  // namespace
 
 hipError_t Stager::setup(const unsigned long long *out_off, const uint32_t *G, int nq, uint32_t nwg_,
-                         bool two, hipStream_t s)
+                         bool two, hipStream_t s, uint32_t pbits)
 {
     qbase_.assign((size_t)nq, 0);
     std::vector<unsigned long long> pout;
     std::vector<uint32_t> pslots;
+    const uint32_t ps = 1u << pbits;
     for (int q = 0; q < nq; q++) {
         qbase_[(size_t)q] = (uint32_t)pout.size();
-        const uint32_t np = (G[q] + kPartSlots - 1) / kPartSlots;
+        const uint32_t np = (G[q] + ps - 1) / ps;
         for (uint32_t i = 0; i < np; i++) {
-            pout.push_back(out_off[q] + (unsigned long long)i * kPartSlots);
-            pslots.push_back(std::min<uint32_t>(kPartSlots, G[q] - i * kPartSlots));
+            pout.push_back(out_off[q] + (unsigned long long)i * ps);
+            pslots.push_back(std::min<uint32_t>(ps, G[q] - i * ps));
         }
     }
+    part_bits = pbits;
     if (pout.size() > kMaxParts) return hipErrorInvalidValue;
     nparts = (uint32_t)pout.size();
     nwg = nwg_;
@@ -513,7 +526,8 @@ hipError_t Stager::reduce(unsigned long long *gsum, unsigned long long *gcnt, in
     return hipGetLastError();
 }
 
-hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint32_t ch_bits)
+hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint32_t ch_bits, uint32_t fmt,
+                                 int32_t mbase)
 {
     const unsigned long long ch = 1ull << ch_bits;
     // expected matches + 25%, plus the two chunks every workgroup holds open
@@ -526,7 +540,9 @@ hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint
     want = (want + ch - 1) & ~(ch - 1);
     hipError_t e = hipSuccess;
     if (!pool_cursor) e = hipMalloc(&pool_cursor, sizeof(unsigned long long));
-    if (e == hipSuccess && (want > pool_cap || pool_ch_bits != ch_bits || rec_wide != two_measures || !pool)) {
+    static const uint32_t rec_bytes[3] = {6u, 16u, 4u};  // NARROW, WIDE, COMPACT
+    if (e == hipSuccess &&
+        (want > pool_cap || pool_ch_bits != ch_bits || rec_bytes[pool_fmt] < rec_bytes[fmt] || !pool)) {
         if (pool) (void)hipFree(pool);
         if (chunk_part) (void)hipFree(chunk_part);
         if (chunk_fill) (void)hipFree(chunk_fill);
@@ -536,15 +552,19 @@ hipError_t Stager::chunk_reserve(unsigned long long records, uint32_t nwg_, uint
         chunk_order = nullptr;
         pool_cap = 0;
         const size_t nch = (size_t)(want >> ch_bits);
-        e = hipMalloc(&pool, (size_t)want * (two_measures ? 16u : 6u));
+        e = hipMalloc(&pool, (size_t)want * rec_bytes[fmt]);
         if (e == hipSuccess) e = hipMalloc(&chunk_part, nch * 4);
         if (e == hipSuccess) e = hipMalloc(&chunk_fill, nch * 4);
         if (e == hipSuccess) e = hipMalloc(&chunk_order, nch * 8);
         if (e == hipSuccess) {
             pool_cap = want;
             pool_ch_bits = ch_bits;
-            rec_wide = two_measures;
         }
+    }
+    if (e == hipSuccess) {
+        // a pool allocated for a wider format holds a narrower one
+        pool_fmt = fmt;
+        pool_mbase = mbase;
     }
     const size_t np = std::max<uint32_t>(nparts, 1);
     if (e == hipSuccess) e = grow_dev(part_chunks, ccap_[0], np + 1);
@@ -572,6 +592,8 @@ ChunkArgs Stager::chunk_args() const
     c.overflow = work + 1;
     c.ch_bits = pool_ch_bits;
     c.nparts = nparts;
+    c.fmt = pool_fmt;
+    c.mbase = pool_mbase;
     return c;
 }
 
@@ -598,14 +620,13 @@ hipError_t Stager::chunk_reduce(unsigned long long *gsum, unsigned long long *gc
     }
     if (e == hipSuccess) e = launch_zero_u32(work, 1, s);
     if (e != hipSuccess) return e;
-    if (rec_wide)
-        k_chunk_reduce<true><<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order,
-                                                                pool, pool_ch_bits, part_out, part_slots, work, gsum,
-                                                                gcnt);
-    else
-        k_chunk_reduce<false><<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order,
-                                                                 pool, pool_ch_bits, part_out, part_slots, work, gsum,
-                                                                 gcnt);
+    auto launch = [&](auto kernel) {
+        kernel<<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order, pool,
+                                                  pool_ch_bits, pool_mbase, part_out, part_slots, work, gsum, gcnt);
+    };
+    if (pool_fmt == REC_WIDE) launch(k_chunk_reduce<REC_WIDE>);
+    else if (pool_fmt == REC_COMPACT) launch(k_chunk_reduce<REC_COMPACT>);
+    else launch(k_chunk_reduce<REC_NARROW>);
     return hipGetLastError();
 }
 

@@ -85,9 +85,11 @@ def test_batch_matches_oracle_every_config(ctx, ncpu):
     b.close()
 
 
-def test_batch_two_measures_every_config(ctx, ncpu):
-    """A star with two measures (SUM of m1 + m2, 16-byte staged records):
-    every configuration equals the oracle, measures spanning the int32 range."""
+@pytest.mark.parametrize("measures", ["two", "one_wide"])
+def test_batch_two_measures_every_config(ctx, ncpu, measures):
+    """Measures spanning the int32 range: two of them (SUM of m1 + m2,
+    16-byte staged records), or one (6-byte one-pass records, the range being
+    too wide for 4-byte ones). Every configuration equals the oracle."""
     import torch
 
     rows = 300_007
@@ -97,8 +99,9 @@ def test_batch_two_measures_every_config(ctx, ncpu):
     host = list(host_star(cfg, rows))
     rng = np.random.default_rng(13)
     m1 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
-    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
-    dm1, dm2 = torch.from_numpy(m1).cuda(), torch.from_numpy(m2).cuda()
+    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32) if measures == "two" else None
+    dm1 = torch.from_numpy(m1).cuda()
+    dm2 = torch.from_numpy(m2).cuda() if m2 is not None else None
     star = oc.Context.star(ds.filters, ds.attrs, ds.fks, dm1, dm2, rows=rows)
     b = ctx.batch(star, [oc.BatchQuery.make(*q) for q in qs])
     expect = []
@@ -210,12 +213,15 @@ def test_batch_rejects_bad_queries(ctx):
         ctx.batch(ds.star, [oc.BatchQuery.make([(1, 0, 10, 2)], [1], 2)] * 65)
 
 
-@pytest.mark.parametrize("aggregation", [1, 2])
-def test_batch_graph_replay_after_columns_grow_matches(ctx, aggregation):
+@pytest.mark.parametrize("aggregation,new_measures", [(1, False), (2, False), (2, True)])
+def test_batch_graph_replay_after_columns_grow_matches(ctx, aggregation, new_measures):
     """A graph captured with record buffers sized for few matches, replayed
     after the fact columns change so that every row matches: the matches that
     no longer fit the buffers go to the outputs directly, and the replay
-    equals a fresh batch over the new columns and the closed form."""
+    equals a fresh batch over the new columns and the closed form. With
+    new_measures the measures also leave the range the one-pass records were
+    packed for (4-byte records take a measure offset of 20 bits), so those
+    matches are added directly too."""
     import torch
 
     rows, D, G = 1_000_000, 50_000, 20_000
@@ -242,6 +248,8 @@ def test_batch_graph_replay_after_columns_grow_matches(ctx, aggregation):
     b.capture(osum, ocnt, stream.cuda_stream)
     # every row now probes a passing dimension row
     fk.copy_(torch.randint(0, D // 100, (rows,), dtype=torch.int32, device="cuda", generator=gen) * 100)
+    if new_measures:
+        m1.copy_(torch.randint(-2 ** 30, 2 ** 30, (rows,), dtype=torch.int32, device="cuda", generator=gen))
     torch.cuda.synchronize()
     b.launch(stream.cuda_stream)
     torch.cuda.synchronize()
