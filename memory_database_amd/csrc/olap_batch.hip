@@ -76,8 +76,8 @@ struct BatchArgs {
     uint32_t m_lds;
     uint32_t ev_lds;
     uint32_t ev_cap;
-    // one-pass staging: per-partition chunk state in LDS (u64 current and
-    // spare chunk, then u32 fill, nparts each)
+    // one-pass staging: per-partition chunk state in LDS (u32 current and
+    // spare chunk ids, then u32 fill, nparts each)
     ChunkArgs chunk;
     uint32_t pc_lds;
 };
@@ -465,8 +465,22 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     // end reads the last pair in bounds, rows - 2 .. rows - 1 (rows >= 2; a
     // 4-byte-aligned load), and takes its second element for its first row;
     // rows past the end are masked out by al0/al1. The last tile loads
-    // itself again, unused columns read m1, and a gather off a table kept in
-    // LDS reads record 0 of the table grec_any.
+    // itself again and unused columns read m1.
+    //
+    // A gather takes the vector memory path one cache line per lane, and a
+    // wave stalls while that path is full: gathers issued by every wave at
+    // once cost each of them the whole burst. So the gathers (table t, row
+    // j of the lane's pair) are spread over three points of the tile:
+    //   A, after the offsets step (the keys have had the count step to
+    //      arrive): (0, 0), (1, 0);
+    //   B, before the barrier that ends the first round's list step:
+    //      (2, 0), (3, 0), (0, 1), (1, 1);
+    //   C, before the barrier that ends its events step: (2, 1), (3, 1).
+    // At B and C a wave that issues them would otherwise wait for slower
+    // waves. The host numbers the tables outside LDS 0, 2, 1, 3 in turn:
+    // 2-4 such tables put 1/2/1, 2/3/1 or 2/4/2 gathers off HBM at A/B/C
+    // (one: 1/1/0); a gather off a table kept in LDS reads record 0 of the
+    // table grec_any (one cache line for the wave).
     constexpr bool kGatherAhead = MODE == AGG_CHUNK;
     const uint64_t rlast = rows - 2u;
     const int32_t *colp[6];
@@ -489,20 +503,21 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         nm1 = ld2(colp[4]);
         nm2 = ld2(colp[5]);
     };
-    auto gather_tile = [&]() {
+    auto gather = [&](int t, int j) {
         if (!kGatherAhead) return;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const bool hbm = ((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds;
-            const uint4 *g = hbm ? a.grec[t] : a.grec_any;
-            const uint32_t kmax = hbm ? max(a.dim_rows[t], 1u) - 1u : 0u;
-#pragma unroll
-            for (int j = 0; j < 2; j++) ng[t][j] = g[min((uint32_t)(j ? nk[t].y : nk[t].x), kmax)];
-        }
+        const bool hbm = ((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds;
+        const uint4 *g = hbm ? a.grec[t] : a.grec_any;
+        const uint32_t kmax = hbm ? max(a.dim_rows[t], 1u) - 1u : 0u;
+        ng[t][j] = g[min((uint32_t)(j ? nk[t].y : nk[t].x), kmax)];
     };
+    auto gather_a = [&]() { gather(0, 0); gather(1, 0); };
+    auto gather_b = [&]() { gather(2, 0); gather(3, 0); gather(0, 1); gather(1, 1); };
+    auto gather_c = [&]() { gather(2, 1); gather(3, 1); };
     if (blockIdx.x < a.ntiles) {
         load_tile(blockIdx.x);
-        gather_tile();
+        gather_a();
+        gather_b();
+        gather_c();
     }
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * R + 2u * tid;
@@ -587,7 +602,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             if (tid == 63u) seg[64] = incl;
         }
         lds_barrier();
-        gather_tile();  // the next tile's keys have had the count and offset steps to arrive
+        gather_a();
+        bool late_gathers = true;  // B and C still to issue
         const uint32_t total = seg[64];
         // rounds over runs of whole query segments that fit the list (one
         // query's padded segment, at most R events, always fits). Lane q
@@ -617,6 +633,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const uint32_t end = seg[tid] + ((qreal[tid] + 63u) & ~63u);
                 for (uint32_t p = seg[tid] + qreal[tid]; p < end; p++) ev[p - lo] = 0xffffu | (tid << 16);
             }
+            if (late_gathers) gather_b();
             lds_barrier();
             // 5. events: each wave takes whole 64-event chunks, all of one query
             const uint32_t chunks = n >> 6;
@@ -626,37 +643,40 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
                 const c_desc &d = ((const c_desc *)a.desc)[q];
+                // the descriptor's words, then the attributes of all four
+                // steps and the measures, each set issued at once (one wait
+                // for each, not one per step). Every step is computed: a
+                // filter-only step, and a step past nsteps (table 0), has
+                // group_mod 0 and multiplier 0 and adds 0
                 const uint32_t G = d.G;
-                // the steps of the query (a wave-uniform branch on nsteps;
-                // a filter-only step has group_mod 0 and multiplier 0 and
-                // adds 0)
-                const uint32_t ns = d.nsteps;
-                auto step = [&](uint32_t st) {
-                    return (uint32_t)__umul64hi(d.M[st] * (uint64_t)rattr[d.tab[st] * R + row], (uint64_t)d.g[st]) *
-                           d.fac[st];
-                };
-                uint32_t gid = step(0);
-                if (ns > 1u) {
-                    gid += step(1);
-                    if (ns > 2u) {
-                        gid += step(2);
-                        if (ns > 3u) gid += step(3);
-                    }
+                const uint32_t lds_tab = d.lds_tab;
+                uint32_t tb[4], fac[4], gm[4];
+                uint64_t M[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    tb[s] = d.tab[s];
+                    fac[s] = d.fac[s];
+                    gm[s] = d.g[s];
+                    M[s] = d.M[s];
                 }
+                uint32_t av[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) av[s] = rattr[tb[s] * R + row];
+                const uint32_t m1v = rm[row];
+                const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
+                uint32_t gid = 0u;
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    gid += (uint32_t)__umul64hi(M[s] * (uint64_t)av[s], (uint64_t)gm[s]) * fac[s];
                 const bool ok = valid && gid < G;
                 if (valid && gid >= G)
                     __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (d.lds_tab != kNoLds) {
-                    if (MODE != AGG_SCATTER && ok) {
-                        const unsigned long long mv = (unsigned long long)(
-                            (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
-                        b_atomic_lds(lds + d.lds_tab, G, gid, mv);
-                    }
+                const long long mv = (long long)(int32_t)m1v + (long long)(int32_t)m2v;
+                if (lds_tab != kNoLds) {
+                    if (MODE != AGG_SCATTER && ok) b_atomic_lds(lds + lds_tab, G, gid, (unsigned long long)mv);
                 } else if (MODE == AGG_DIRECT) {
                     if (ok) {
-                        const unsigned long long mv = (unsigned long long)(
-                            (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
-                        b_atomic_dev(&a.gsum[d.out_off + gid], mv);
+                        b_atomic_dev(&a.gsum[d.out_off + gid], (unsigned long long)mv);
                         b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
                     }
                 } else {
@@ -665,8 +685,6 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     const uint32_t pbase = compact ? d.pbase_c : d.pbase;
                     const uint32_t part = pbase + (gid >> pbits);
                     const uint32_t glow = gid & ((1u << pbits) - 1u);
-                    const long long mv =
-                        (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll);
                     const long long moff = mv - (long long)a.chunk.mbase;
                     // a measure outside the range 4-byte records were packed
                     // for (the columns changed since the range was taken) is
@@ -682,6 +700,13 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     // counters: the COUNT pass's per-partition counts, the
                     // SCATTER pass's cursors, or the one-pass chunk fills
                     stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_fill : scur;
+                    // the partition's chunk ids (fixed during the events
+                    // step), read before the fill atomic
+                    uint32_t id_cur = 0u, id_spare = 0u;
+                    if (MODE == AGG_CHUNK && rec) {
+                        id_cur = pc_cur[part];
+                        id_spare = pc_spare[part];
+                    }
                     uint32_t pos = 0u;
                     if (G <= (1u << pbits)) {
                         // one partition: one counter / cursor atomic for the wave
@@ -692,7 +717,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                             if ((int)lane == leader)
                                 at = __hip_atomic_fetch_add(ctr + pbase, (uint32_t)__popcll(act),
                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            pos = (uint32_t)__shfl((int)at, leader) + lane_rank(act);
+                            pos = (uint32_t)__builtin_amdgcn_readlane((int)at, leader) + lane_rank(act);
                         }
                     } else if (rec) {
                         pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -700,7 +725,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                     if (MODE == AGG_CHUNK && rec) {
                         // a round adds at most R = CH records to a partition
                         // whose fill is below CH: pos < 2 CH
-                        const uint32_t id = pos < CH ? pc_cur[part] : pc_spare[part];
+                        const uint32_t id = pos < CH ? id_cur : id_spare;
                         const uint32_t off = pos < CH ? pos : pos - CH;
                         const unsigned long long base = (unsigned long long)id << a.chunk.ch_bits;
                         if (id == kNoChunk) {
@@ -712,28 +737,29 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         } else if (compact) {
                             ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | (uint32_t)moff;
                         } else if (a.m2) {
-                            ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, rm[row], rm[R + row], 0u);
+                            ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, m1v, m2v, 0u);
                         } else {
                             // one measure: a chunk holds CH u32 measures,
                             // then CH u16 slots (6 bytes a record)
                             char *cp = (char *)a.chunk.pool + 6ull * base;
-                            ((uint32_t *)cp)[off] = rm[row];
+                            ((uint32_t *)cp)[off] = m1v;
                             ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
                         }
                     }
                     if (MODE == AGG_SCATTER && ok) {
                         const unsigned long long idx = sstart[part] + pos;
                         if (idx >= a.stage.cap) {
-                            stage_overflow(a.stage, part, gid & (kPartSlots - 1u),
-                                           (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll));
+                            stage_overflow(a.stage, part, gid & (kPartSlots - 1u), mv);
                         } else if (a.stage.rec4) {
-                            a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), rm[row], a.m2 ? rm[R + row] : 0u, 0u);
+                            a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), m1v, m2v, 0u);
                         } else {
-                            a.stage.rec2[idx] = make_uint2(gid & (kPartSlots - 1u), rm[row]);
+                            a.stage.rec2[idx] = make_uint2(gid & (kPartSlots - 1u), m1v);
                         }
                     }
                 }
             }
+            if (late_gathers) gather_c();
+            late_gathers = false;
             lds_barrier();
             if (MODE == AGG_CHUNK) {
                 // 6. full chunks retire (fill CH), the spare becomes the
@@ -752,6 +778,10 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             }
             qa = qb;
             lo = hi;
+        }
+        if (late_gathers) {  // a tile without matches
+            gather_b();
+            gather_c();
         }
     }
     if (MODE == AGG_CHUNK) {
@@ -1034,6 +1064,37 @@ olap_status plan_config(olap_batch b, int c)
             cur += up16(4u * (uint32_t)b->star.dim_rows[t]);
         }
     }
+    // the kernels' numbering of the tables, pm[t]: the probed tables outside
+    // LDS take positions 0, 2, 1, 3 in turn (the one-pass form spreads its
+    // gathers by position), the others the positions left
+    int pm[4];
+    {
+        static const int kHbmPos[4] = {0, 2, 1, 3};
+        bool taken[4] = {false, false, false, false};
+        auto hbm = [&](int t) { return ((b->used >> t) & 1u) && a.mask_lds[t] == kNoLds; };
+        int n = 0;
+        for (int t = 0; t < 4; t++)
+            if (hbm(t)) {
+                pm[t] = kHbmPos[n++];
+                taken[pm[t]] = true;
+            }
+        int next = 0;
+        for (int t = 0; t < 4; t++) {
+            if (hbm(t)) continue;
+            while (taken[next]) next++;
+            pm[t] = next;
+            taken[next] = true;
+        }
+        uint32_t ml[4], al[4];
+        for (int t = 0; t < 4; t++) {
+            ml[pm[t]] = a.mask_lds[t];
+            al[pm[t]] = a.attr_lds[t];
+        }
+        for (int t = 0; t < 4; t++) {
+            a.mask_lds[t] = ml[t];
+            a.attr_lds[t] = al[t];
+        }
+    }
     a.tab_begin = cur / 4u;
     // query tables: highest expected events per byte first
     std::vector<int> qs((size_t)b->nq);
@@ -1076,7 +1137,7 @@ olap_status plan_config(olap_batch b, int c)
         pbase_c += (d.G + (1u << kCompactPartBits) - 1) >> kCompactPartBits;
         for (int s = 0; s < 4; s++) {
             const bool on = s < Q.ndims;
-            d.tab[s] = on ? (uint32_t)Q.table[s] : 0u;
+            d.tab[s] = on ? (uint32_t)pm[Q.table[s]] : 0u;
             d.fac[s] = on ? (uint32_t)Q.factor[s] : 0u;
             d.g[s] = on ? (uint32_t)Q.group_mod[s] : 0u;
             d.M[s] = d.g[s] ? (UINT64_MAX / d.g[s] + 1ull) : 0ull;
@@ -1086,20 +1147,23 @@ olap_status plan_config(olap_batch b, int c)
                                   b->setup_s);
     if (e == hipSuccess) e = hipStreamSynchronize(b->setup_s);  // desc is read by the copy
     if (e != hipSuccess) return bhip(ctx, e, "batch descriptors");
+    a.used = 0u;
     for (int t = 0; t < 4; t++) {
-        a.fk[t] = b->star.fk[t];
-        a.gmask[t] = b->ws->mask[t];
-        a.grec[t] = b->ws->rec[t];
-        a.gattr[t] = b->star.attr[t];
-        a.dim_rows[t] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
-        a.nouse[t] = b->nouse[t];
+        const int k = pm[t];
+        a.fk[k] = b->star.fk[t];
+        a.gmask[k] = b->ws->mask[t];
+        a.grec[k] = b->ws->rec[t];
+        a.gattr[k] = b->star.attr[t];
+        a.dim_rows[k] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
+        a.nouse[k] = b->nouse[t];
+        if ((b->used >> t) & 1u) {
+            a.used |= 1u << k;
+            if (!a.grec_any) a.grec_any = b->ws->rec[t];
+        }
     }
-    for (int t = 3; t >= 0; t--)
-        if ((b->used >> t) & 1u) a.grec_any = b->ws->rec[t];
     a.m1 = b->star.m1;
     a.m2 = b->star.m2;
     a.rows = (uint64_t)b->star.rows;
-    a.used = b->used;
     a.nq = (uint32_t)b->nq;
     a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
