@@ -27,22 +27,25 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kNoLds = 0xffffffffu;
 
-// one query as the batched kernel reads it (112 bytes)
+// one query as the batched kernels read it (112 bytes). The group-by terms
+// are kept by table (the kernels' numbering): gid = sum over the four tables
+// of (attr mod g[t]) * fac[t], a table the query does not group by having
+// g 0, multiplier 0 and factor 0 (it adds 0 whatever its attribute)
 struct BQDesc {
     uint32_t nsteps;
     uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when in HBM
     uint32_t G;
     uint32_t out_off;   // first slot in the concatenated outputs
-    uint32_t tab[4];
-    uint32_t fac[4];
-    uint32_t g[4];      // group_mod (0: code 0)
-    uint64_t M[4];      // fast-modulo multiplier: UINT64_MAX / g + 1
-    uint32_t pbase;     // first staged-aggregation partition of the query
+    uint32_t fac[4];    // words 4..7
+    uint32_t g[4];      // words 8..11: group_mod (0 or 1: code 0)
+    uint32_t pad0[4];
+    uint64_t M[4];      // words 16..23: fast-modulo multiplier UINT64_MAX / g + 1 (0 for g 0)
+    uint32_t pbase;     // word 24: first staged-aggregation partition of the query
     uint32_t pbase_c;   // the same with the compact one-pass partitions (2^12 slots)
-    uint32_t pad[2];
+    uint32_t pad1[2];
 };
 static_assert(sizeof(BQDesc) == 112, "descriptor layout");
-constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;  // word 16: M[], word 24: pbase
+constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;
 
 struct BatchArgs {
     const int32_t *fk[4];
@@ -202,12 +205,6 @@ __device__ __forceinline__ v4i bload4(const int32_t *__restrict__ p, uint64_t r,
     return v;
 }
 
-__device__ __forceinline__ uint32_t sel4(uint32_t t, uint32_t a0, uint32_t a1, uint32_t a2,
-                                         uint32_t a3)
-{
-    return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
-}
-
 // MODE AGG_DIRECT: queries without an LDS table use global atomics.
 // AGG_COUNT / AGG_SCATTER: the two passes of the staged aggregation; queries
 // with an LDS table are aggregated in the COUNT pass only.
@@ -299,13 +296,13 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
                     // descriptor words of query q (BQDesc layout) read from the
                     // LDS copy through an explicit address-space-3 pointer
                     const l_u32 *dw = (const l_u32 *)lds + a.desc_lds + kDescWords * (uint32_t)q;
-                    const uint32_t nsteps = dw[0], lds_tab = dw[1], G = dw[2];
+                    const uint32_t lds_tab = dw[1], G = dw[2];
                     uint32_t gid = 0;
-                    for (uint32_t s = 0; s < nsteps; s++) {
-                        const uint32_t av = sel4(dw[4 + s], at[0][k], at[1][k], at[2][k], at[3][k]);
-                        const uint64_t M = ((const l_u64 *)(dw + 16))[s];
-                        const uint32_t code = (uint32_t)__umul64hi(M * (uint64_t)av, (uint64_t)dw[12 + s]);
-                        gid += code * dw[8 + s];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const uint64_t M = ((const l_u64 *)(dw + 16))[t];
+                        const uint32_t code = (uint32_t)__umul64hi(M * (uint64_t)at[t][k], (uint64_t)dw[8 + t]);
+                        gid += code * dw[4 + t];
                     }
                     if (gid >= G) {
                         __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -643,31 +640,22 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
                 const c_desc &d = ((const c_desc *)a.desc)[q];
-                // the descriptor's words, then the attributes of all four
-                // steps and the measures, each set issued at once (one wait
-                // for each, not one per step). Every step is computed: a
-                // filter-only step, and a step past nsteps (table 0), has
-                // group_mod 0 and multiplier 0 and adds 0
-                const uint32_t G = d.G;
-                const uint32_t lds_tab = d.lds_tab;
-                uint32_t tb[4], fac[4], gm[4];
-                uint64_t M[4];
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    tb[s] = d.tab[s];
-                    fac[s] = d.fac[s];
-                    gm[s] = d.g[s];
-                    M[s] = d.M[s];
-                }
+                // the row's attributes of the four tables and its measures
+                // (at fixed LDS offsets, issued with the descriptor's scalar
+                // loads), then the group-by terms of every table (a table
+                // the query does not group by adds 0, whatever the staged
+                // word holds)
                 uint32_t av[4];
 #pragma unroll
-                for (int s = 0; s < 4; s++) av[s] = rattr[tb[s] * R + row];
+                for (int t = 0; t < 4; t++) av[t] = rattr[(uint32_t)t * R + row];
                 const uint32_t m1v = rm[row];
                 const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
+                const uint32_t G = d.G;
+                const uint32_t lds_tab = d.lds_tab;
                 uint32_t gid = 0u;
 #pragma unroll
-                for (int s = 0; s < 4; s++)
-                    gid += (uint32_t)__umul64hi(M[s] * (uint64_t)av[s], (uint64_t)gm[s]) * fac[s];
+                for (int t = 0; t < 4; t++)
+                    gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
                 const bool ok = valid && gid < G;
                 if (valid && gid >= G)
                     __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -713,11 +701,16 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         const unsigned long long act = __ballot(rec);
                         if (act) {
                             const int leader = __ffsll(act) - 1;
+                            // the leader's rank is 0: adding it keeps the
+                            // address per lane, so the compiler does not
+                            // rewrite this single-lane atomic as a wave
+                            // reduction
+                            const uint32_t rank = lane_rank(act);
                             uint32_t at = 0u;
                             if ((int)lane == leader)
-                                at = __hip_atomic_fetch_add(ctr + pbase, (uint32_t)__popcll(act),
+                                at = __hip_atomic_fetch_add(ctr + pbase + rank, (uint32_t)__popcll(act),
                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            pos = (uint32_t)__builtin_amdgcn_readlane((int)at, leader) + lane_rank(act);
+                            pos = (uint32_t)__builtin_amdgcn_readlane((int)at, leader) + rank;
                         }
                     } else if (rec) {
                         pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1135,12 +1128,11 @@ olap_status plan_config(olap_batch b, int c)
         pbase += (d.G + kPartSlots - 1) / kPartSlots;
         d.pbase_c = pbase_c;
         pbase_c += (d.G + (1u << kCompactPartBits) - 1) >> kCompactPartBits;
-        for (int s = 0; s < 4; s++) {
-            const bool on = s < Q.ndims;
-            d.tab[s] = on ? (uint32_t)pm[Q.table[s]] : 0u;
-            d.fac[s] = on ? (uint32_t)Q.factor[s] : 0u;
-            d.g[s] = on ? (uint32_t)Q.group_mod[s] : 0u;
-            d.M[s] = d.g[s] ? (UINT64_MAX / d.g[s] + 1ull) : 0ull;
+        for (int s = 0; s < Q.ndims; s++) {
+            const int t = pm[Q.table[s]];
+            d.fac[t] = (uint32_t)Q.factor[s];
+            d.g[t] = (uint32_t)Q.group_mod[s];
+            d.M[t] = d.g[t] ? (UINT64_MAX / d.g[t] + 1ull) : 0ull;
         }
     }
     hipError_t e = hipMemcpyAsync(b->ws->desc[c], desc.data(), sizeof(BQDesc) * b->nq, hipMemcpyHostToDevice,
