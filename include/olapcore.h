@@ -221,7 +221,10 @@ typedef struct olap_batch_s *olap_batch;
 
 /* Validate the queries, build the per-table query masks (bit q of mask[t][r]
  * set when query q passes dimension row r of table t or does not probe t)
- * and plan the LDS layout (synchronises). */
+ * with each dimension row's group attribute beside its mask, and plan the
+ * LDS layout (synchronises). The dimension columns are read here only: runs
+ * use the masks and attributes as of creation (the fact columns are read by
+ * every run). */
 olap_status olap_batch_create(olap_ctx ctx, const olap_star *star, const olap_batch_query *queries,
                               int32_t nq, olap_batch *out);
 /* offsets[q] = first output slot of query q, offsets[nq] = total slots (host) */

@@ -54,7 +54,6 @@ struct BatchArgs {
     uint64_t rows;
     uint64_t ntiles;
     const unsigned long long *gmask[4];
-    const int32_t *gattr[4];
     const uint4 *grec[4];   // per dimension row: mask (2 words), attribute, 0
     const uint4 *grec_any;  // the records of some probed table (a valid address)
     uint32_t dim_rows[4];
@@ -219,10 +218,8 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
             const uint32_t *src = (const uint32_t *)a.gmask[t];
             for (uint32_t i = threadIdx.x; i < 2u * a.dim_rows[t]; i += BS) lds[a.mask_lds[t] + i] = src[i];
         }
-        if (a.attr_lds[t] != kNoLds) {
-            const uint32_t *src = (const uint32_t *)a.gattr[t];
-            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = src[i];
-        }
+        if (a.attr_lds[t] != kNoLds)  // the attributes as the records hold them
+            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = a.grec[t][i].z;
     }
     {
         const uint32_t *src = (const uint32_t *)a.desc;
@@ -410,10 +407,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             const uint32_t *src = (const uint32_t *)a.gmask[t];
             for (uint32_t i = threadIdx.x; i < 2u * a.dim_rows[t]; i += BS) lds[a.mask_lds[t] + i] = src[i];
         }
-        if (a.attr_lds[t] != kNoLds) {
-            const uint32_t *src = (const uint32_t *)a.gattr[t];
-            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = src[i];
-        }
+        if (a.attr_lds[t] != kNoLds)  // the attributes as the records hold them
+            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = a.grec[t][i].z;
     }
     // (descriptors are read from global memory with scalar loads: no LDS copy)
     for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
@@ -1145,7 +1140,6 @@ olap_status plan_config(olap_batch b, int c)
         a.fk[k] = b->star.fk[t];
         a.gmask[k] = b->ws->mask[t];
         a.grec[k] = b->ws->rec[t];
-        a.gattr[k] = b->star.attr[t];
         a.dim_rows[k] = (uint32_t)std::max<int64_t>(b->star.dim_rows[t], 0);
         a.nouse[k] = b->nouse[t];
         if ((b->used >> t) & 1u) {

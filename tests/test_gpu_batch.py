@@ -132,6 +132,46 @@ def test_batch_two_measures_every_config(ctx, ncpu, measures):
     b.close()
 
 
+def test_batch_filter_only_table_without_attribute_column(ctx, ncpu):
+    """A table that the queries only filter on may come without a group
+    attribute column (NULL); the small date table is then still copied into
+    LDS by the configurations that keep small tables there. Every
+    configuration equals the oracle."""
+    import torch
+
+    rows = 200_003
+    cfg = wl.scaled(wl.STAR_4DIM_600M, rows, name="batch_noattr")
+    ds = wl.DeviceBatchStar(ctx, cfg)
+    host = list(host_star(cfg, rows))
+    # date (slot 3) filter-only in every query that probes it
+    qs = [([(3, 0, 50, 0), (0, 0, 30, 25)], [0, 1], 25),
+          ([(3, 10, 60, 0)], [0], 1),
+          ([(3, 0, 90, 0), (2, 0, 10, 7), (1, 0, 40, 5)], [0, 1, 7], 35),
+          ([(1, 5, 70, 5), (3, 20, 100, 0)], [1, 0], 5),
+          ([(0, 0, 100, 25)], [1], 25)]
+    star = oc.Context.star(ds.filters, [ds.attrs[0], ds.attrs[1], ds.attrs[2], None], ds.fks, ds.m1, None,
+                           rows=rows)
+    b = ctx.batch(star, [oc.BatchQuery.make(*q) for q in qs])
+    expect = [oracle_query(host, steps, factor, G, ncpu) for steps, factor, G in qs]
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        try:
+            b.set_config(k)
+        except oc.OlapError as e:
+            _say(f"  config {k} skipped: {e}")
+            continue
+        _say(f"  config {k}")
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        s, c = osum.cpu().numpy(), ocnt.cpu().numpy()
+        for i, (es, ec) in enumerate(expect):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            np.testing.assert_array_equal(c[lo:hi], ec, err_msg=f"config {k} query {i} counts")
+            np.testing.assert_array_equal(s[lo:hi], es, err_msg=f"config {k} query {i} sums")
+    b.close()
+
+
 def test_batch_small_groups_and_ragged_rows(ctx, ncpu):
     """Small group counts (LDS tables for every query), a ragged row count and
     a batch of one."""
