@@ -511,6 +511,23 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         gather_b();
         gather_c();
     }
+    // 6. (one-pass staging) full chunks retire (fill CH), the spare becomes
+    // the current chunk and a new spare is claimed. It runs between the
+    // rounds of a tile, and for the last round of a tile while the next
+    // tile's offsets are computed (no step in between touches the chunk
+    // state; the barrier after the offsets orders it before the events)
+    auto retire_chunks = [&]() {
+        for (uint32_t p = tid; p < np; p += BS) {
+            const uint32_t f = pc_fill[p];
+            if (f >= CH) {
+                const uint32_t cur = pc_cur[p];
+                if (cur != kNoChunk) a.chunk.chunk_fill[cur] = CH;
+                pc_cur[p] = pc_spare[p];
+                pc_fill[p] = f - CH;
+                pc_spare[p] = chunk_claim(a.chunk, p);
+            }
+        }
+    };
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t r0 = tile * R + 2u * tid;
         const uint64_t next = tile + gridDim.x < a.ntiles ? tile + gridDim.x : tile;
@@ -593,6 +610,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             qcnt[tid] = 0u;  // ready for the next tile (used again after two barriers)
             if (tid == 63u) seg[64] = incl;
         }
+        if (MODE == AGG_CHUNK) retire_chunks();  // after the previous tile's last round
         lds_barrier();
         gather_a();
         bool late_gathers = true;  // B and C still to issue
@@ -749,19 +767,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             if (late_gathers) gather_c();
             late_gathers = false;
             lds_barrier();
-            if (MODE == AGG_CHUNK) {
-                // 6. full chunks retire (fill CH), the spare becomes the
-                // current chunk and a new spare is claimed
-                for (uint32_t p = tid; p < np; p += BS) {
-                    const uint32_t f = pc_fill[p];
-                    if (f >= CH) {
-                        const uint32_t cur = pc_cur[p];
-                        if (cur != kNoChunk) a.chunk.chunk_fill[cur] = CH;
-                        pc_cur[p] = pc_spare[p];
-                        pc_fill[p] = f - CH;
-                        pc_spare[p] = chunk_claim(a.chunk, p);
-                    }
-                }
+            if (MODE == AGG_CHUNK && qb < 64u && hi < total) {  // another round follows
+                retire_chunks();
                 lds_barrier();
             }
             qa = qb;
@@ -773,7 +780,9 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         }
     }
     if (MODE == AGG_CHUNK) {
-        // fills of the chunks still open (fill < CH after the last round)
+        retire_chunks();  // after the last tile's last round
+        lds_barrier();
+        // fills of the chunks still open (fill < CH)
         for (uint32_t p = tid; p < np; p += BS) {
             if (pc_cur[p] != kNoChunk) a.chunk.chunk_fill[pc_cur[p]] = pc_fill[p];
             if (pc_spare[p] != kNoChunk) a.chunk.chunk_fill[pc_spare[p]] = 0u;
