@@ -32,7 +32,7 @@ constexpr uint32_t kNoLds = 0xffffffffu;
 // of (attr mod g[t]) * fac[t], a table the query does not group by having
 // g 0, multiplier 0 and factor 0 (it adds 0 whatever its attribute)
 struct BQDesc {
-    uint32_t nsteps;
+    uint32_t nsteps;    // probe steps (the kernels compute all four table terms)
     uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when in HBM
     uint32_t G;
     uint32_t out_off;   // first slot in the concatenated outputs
@@ -354,8 +354,9 @@ __global__ __launch_bounds__(BS) void k_batch(BatchArgs a)
 //     of one aggregation partition takes one counter/cursor atomic per wave.
 // A tile with more matches than the list holds runs steps 4-5 in rounds, each
 // over a run of whole query segments. In the one-pass staged mode every event
-// of step 5 writes its record, and a sixth step after each round retires the
-// full record chunks.
+// of step 5 writes its record, and a sixth step retires the full record
+// chunks: between rounds, and after a tile's last round while the next
+// tile's offsets are computed.
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long m)
 {
