@@ -648,8 +648,12 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
             lds_barrier();
             // 5. events: each wave takes whole 64-event chunks, all of one query
             const uint32_t chunks = n >> 6;
+            // each chunk's event words are read one chunk ahead (the index
+            // clamped to the last chunk: the read needs no branch)
+            uint32_t e_next = ev[min(tid >> 6, chunks - 1u) * 64u + lane];
             for (uint32_t ch = tid >> 6; ch < chunks; ch += BS / 64u) {
-                const uint32_t e = ev[ch * 64u + lane];
+                const uint32_t e = e_next;
+                e_next = ev[min(ch + BS / 64u, chunks - 1u) * 64u + lane];
                 const uint32_t q = __builtin_amdgcn_readfirstlane(e >> 16);
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
