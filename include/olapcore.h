@@ -146,7 +146,9 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape);
  * -1 automatic (from the expected matches), 0 one global atomic pair per
  * matching row, 1 staged (count, scatter records to HBM, reduce per
  * 8192-slot partition in LDS; synchronises once per run to size the record
- * buffer). The autotuner also tries both. */
+ * buffer), 2 as 0 behind a per-workgroup LDS cache of up to 4096 groups,
+ * which the first groups to reach a slot keep (skewed keys: the hot groups
+ * stop contending on global atomics). The autotuner also tries all three. */
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out);

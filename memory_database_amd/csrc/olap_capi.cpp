@@ -795,9 +795,23 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
             return run_rowwise_stage(q, sum_p, count, s);
         ScanArgs a;
         fill_args(a, q, L, sum_p, count);
+        uint32_t dyn = L.dyn_lds;
+        if (strategy == OLAP_ROWWISE && !L.gt_lds && q->agg_layout == 2) {
+            // global atomics behind an LDS cache of hot groups: 16 bytes a
+            // slot, 256..4096 slots in the LDS the steps leave
+            const uint32_t base = align_up(L.dyn_lds, 16u);
+            const uint32_t avail = kLdsBytes - 16u - std::min(base, kLdsBytes - 16u);
+            uint32_t bits = 12;
+            while (bits > 8 && (16u << bits) > avail) bits--;
+            if ((16u << bits) <= avail) {
+                a.hot_off = base / 4u;
+                a.hot_bits = bits;
+                dyn = base + (16u << bits);
+            }
+        }
         hipError_t e;
         if (strategy == OLAP_ROWWISE)
-            e = run_rowwise(a, nd, ctx->num_cus, L.dyn_lds, vec, effective_shape(q), s);
+            e = run_rowwise(a, nd, ctx->num_cus, dyn, vec, effective_shape(q), s);
         else if (strategy == OLAP_VWM_SV) e = run_vwm_sv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         else e = run_vwm_dv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "scan kernel");
@@ -860,7 +874,7 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape)
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout)
 {
     if (!q) return OLAP_E_INVALID;
-    if (layout < -1 || layout > 1) return fail(q->ctx, OLAP_E_INVALID, "layout must be -1, 0 or 1");
+    if (layout < -1 || layout > 2) return fail(q->ctx, OLAP_E_INVALID, "layout must be -1, 0, 1 or 2");
     q->agg_layout = layout;
     return OLAP_OK;
 }
@@ -873,11 +887,14 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
     DeviceGuard g(ctx->device);
     hipStream_t s = as_stream(stream);
     // candidates: every launch shape; a row-wise run whose group table exceeds
-    // LDS also tries the staged aggregation (one fixed shape)
+    // LDS also tries every shape behind the LDS cache of hot groups and the
+    // staged aggregation (one fixed shape)
     std::vector<std::pair<int, int>> cands;  // (shape, aggregation layout)
     const bool table_in_lds = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1).gt_lds != 0;
     for (int i = 0; i < olap_num_shapes(strategy); i++)
         cands.push_back({i, strategy == OLAP_ROWWISE && !table_in_lds ? 0 : -1});
+    if (strategy == OLAP_ROWWISE && !table_in_lds)  // the same behind an LDS cache of hot groups
+        for (int i = 0; i < olap_num_shapes(strategy); i++) cands.push_back({i, 2});
     if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) cands.push_back({0, 1});
     const int n = (int)cands.size();
     hipEvent_t e0, e1;

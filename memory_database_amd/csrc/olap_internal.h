@@ -19,6 +19,7 @@ constexpr int kTileRW = kBlock * kChunk * kUnroll;  // 8192 rows
 constexpr int kTileV = 4096;             // rows per tile of the compacting kernels
 constexpr int kPerThreadV = kTileV / kBlock;        // list entries per thread (8)
 constexpr uint32_t kLdsBytes = 160 * 1024;          // LDS per CU on gfx950
+constexpr uint32_t kHotEmpty = 0xffffffffu;         // an unclaimed slot of the LDS cache of hot groups
 constexpr uint32_t kLdsListBytes = kTileV * 2 + kTileV * 4 + 16;  // u16 off + u32 gid + counter
 
 struct ScanShape {
@@ -57,6 +58,12 @@ struct ScanArgs {
     uint32_t count_only;        // no measure: only counts
     uint32_t eager1;            // row-wise: load step 1's keys with step 0's (most of its lines are live);
                                 // selects the kernel variant, not read by the kernels
+    // group table beyond LDS: an LDS cache of 2^hot_bits groups (0: none) at
+    // word offset hot_off -- tags [S] (kHotEmpty or the group the slot
+    // holds), counts [S], sums [S] (u64) -- that absorbs the matches of the
+    // groups that claim a slot first; every other match takes global atomics
+    uint32_t hot_off;
+    uint32_t hot_bits;
     // column-wise state
     int32_t *gvec;              // per-row gid vector (static-vector column-wise)
     int32_t gvec_null;          // NULL value written into gvec

@@ -363,6 +363,15 @@ __device__ __forceinline__ void lds_init(const ScanArgs &a, uint32_t *lds)
             lsum[g] = 0ull;
             lcnt[g] = 0u;
         }
+    } else if (a.hot_bits) {
+        const uint32_t S = 1u << a.hot_bits;
+        uint32_t *tags = lds + a.hot_off;
+        unsigned long long *hsum = (unsigned long long *)(tags + 2u * S);
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            tags[i] = kHotEmpty;
+            tags[S + i] = 0u;
+            hsum[i] = 0ull;
+        }
     }
 }
 
@@ -380,6 +389,24 @@ __device__ __forceinline__ void aggregate(const ScanArgs &a, uint32_t *lds, uint
         if (!a.count_only) atomic_add_u64_lds(&lsum[gid], m);
         atomic_add_u32_lds(&lcnt[gid], 1u);
     } else {
+        if (a.hot_bits) {
+            // the group's cache slot (multiplicative hash); an empty slot is
+            // claimed by the first group that reaches it and keeps it
+            const uint32_t S = 1u << a.hot_bits;
+            uint32_t *tags = lds + a.hot_off;
+            const uint32_t slot = (gid * 0x9e3779b1u) >> (32u - a.hot_bits);
+            uint32_t tag = __hip_atomic_load(&tags[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (tag == kHotEmpty) {
+                __hip_atomic_compare_exchange_strong(&tags[slot], &tag, gid, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (tag == kHotEmpty) tag = gid;  // claimed here
+            }
+            if (tag == gid) {
+                if (!a.count_only) atomic_add_u64_lds((unsigned long long *)(tags + 2u * S) + slot, m);
+                atomic_add_u32_lds(tags + S + slot, 1u);
+                return;
+            }
+        }
         if (!a.count_only) atomic_add_u64_dev(&a.gsum[gid], m);
         atomic_add_u64_dev(&a.gcnt[gid], 1ull);
     }
@@ -388,7 +415,22 @@ __device__ __forceinline__ void aggregate(const ScanArgs &a, uint32_t *lds, uint
 template <bool GLDS>
 __device__ __forceinline__ void flush(const ScanArgs &a, uint32_t *lds)
 {
-    if (!GLDS) return;
+    if (!GLDS) {
+        if (!a.hot_bits) return;
+        // the cached groups go out with one atomic pair per slot
+        __syncthreads();
+        const uint32_t S = 1u << a.hot_bits;
+        const uint32_t *tags = lds + a.hot_off;
+        const unsigned long long *hsum = (const unsigned long long *)(tags + 2u * S);
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            const uint32_t c = tags[S + i];
+            if (c) {
+                atomic_add_u64_dev(&a.gcnt[tags[i]], (unsigned long long)c);
+                if (!a.count_only) atomic_add_u64_dev(&a.gsum[tags[i]], hsum[i]);
+            }
+        }
+        return;
+    }
     __syncthreads();
     const unsigned long long *lsum = (const unsigned long long *)(lds + a.gt_off);
     const uint32_t *lcnt = lds + a.gt_off + 2u * a.G;

@@ -480,7 +480,8 @@ class Query:
         self.ctx.check(self.ctx.lib.olap_query_set_shape(self.h, shape), "set_shape")
 
     def set_aggregation(self, layout: int):
-        """-1 automatic, 0 global atomics, 1 staged (group tables beyond LDS)."""
+        """-1 automatic, 0 global atomics, 1 staged, 2 global atomics behind an
+        LDS cache of hot groups (group tables beyond LDS)."""
         self.ctx.check(self.ctx.lib.olap_query_set_aggregation(self.h, layout), "set_aggregation")
 
     def device_bytes(self) -> int:

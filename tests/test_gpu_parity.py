@@ -160,19 +160,25 @@ def test_wide_codes_and_global_group_table(ctx, ncpu):
     check_all(ctx, p, dvs, fks, m, None, ncpu)
 
 
-@pytest.mark.parametrize("G,sel,rows,two", [(100_000, 60, 3_000_001, False),
-                                            (1_000_000, 90, 2_000_000, True),
-                                            (5_000, 100, 1_000_003, False)])
-def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two):
+@pytest.mark.parametrize("G,sel,rows,two,skew", [(100_000, 60, 3_000_001, False, False),
+                                                 (1_000_000, 90, 2_000_000, True, False),
+                                                 (1_000_000, 90, 2_000_000, False, True),
+                                                 (5_000, 100, 1_000_003, False, False)])
+def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two, skew):
     """Group tables beyond LDS: global atomics, staged (count / scatter /
-    reduce) and the automatic choice all give the oracle's result."""
+    reduce), global atomics behind the LDS cache of hot groups and the
+    automatic choice all give the oracle's result, with uniform group codes
+    and with codes skewed onto a few hot groups."""
     import torch
 
     rng = np.random.default_rng(G + sel)
-    _say(f"staged aggregation G={G} sel={sel}% rows={rows} two_measures={two}")
+    _say(f"staged aggregation G={G} sel={sel}% rows={rows} two_measures={two} skew={skew}")
     D = 400_000
     f = rng.integers(0, 100, D, dtype=np.int32)
-    g = rng.integers(0, G, D, dtype=np.int32)
+    if skew:  # Zipf-like: most dimension rows carry one of a few group codes
+        g = np.minimum(rng.zipf(1.3, D) - 1, G - 1).astype(np.int32)
+    else:
+        g = rng.integers(0, G, D, dtype=np.int32)
     dv = po.build_dimvec(f, g, 0, sel, 1, 4)
     D2 = 3000
     dv2 = po.build_dimvec(rng.integers(0, 100, D2, dtype=np.int32), np.zeros(D2, np.int32),
@@ -190,7 +196,7 @@ def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two):
     q = ctx.query(plan, cols)
     osum = torch.empty(G, dtype=torch.int64, device="cuda")
     ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
-    for layout in (0, 1, -1, 1):
+    for layout in (0, 1, 2, -1, 1, 2):
         _say(f"  layout {layout}")
         q.set_aggregation(layout)
         q.run("rowwise", osum, ocnt)
