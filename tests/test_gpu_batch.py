@@ -172,6 +172,42 @@ def test_batch_filter_only_table_without_attribute_column(ctx, ncpu):
     b.close()
 
 
+@pytest.mark.parametrize("sizes", [(2_556, 30_000, 2_000, 2_000), (2_556, 30_000, 200_000, 60_000)],
+                         ids=["one_table_in_hbm", "three_tables_in_hbm"])
+def test_batch_tables_outside_lds_every_config(ctx, ncpu, sizes):
+    """Stars whose configurations keep one or three of the four tables
+    outside LDS (the kernels renumber those tables and spread their gathers
+    by position). Every configuration equals the oracle."""
+    import torch
+
+    rows = 300_007
+    d, c, p, s = sizes
+    dims = [wl.Dim(3, d, 50, 7), wl.Dim(0, c, 20, 25), wl.Dim(2, p, 4, 1), wl.Dim(1, s, 20, 5)]
+    cfg = wl.scaled(wl.STAR_4DIM_600M, rows, name=f"batch_sizes_{p}_{s}", dims=dims)
+    ds = wl.DeviceBatchStar(ctx, cfg)
+    qs = wl.batch_queries(cfg, 24, 17, max_groups=20_000)
+    host = host_star(cfg, rows)
+    b = ctx.batch(ds.star, [oc.BatchQuery.make(*q) for q in qs])
+    expect = [oracle_query(host, steps, factor, G, ncpu) for steps, factor, G in qs]
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        try:
+            b.set_config(k)
+        except oc.OlapError as e:
+            _say(f"  config {k} skipped: {e}")
+            continue
+        _say(f"  config {k}")
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        sm, cn = osum.cpu().numpy(), ocnt.cpu().numpy()
+        for i, (es, ec) in enumerate(expect):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            np.testing.assert_array_equal(cn[lo:hi], ec, err_msg=f"config {k} query {i} counts")
+            np.testing.assert_array_equal(sm[lo:hi], es, err_msg=f"config {k} query {i} sums")
+    b.close()
+
+
 def test_batch_small_groups_and_ragged_rows(ctx, ncpu):
     """Small group counts (LDS tables for every query), a ragged row count and
     a batch of one."""
