@@ -148,7 +148,9 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape);
  * 8192-slot partition in LDS; synchronises once per run to size the record
  * buffer), 2 as 0 behind a per-workgroup LDS cache of up to 4096 groups,
  * which the first groups to reach a slot keep (skewed keys: the hot groups
- * stop contending on global atomics). The autotuner also tries all three. */
+ * stop contending on global atomics), 3 one-pass staged (one scan writes
+ * every match's record into chunks of a record pool, per partition; one
+ * reduce; no host synchronisation). The autotuner also tries all four. */
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out);

@@ -376,19 +376,6 @@ __device__ __forceinline__ void lds_barrier()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// claim a chunk of the pool for partition p: its id (first record id <<
-// ch_bits), or kNoChunk when the pool is used up
-__device__ __forceinline__ uint32_t chunk_claim(const ChunkArgs &c, uint32_t p)
-{
-    const unsigned long long ch = 1ull << c.ch_bits;
-    const unsigned long long nc =
-        __hip_atomic_fetch_add(c.cursor, ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (nc + ch > c.cap) return kNoChunk;
-    const uint32_t id = (uint32_t)(nc >> c.ch_bits);
-    c.chunk_part[id] = p;
-    return id;
-}
-
 // w rotated right by r (0..63)
 __device__ __forceinline__ unsigned long long rotr64(unsigned long long w, uint32_t r)
 {

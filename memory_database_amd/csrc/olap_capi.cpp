@@ -325,6 +325,47 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     return OLAP_OK;
 }
 
+// one-pass staged aggregation of a group table beyond LDS: every match
+// writes a record into its partition's pool chunk during the scan, and one
+// reduce adds the records up (no count pass, no host read)
+olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, hipStream_t s)
+{
+    olap_ctx ctx = q->ctx;
+    const int nd = q->plan.dimvec_nums;
+    const uint32_t G = (uint32_t)q->plan.group_nums;
+    const uint32_t nparts = (G + kPartSlots - 1) / kPartSlots;
+    if (nparts > kMaxParts) return fail(ctx, OLAP_E_CAPACITY, "too many partitions");
+    const uint32_t pc_bytes = align_up(nparts * 12u, 16u);  // current, spare, fill per partition
+    Layout L = plan_layout(q, true, false, false, -1, pc_bytes);
+    const uint32_t pc_lds = align_up(L.dyn_lds, 16u) / 4u;
+    const uint32_t dyn = pc_lds * 4u + pc_bytes;
+    uint32_t grid = 0;
+    HIPC(ctx, rowwise_chunk_grid(nd, ctx->num_cus, dyn, (uint64_t)q->cols.rows, &grid));
+    if (!q->stager) q->stager = new Stager();
+    Stager &S = *q->stager;
+    if (S.nparts != nparts || S.nwg != grid || S.part_bits != kPartBits) {
+        const unsigned long long off0 = 0;
+        HIPC(ctx, S.setup(&off0, &G, 1, grid, q->cols.m2 != nullptr, s));
+    }
+    const uint32_t ch_bits = (uint32_t)__builtin_ctz(rowwise_chunk_rows());
+    const uint32_t fmt = q->cols.m2 ? REC_WIDE : REC_NARROW;
+    // every workgroup holds two chunks open per partition: refuse a reserve
+    // above 1/32 of HBM (many partitions), which the two-pass form avoids
+    const double open_bytes = 2.0 * nparts * grid * (double)rowwise_chunk_rows() * (fmt == REC_WIDE ? 16.0 : 6.0);
+    if (open_bytes > (double)std::max<int64_t>(ctx->hbm_bytes, 1ll << 32) / 32.0)
+        return fail(ctx, OLAP_E_CAPACITY, "one-pass staging: %u partitions x %u workgroups would hold %.1f GB of open chunks",
+                    nparts, grid, open_bytes / 1e9);
+    HIPC(ctx, S.chunk_reserve((unsigned long long)std::max(0.0, q->est_events), grid, ch_bits, fmt, 0));
+    HIPC(ctx, S.chunk_clear(s));
+    ScanArgs a;
+    fill_args(a, q, L, sum, count);
+    a.chunk = S.chunk_args();
+    a.pc_lds = pc_lds;
+    HIPC(ctx, run_rowwise_chunk(a, nd, dyn, grid, s));
+    HIPC(ctx, S.chunk_reduce((unsigned long long *)sum, (unsigned long long *)count, ctx->num_cus, s));
+    return OLAP_OK;
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -791,6 +832,8 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
     case OLAP_VWM_SV:
     case OLAP_VWM_DV: {
         Layout L = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1);
+        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && q->agg_layout == 3)
+            return run_rowwise_onepass(q, sum_p, count, s);
         if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q))
             return run_rowwise_stage(q, sum_p, count, s);
         ScanArgs a;
@@ -874,7 +917,7 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape)
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout)
 {
     if (!q) return OLAP_E_INVALID;
-    if (layout < -1 || layout > 2) return fail(q->ctx, OLAP_E_INVALID, "layout must be -1, 0, 1 or 2");
+    if (layout < -1 || layout > 3) return fail(q->ctx, OLAP_E_INVALID, "layout must be -1, 0, 1, 2 or 3");
     q->agg_layout = layout;
     return OLAP_OK;
 }
@@ -887,15 +930,19 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
     DeviceGuard g(ctx->device);
     hipStream_t s = as_stream(stream);
     // candidates: every launch shape; a row-wise run whose group table exceeds
-    // LDS also tries every shape behind the LDS cache of hot groups and the
-    // staged aggregation (one fixed shape)
+    // LDS also tries every shape behind the LDS cache of hot groups, and the
+    // two-pass and one-pass staged aggregations (one fixed shape each; the
+    // one-pass form is skipped where its open chunks would not fit)
     std::vector<std::pair<int, int>> cands;  // (shape, aggregation layout)
     const bool table_in_lds = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1).gt_lds != 0;
     for (int i = 0; i < olap_num_shapes(strategy); i++)
         cands.push_back({i, strategy == OLAP_ROWWISE && !table_in_lds ? 0 : -1});
     if (strategy == OLAP_ROWWISE && !table_in_lds)  // the same behind an LDS cache of hot groups
         for (int i = 0; i < olap_num_shapes(strategy); i++) cands.push_back({i, 2});
-    if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) cands.push_back({0, 1});
+    if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) {
+        cands.push_back({0, 1});
+        cands.push_back({0, 3});
+    }
     const int n = (int)cands.size();
     hipEvent_t e0, e1;
     HIPC(ctx, hipEventCreate(&e0));
@@ -909,6 +956,10 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
         q->row_shape = cands[(size_t)i].first;
         q->agg_layout = cands[(size_t)i].second;
         st = olap_query_run(q, strategy, sum, count, stream);  // warm
+        if (st == OLAP_E_CAPACITY && cands[(size_t)i].second == 3) {  // one-pass too large here
+            st = OLAP_OK;
+            continue;
+        }
         std::vector<double> t;
         for (int r = 0; r < 3 && st == OLAP_OK; r++) {
             hipError_t e = hipEventRecord(e0, s);
@@ -944,8 +995,9 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
             found = true;
         }
     if (!found) ctx->tuned.push_back({key, q->row_shape});
-    // shape index; num_shapes(strategy) stands for the staged aggregation
-    if (shape_out) *shape_out = q->agg_layout == 1 ? olap_num_shapes(strategy) : q->row_shape;
+    // shape index; num_shapes(strategy) stands for a staged aggregation
+    if (shape_out)
+        *shape_out = q->agg_layout == 1 || q->agg_layout == 3 ? olap_num_shapes(strategy) : q->row_shape;
     if (ms_out) *ms_out = best_ms;
     // leave the outputs holding a complete run of the chosen shape
     return olap_query_run(q, strategy, sum, count, stream);

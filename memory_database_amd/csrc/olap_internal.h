@@ -73,6 +73,11 @@ struct ScanArgs {
     int32_t *list_gid;
     uint32_t *list_cnt;         // entries per tile
     StageArgs stage;            // staged aggregation passes
+    // one-pass staging (AGG_CHUNK): the record pool and, at word offset
+    // pc_lds, the per-partition chunk state (u32 current and spare chunk
+    // ids, then u32 fill, chunk.nparts each)
+    ChunkArgs chunk;
+    uint32_t pc_lds;
 };
 
 }  // namespace olap
@@ -146,6 +151,11 @@ hipError_t launch_pack(const void *dimvec, int32_t code_bytes, int64_t rows, uin
                        uint32_t *words, uint32_t nwords, hipStream_t s);
 
 // strategy kernels; args.gsum/gcnt must be zeroed by the caller
+// one-pass staged row-wise scan (AGG_CHUNK): rows per tile (= records per
+// pool chunk), the persistent grid, the launch
+uint32_t rowwise_chunk_rows();
+hipError_t rowwise_chunk_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows, uint32_t *grid);
+hipError_t run_rowwise_chunk(ScanArgs &a, int ndims, uint32_t dyn_lds, uint32_t grid, hipStream_t s);
 hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
                        int shape, hipStream_t s);
 int num_row_shapes();

@@ -484,9 +484,72 @@ __device__ __forceinline__ void emit_staged(const ScanArgs &a, uint32_t *lds, ui
     else stage_scatter(a.stage, lds, gid >> kPartBits, gid & (kPartSlots - 1u), m1, m2);
 }
 
+// one match of a one-pass staged scan: its record goes to its partition's
+// current chunk, or to the spare once the current one is full (a tile adds
+// at most one chunk of records to a partition whose fill is below a chunk;
+// a full chunk retires after the tile); with no chunk (the pool is used up)
+// it is added to the outputs directly
+__device__ __forceinline__ void emit_chunk(const ScanArgs &a, uint32_t *lds, uint32_t gid, int32_t m1, int32_t m2)
+{
+    if (gid >= a.G) {
+        atomic_add_u32_dev(a.err, 1u);
+        return;
+    }
+    const ChunkArgs &c = a.chunk;
+    const uint32_t np = c.nparts;
+    const uint32_t CH = 1u << c.ch_bits;
+    const uint32_t part = gid >> kPartBits;
+    const uint32_t glow = gid & (kPartSlots - 1u);
+    stage_l_u32 *pc_cur = (stage_l_u32 *)lds + a.pc_lds;
+    stage_l_u32 *pc_fill = pc_cur + 2u * np;
+    const uint32_t pos = __hip_atomic_fetch_add(pc_fill + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t id = pos < CH ? pc_cur[part] : pc_cur[np + part];
+    const uint32_t off = pos < CH ? pos : pos - CH;
+    if (id == kNoChunk) {
+        const unsigned long long o = c.part_out[part] + glow;
+        atomic_add_u64_dev(&a.gsum[o], (unsigned long long)((long long)m1 + (long long)m2));
+        atomic_add_u64_dev(&a.gcnt[o], 1ull);
+        atomic_add_u32_dev(c.overflow, 1u);
+        return;
+    }
+    const unsigned long long base = (unsigned long long)id << c.ch_bits;
+    if (c.fmt == REC_WIDE) {
+        ((uint4 *)c.pool)[base + off] = make_uint4(glow, (uint32_t)m1, (uint32_t)m2, 0u);
+    } else {
+        // a chunk holds CH u32 measures, then CH u16 slots
+        char *cp = (char *)c.pool + 6ull * base;
+        ((uint32_t *)cp)[off] = (uint32_t)m1;
+        ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
+    }
+}
+
+// one-pass staging, after a tile (all threads, between two barriers): full
+// chunks retire, the spare becomes the current chunk and a new spare is
+// claimed
+__device__ __forceinline__ void retire_chunks(const ScanArgs &a, uint32_t *lds)
+{
+    const ChunkArgs &c = a.chunk;
+    const uint32_t np = c.nparts;
+    const uint32_t CH = 1u << c.ch_bits;
+    stage_l_u32 *pc_cur = (stage_l_u32 *)lds + a.pc_lds;
+    stage_l_u32 *pc_spare = pc_cur + np;
+    stage_l_u32 *pc_fill = pc_spare + np;
+    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+        const uint32_t f = pc_fill[p];
+        if (f >= CH) {
+            const uint32_t cur = pc_cur[p];
+            if (cur != kNoChunk) c.chunk_fill[cur] = CH;
+            pc_cur[p] = pc_spare[p];
+            pc_fill[p] = f - CH;
+            pc_spare[p] = chunk_claim(c, p);
+        }
+    }
+}
+
 // MODE: AGG_DIRECT aggregates into the LDS table (GLDS) or with global
 // atomics; AGG_COUNT / AGG_SCATTER are the two scan passes of the staged
-// aggregation (the COUNT pass never reads the measures)
+// aggregation (the COUNT pass never reads the measures); AGG_CHUNK is the
+// one-pass staged aggregation (records into pool chunks)
 template <int NDIM, bool VEC, bool GLDS, int BS, int U, int MODE = AGG_DIRECT, bool EAGER = false>
 __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
 {
@@ -494,6 +557,15 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
     extern __shared__ __align__(16) uint32_t lds[];
     lds_init<NDIM, GLDS>(a, lds);
     stage_begin<MODE>(a.stage, lds);
+    if (MODE == AGG_CHUNK) {
+        stage_l_u32 *pc = (stage_l_u32 *)lds + a.pc_lds;
+        const uint32_t np = a.chunk.nparts;
+        for (uint32_t p = threadIdx.x; p < np; p += BS) {
+            pc[p] = chunk_claim(a.chunk, p);
+            pc[np + p] = chunk_claim(a.chunk, p);
+            pc[2u * np + p] = 0u;
+        }
+    }
     __syncthreads();
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
@@ -575,10 +647,26 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
                     if ((alive[u] >> k) & 1u) {
                         if (MODE == AGG_DIRECT)
                             aggregate<GLDS>(a, lds, gid[u][k], measure4(a, m1[u], m2[u], k));
+                        else if (MODE == AGG_CHUNK)
+                            emit_chunk(a, lds, gid[u][k], m1[u][k], a.m2 ? m2[u][k] : 0);
                         else
                             emit_staged<MODE>(a, lds, gid[u][k], m1[u][k], a.m2 ? m2[u][k] : 0);
                     }
             }
+        }
+        if (MODE == AGG_CHUNK) {
+            __syncthreads();
+            retire_chunks(a, lds);
+            __syncthreads();
+        }
+    }
+    if (MODE == AGG_CHUNK) {
+        // fills of the chunks still open (below a chunk after the last retire)
+        const uint32_t np = a.chunk.nparts;
+        const stage_l_u32 *pc = (const stage_l_u32 *)lds + a.pc_lds;
+        for (uint32_t p = tid; p < np; p += BS) {
+            if (pc[p] != kNoChunk) a.chunk.chunk_fill[pc[p]] = pc[2u * np + p];
+            if (pc[np + p] != kNoChunk) a.chunk.chunk_fill[pc[np + p]] = 0u;
         }
     }
     if (MODE == AGG_DIRECT) {
@@ -1096,6 +1184,48 @@ hipError_t rowwise_staged_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_
     const uint64_t tiles = (rows + (uint64_t)kStageBS * 4u * kStageU - 1) / ((uint64_t)kStageBS * 4u * kStageU);
     *grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)best * num_cus, tiles));
     return hipSuccess;
+}
+
+// one-pass staged form: tiles of kChunkBS threads x kChunkU chunks of 4
+// rows, one pool chunk (kChunkRows records) per tile and partition
+constexpr int kChunkBS = 1024;
+constexpr int kChunkU = 4;
+constexpr uint32_t kChunkRows = (uint32_t)kChunkBS * 4u * kChunkU;
+
+static const void *chunk_kernel(int ndims)
+{
+    switch (ndims) {
+    case 1: return (const void *)k_rowwise<1, true, false, kChunkBS, kChunkU, AGG_CHUNK>;
+    case 2: return (const void *)k_rowwise<2, true, false, kChunkBS, kChunkU, AGG_CHUNK>;
+    case 3: return (const void *)k_rowwise<3, true, false, kChunkBS, kChunkU, AGG_CHUNK>;
+    default: return (const void *)k_rowwise<4, true, false, kChunkBS, kChunkU, AGG_CHUNK>;
+    }
+}
+
+uint32_t rowwise_chunk_rows() { return kChunkRows; }
+
+hipError_t rowwise_chunk_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows, uint32_t *grid)
+{
+    const void *k = chunk_kernel(ndims);
+    if (dyn_lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kChunkBS, dyn_lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const uint64_t tiles = (rows + kChunkRows - 1) / kChunkRows;
+    *grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)per_cu * num_cus, tiles));
+    return hipSuccess;
+}
+
+hipError_t run_rowwise_chunk(ScanArgs &a, int ndims, uint32_t dyn_lds, uint32_t grid, hipStream_t s)
+{
+    a.ntiles = (a.rows + kChunkRows - 1) / kChunkRows;
+    if (a.ntiles == 0) return hipSuccess;
+    void *args[] = {&a};
+    return hipLaunchKernel(chunk_kernel(ndims), dim3(grid), dim3(kChunkBS), args, dyn_lds, s);
 }
 
 hipError_t run_rowwise_staged(ScanArgs &a, int ndims, uint32_t dyn_lds, int mode, uint32_t grid,

@@ -108,6 +108,19 @@ __device__ __forceinline__ void stage_overflow(const StageArgs &s, uint32_t p, u
     __hip_atomic_fetch_add(s.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// one-pass staging: claim a chunk of the pool for partition p; its id (first
+// record id >> ch_bits), or kNoChunk when the pool is used up
+__device__ __forceinline__ uint32_t chunk_claim(const ChunkArgs &c, uint32_t p)
+{
+    const unsigned long long ch = 1ull << c.ch_bits;
+    const unsigned long long nc =
+        __hip_atomic_fetch_add(c.cursor, ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nc + ch > c.cap) return kNoChunk;
+    const uint32_t id = (uint32_t)(nc >> c.ch_bits);
+    c.chunk_part[id] = p;
+    return id;
+}
+
 // write one record of partition p
 __device__ __forceinline__ void stage_scatter(const StageArgs &s, uint32_t *lds, uint32_t p,
                                               uint32_t gid_low, int32_t m1, int32_t m2)

@@ -480,8 +480,9 @@ class Query:
         self.ctx.check(self.ctx.lib.olap_query_set_shape(self.h, shape), "set_shape")
 
     def set_aggregation(self, layout: int):
-        """-1 automatic, 0 global atomics, 1 staged, 2 global atomics behind an
-        LDS cache of hot groups (group tables beyond LDS)."""
+        """-1 automatic, 0 global atomics, 1 staged (two passes), 2 global
+        atomics behind an LDS cache of hot groups, 3 one-pass staged (group
+        tables beyond LDS)."""
         self.ctx.check(self.ctx.lib.olap_query_set_aggregation(self.h, layout), "set_aggregation")
 
     def device_bytes(self) -> int:

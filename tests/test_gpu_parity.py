@@ -165,10 +165,11 @@ def test_wide_codes_and_global_group_table(ctx, ncpu):
                                                  (1_000_000, 90, 2_000_000, False, True),
                                                  (5_000, 100, 1_000_003, False, False)])
 def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two, skew):
-    """Group tables beyond LDS: global atomics, staged (count / scatter /
-    reduce), global atomics behind the LDS cache of hot groups and the
-    automatic choice all give the oracle's result, with uniform group codes
-    and with codes skewed onto a few hot groups."""
+    """Group tables beyond LDS: global atomics, two-pass staged (count /
+    scatter / reduce), global atomics behind the LDS cache of hot groups,
+    one-pass staged (records into pool chunks, reduce) and the automatic
+    choice all give the oracle's result, with uniform group codes and with
+    codes skewed onto a few hot groups."""
     import torch
 
     rng = np.random.default_rng(G + sel)
@@ -196,7 +197,7 @@ def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two, skew):
     q = ctx.query(plan, cols)
     osum = torch.empty(G, dtype=torch.int64, device="cuda")
     ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
-    for layout in (0, 1, 2, -1, 1, 2):
+    for layout in (0, 1, 2, 3, -1, 1, 2, 3):
         _say(f"  layout {layout}")
         q.set_aggregation(layout)
         q.run("rowwise", osum, ocnt)
