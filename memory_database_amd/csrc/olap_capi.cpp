@@ -832,7 +832,9 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
     case OLAP_VWM_SV:
     case OLAP_VWM_DV: {
         Layout L = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1);
-        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && q->agg_layout == 3)
+        // the one-pass form keeps the dimension vectors in LDS, also where
+        // the group table would have displaced them
+        if (strategy == OLAP_ROWWISE && vec && q->agg_layout == 3)
             return run_rowwise_onepass(q, sum_p, count, s);
         if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q))
             return run_rowwise_stage(q, sum_p, count, s);
@@ -939,9 +941,16 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
         cands.push_back({i, strategy == OLAP_ROWWISE && !table_in_lds ? 0 : -1});
     if (strategy == OLAP_ROWWISE && !table_in_lds)  // the same behind an LDS cache of hot groups
         for (int i = 0; i < olap_num_shapes(strategy); i++) cands.push_back({i, 2});
-    if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) {
-        cands.push_back({0, 1});
-        cands.push_back({0, 3});
+    if (strategy == OLAP_ROWWISE && !table_in_lds && columns_vec_ok(q)) cands.push_back({0, 1});
+    // one-pass staged, also where a group table in LDS pushed dimension
+    // vectors out to HBM (the one-pass form keeps them in LDS)
+    if (strategy == OLAP_ROWWISE && columns_vec_ok(q)) {
+        const Layout with_table = plan_layout(q, true, true, false, -1);
+        const Layout steps_only = plan_layout(q, true, false, false, -1);
+        bool displaced = false;
+        for (int st_i = 0; st_i < q->plan.dimvec_nums; st_i++)
+            displaced |= steps_only.st[st_i].in_lds && !with_table.st[st_i].in_lds;
+        if (!table_in_lds || displaced) cands.push_back({0, 3});
     }
     const int n = (int)cands.size();
     hipEvent_t e0, e1;
