@@ -143,14 +143,18 @@ olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *thre
                             int32_t *rows_per_thread);
 olap_status olap_query_set_shape(olap_query q, int32_t shape);
 /* Aggregation layout of row-wise runs whose group table exceeds LDS:
- * -1 automatic (from the expected matches), 0 one global atomic pair per
- * matching row, 1 staged (count, scatter records to HBM, reduce per
- * 8192-slot partition in LDS; synchronises once per run to size the record
- * buffer), 2 as 0 behind a per-workgroup LDS cache of up to 4096 groups,
- * which the first groups to reach a slot keep (skewed keys: the hot groups
- * stop contending on global atomics), 3 one-pass staged (one scan writes
- * every match's record into chunks of a record pool, per partition; one
- * reduce; no host synchronisation). The autotuner also tries all four. */
+ * -1 automatic (from the expected matches: many matches take the one-pass
+ * staged form, or the two-pass one where its open chunks would not fit);
+ * 0 one global atomic pair per matching row; 1 two-pass staged (count,
+ * scatter records to HBM, reduce per 8192-slot partition in LDS;
+ * synchronises once per run to size the record buffer); 2 as 0 behind a
+ * per-workgroup LDS cache of up to 4096 groups, which the first groups to
+ * reach a slot keep (skewed keys: the hot groups stop contending on global
+ * atomics); 3 one-pass staged (one scan writes every match's record into
+ * chunks of a record pool, per partition; one reduce; no host
+ * synchronisation; OLAP_E_CAPACITY where each workgroup's two open chunks
+ * per partition would exceed 1/32 of HBM). The autotuner also tries all
+ * four. */
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out);

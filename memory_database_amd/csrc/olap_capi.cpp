@@ -836,8 +836,15 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
         // the group table would have displaced them
         if (strategy == OLAP_ROWWISE && vec && q->agg_layout == 3)
             return run_rowwise_onepass(q, sum_p, count, s);
-        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q))
+        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q)) {
+            // automatic choice: one pass where its open chunks fit, else two
+            if (q->agg_layout < 0) {
+                st = run_rowwise_onepass(q, sum_p, count, s);
+                if (st != OLAP_E_CAPACITY) return st;
+                ctx->err.clear();
+            }
             return run_rowwise_stage(q, sum_p, count, s);
+        }
         ScanArgs a;
         fill_args(a, q, L, sum_p, count);
         uint32_t dyn = L.dyn_lds;
