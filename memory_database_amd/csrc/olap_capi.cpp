@@ -299,6 +299,8 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     if (nparts > kMaxParts) return fail(ctx, OLAP_E_CAPACITY, "too many partitions");
     const uint32_t cnt_bytes = (nparts * 4u + 15u) & ~15u;
     const uint32_t stage_bytes = cnt_bytes + nparts * 8u;
+    if (stage_bytes + 1024u > kLdsBytes)
+        return fail(ctx, OLAP_E_CAPACITY, "staged aggregation: %u partitions exceed LDS", nparts);
     Layout L = plan_layout(q, true, false, false, -1, stage_bytes);
     const uint32_t cur_lds = align_up(L.dyn_lds, 16u) / 4u;
     const uint32_t start_lds = cur_lds + cnt_bytes / 4u;
@@ -336,6 +338,8 @@ olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, h
     const uint32_t nparts = (G + kPartSlots - 1) / kPartSlots;
     if (nparts > kMaxParts) return fail(ctx, OLAP_E_CAPACITY, "too many partitions");
     const uint32_t pc_bytes = align_up(nparts * 12u, 16u);  // current, spare, fill per partition
+    if (pc_bytes + 1024u > kLdsBytes)
+        return fail(ctx, OLAP_E_CAPACITY, "one-pass staging: %u partitions exceed LDS", nparts);
     Layout L = plan_layout(q, true, false, false, -1, pc_bytes);
     const uint32_t pc_lds = align_up(L.dyn_lds, 16u) / 4u;
     const uint32_t dyn = pc_lds * 4u + pc_bytes;

@@ -211,6 +211,43 @@ def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two, skew):
     q.close()
 
 
+def test_staged_forms_refuse_partitions_beyond_lds(ctx, ncpu):
+    """2^27 groups make 16,384 partitions, whose per-partition state does not
+    fit LDS: both staged forms are refused with E_CAPACITY, and the automatic
+    choice (few expected matches: global atomics) still equals the oracle."""
+    import torch
+
+    G = 1 << 27
+    rows = 200_003
+    rng = np.random.default_rng(27)
+    _say(f"G=2^27 over {rows} rows")
+    D = 100_000
+    dv = po.build_dimvec(rng.integers(0, 100, D, dtype=np.int32), rng.integers(0, G, D, dtype=np.int32),
+                         0, 50, 1, 4)
+    dvs = [dv, None, None, None]
+    fks = [rng.integers(0, D, rows, dtype=np.int32), None, None, None]
+    m1 = rng.integers(1, 1000, rows, dtype=np.int32)
+    p = po.plan([0], [1], G)
+    es, ec, _ = po.run("rowwise", p, po.Star(dvs, fks, m1), nthreads=ncpu)
+    plan = oc.plan_from([0], [1], G)
+    cols = oc.Context.columns([_dev(d, torch) for d in dvs], [_dev(x, torch) for x in fks],
+                              _dev(m1, torch), None, rows=rows)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    for layout in (1, 3):
+        q.set_aggregation(layout)
+        with pytest.raises(oc.OlapError) as err:
+            q.run("rowwise", osum, ocnt)
+        assert err.value.status == oc.E_CAPACITY
+    q.set_aggregation(-1)
+    q.run("rowwise", osum, ocnt)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
+
+
 def test_large_dimensions_spill_to_hbm(ctx, ncpu):
     """Dimension vectors larger than LDS are probed from HBM."""
     rng = np.random.default_rng(11)
