@@ -944,8 +944,8 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
     hipStream_t s = as_stream(stream);
     // candidates: every launch shape; a row-wise run whose group table exceeds
     // LDS also tries every shape behind the LDS cache of hot groups, and the
-    // two-pass and one-pass staged aggregations (one fixed shape each; the
-    // one-pass form is skipped where its open chunks would not fit)
+    // two-pass and one-pass staged aggregations (one fixed shape each,
+    // skipped where their partition state or open chunks would not fit)
     std::vector<std::pair<int, int>> cands;  // (shape, aggregation layout)
     const bool table_in_lds = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1).gt_lds != 0;
     for (int i = 0; i < olap_num_shapes(strategy); i++)
@@ -976,7 +976,8 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
         q->row_shape = cands[(size_t)i].first;
         q->agg_layout = cands[(size_t)i].second;
         st = olap_query_run(q, strategy, sum, count, stream);  // warm
-        if (st == OLAP_E_CAPACITY && cands[(size_t)i].second == 3) {  // one-pass too large here
+        if (st == OLAP_E_CAPACITY && (cands[(size_t)i].second == 1 || cands[(size_t)i].second == 3)) {
+            // a staged form whose partition state or open chunks do not fit
             st = OLAP_OK;
             continue;
         }

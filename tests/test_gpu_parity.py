@@ -245,6 +245,11 @@ def test_staged_forms_refuse_partitions_beyond_lds(ctx, ncpu):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
     np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    # the autotuner skips the refused forms and leaves a complete run
+    shape, threads, rpt, ms = q.autotune("rowwise", osum, ocnt)
+    _say(f"  autotune -> {shape} ({ms:.3f} ms)")
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
     q.close()
 
 
