@@ -175,6 +175,20 @@ olap_status olap_cwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns 
 olap_status olap_cwm_sv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
                         int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
 
+/* olap_rowwise for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments, results and errors as
+ * olap_rowwise, but every piece of device scratch (dimension statistics, the
+ * error word, the total count and the packed dimension vectors) lives in a
+ * fixed per-GPU array of 8 MiB held until the process ends, and no device
+ * memory is allocated or freed. Packed dimension vectors (1..32 bits per
+ * code, rounded up per table to 16 bytes) beyond 8 MiB - 256 bytes are
+ * refused with OLAP_E_CAPACITY. Group tables beyond LDS take global atomics.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several host threads or streams) race. Synchronises the stream. */
+olap_status olap_rowwise_noalloc(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                                 int64_t *sum, int64_t *count, int64_t *total_count, void *stream);
+
 /* The column-wise pairs split as in the reference: join_cwm_sv (:908) writes
  * a per-row gid vector (rows int32, device, OLAP_GID_NULL for failing rows),
  * agg_cwm_sv (:764) aggregates it. join_cwm_dv (:840) leaves the surviving

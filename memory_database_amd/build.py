@@ -19,7 +19,8 @@ DRIVER_PATH = os.path.join(HERE, "olapcore_test")
 
 ARCH = os.environ.get("OLAP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["olap_kernels.hip", "olap_stage.hip", "olap_batch.hip", "olap_capi.cpp",
+SOURCES = ["olap_kernels.hip", "olap_stage.hip", "olap_batch.hip", "olap_noalloc.hip",
+           "olap_capi.cpp",
            "olapcore_compat.cpp",
            "olap_server.cpp"]
 

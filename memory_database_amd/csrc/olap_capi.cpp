@@ -370,6 +370,156 @@ olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, h
     return OLAP_OK;
 }
 
+// argument checks of a query (plan and columns), before any device work
+olap_status check_query_args(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols)
+{
+    const int nd = plan->dimvec_nums;
+    if (nd < 1 || nd > OLAP_MAX_DIMS) return fail(ctx, OLAP_E_INVALID, "dimvec_nums must be 1..4");
+    if (plan->group_nums < 1 || plan->group_nums > INT32_MAX)
+        return fail(ctx, OLAP_E_INVALID, "group_nums must be in [1, 2^31)");
+    if (cols->rows < 0 || cols->rows > (int64_t)UINT32_MAX - kTileRW)
+        return fail(ctx, OLAP_E_CAPACITY, "rows must be in [0, 2^32 - %d)", kTileRW);
+    bool seen[OLAP_MAX_DIMS] = {false, false, false, false};
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        if (t < 0 || t >= OLAP_MAX_DIMS || seen[t])
+            return fail(ctx, OLAP_E_INVALID, "orders must be distinct table slots 0..3");
+        seen[t] = true;
+        if (plan->factor[s] < 0) return fail(ctx, OLAP_E_INVALID, "factor must be >= 0");
+        if (cols->dim_rows[t] < 0 || cols->dim_rows[t] > INT32_MAX)
+            return fail(ctx, OLAP_E_CAPACITY, "dim_rows of table %d out of range", t);
+        if (cols->code_bytes[t] != 1 && cols->code_bytes[t] != 4)
+            return fail(ctx, OLAP_E_INVALID, "code_bytes of table %d must be 1 or 4", t);
+        if (cols->rows > 0 && !cols->fk[t])
+            return fail(ctx, OLAP_E_INVALID, "fk column of table %d is null", t);
+        if (cols->dim_rows[t] > 0 && !cols->dimvec[t])
+            return fail(ctx, OLAP_E_INVALID, "dimension vector of table %d is null", t);
+    }
+    return OLAP_OK;
+}
+
+// Validate the dimension vectors of q (max code, negative codes, non-NULL
+// codes per step), choose the probe order and pack the vectors. dstats:
+// 12 bytes per step of device scratch. packed: where the packed words go
+// (packed_cap words); null allocates them for q. Synchronises s.
+olap_status query_prepare(olap_query_s *q, int32_t *dstats, uint32_t *packed, size_t packed_cap,
+                          hipStream_t s0)
+{
+    olap_ctx ctx = q->ctx;
+    const olap_plan *plan = &q->plan;
+    const olap_columns *cols = &q->cols;
+    const int nd = plan->dimvec_nums;
+    // validation: max code, negative non-NULL codes and non-NULL codes per step
+    int32_t hstats[3 * OLAP_MAX_DIMS];
+    for (int s = 0; s < nd; s++) {
+        hstats[3 * s] = -1;
+        hstats[3 * s + 1] = 0;
+        hstats[3 * s + 2] = 0;
+    }
+    hipError_t e = hipMemcpyAsync(dstats, hstats, 12 * nd, hipMemcpyHostToDevice, s0);
+    for (int s = 0; s < nd && e == hipSuccess; s++) {
+        const int t = plan->orders[s];
+        e = launch_dim_stats(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t],
+                             dstats + 3 * s, s0);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hstats, dstats, 12 * nd, hipMemcpyDeviceToHost, s0);
+    if (e == hipSuccess) e = hipStreamSynchronize(s0);
+    if (e != hipSuccess) return hip_fail(ctx, e, "query_create: validation");
+    int64_t max_gid = 0;
+    uint32_t lw[OLAP_MAX_DIMS] = {0, 0, 0, 0};
+    uint32_t nwords[OLAP_MAX_DIMS] = {0, 0, 0, 0};
+    size_t total_words = 0;
+    double pass = (double)cols->rows;
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        if (hstats[3 * s + 1] != 0)
+            return fail(ctx, OLAP_E_INVALID, "table %d holds %d negative non-NULL codes", t,
+                        hstats[3 * s + 1]);
+        pass *= cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
+        const int64_t mx = hstats[3 * s];
+        if (mx > 0) max_gid += mx * (int64_t)plan->factor[s];
+        // smallest power-of-two width whose all-ones value exceeds every code
+        uint32_t l = 0;
+        while (l < 5 && ((1ull << (1u << l)) - 1ull) <= (uint64_t)(mx < 0 ? 0 : mx)) l++;
+        lw[s] = l;
+        const uint64_t per = 32u >> l;
+        nwords[s] = (uint32_t)(((uint64_t)cols->dim_rows[t] + per - 1) / per);
+        total_words += align_up(nwords[s], 4u);
+    }
+    if (max_gid >= plan->group_nums)
+        return fail(ctx, OLAP_E_INVALID,
+                    "largest reachable gid %lld is outside group_nums %lld (factor/codes mismatch)",
+                    (long long)max_gid, (long long)plan->group_nums);
+    q->max_gid = max_gid;
+    q->est_events = pass;
+    if (packed) {
+        if (total_words > packed_cap)
+            return fail(ctx, OLAP_E_CAPACITY,
+                        "packed dimension vectors take %zu bytes, beyond the %zu-byte scratch",
+                        total_words * 4, packed_cap * 4);
+        q->packed = packed;
+        q->owns_packed = false;
+    } else {
+        e = hipMalloc(&q->packed, std::max<size_t>(total_words, 4) * 4);
+        if (e != hipSuccess) {
+            q->packed = nullptr;
+            return hip_fail(ctx, e, "query_create: packed vectors");
+        }
+        q->owns_packed = true;
+    }
+    q->packed_words = total_words;
+    // Probe order. A row passes only if every step passes and its gid is the
+    // sum of code * factor over the steps, so the order changes no result; it
+    // decides how many later foreign-key lines the early exit still fetches
+    // and how many rows reach each probe. Independent filters are best taken
+    // by falling (1 - pass fraction) / cost: a step whose packed vector is
+    // larger than LDS stays in HBM and costs an L2 gather per row that
+    // reaches it (cost 4 against 1 for an LDS probe), so a selective step in
+    // LDS can go before a slightly more selective one in HBM. With every
+    // vector in LDS this is the most selective step first.
+    int perm[OLAP_MAX_DIMS];
+    double frac[OLAP_MAX_DIMS], rank[OLAP_MAX_DIMS];
+    for (int s = 0; s < nd; s++) {
+        const int t = plan->orders[s];
+        perm[s] = s;
+        frac[s] = cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
+        const bool hbm = (uint64_t)nwords[s] * 4u > (uint64_t)kLdsBytes;
+        rank[s] = (1.0 - frac[s]) / (hbm ? 4.0 : 1.0);
+    }
+    std::stable_sort(perm, perm + nd, [&](int x, int y) {
+        return rank[x] != rank[y] ? rank[x] > rank[y] : frac[x] < frac[y];
+    });
+    // step 1 is fetched with step 0 when most of its 128-byte lines (32 keys)
+    // hold a row that passed step 0: a streamed column then costs less than
+    // the dependent partial fetches
+    q->eager1 = nd > 1 && 1.0 - std::pow(1.0 - frac[perm[0]], 32.0) > 0.5 ? 1u : 0u;
+    size_t off = 0;
+    for (int s = 0; s < OLAP_MAX_DIMS; s++) memset(&q->steps[s], 0, sizeof(StepDesc));
+    for (int i = 0; i < nd && e == hipSuccess; i++) {
+        const int s = perm[i];
+        const int t = plan->orders[s];
+        StepDesc &st = q->steps[i];
+        st.fk = cols->fk[t];
+        st.words = q->packed + off;
+        st.dim_rows = (uint32_t)cols->dim_rows[t];
+        st.lw = lw[s];
+        st.mask = lw[s] == 5 ? 0xffffffffu : ((1u << (1u << lw[s])) - 1u);
+        st.factor = (uint32_t)plan->factor[s];
+        st.nwords = nwords[s];
+        e = launch_pack(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t], lw[s],
+                        q->packed + off, nwords[s], s0);
+        off += align_up(nwords[s], 4u);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s0);
+    if (e != hipSuccess) {
+        if (q->owns_packed) dev_free(q->packed);
+        q->packed = nullptr;
+        return hip_fail(ctx, e, "query_create: pack");
+    }
+    return OLAP_OK;
+}
+
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -495,7 +645,8 @@ olap_status olap_ctx_create(int device, olap_ctx *out)
         delete ctx;
         return OLAP_E_NOMEM;
     }
-    if (launch_zero_u32(ctx->d_err, 4, nullptr) != hipSuccess ||
+    if (noalloc_scratch_init(device, &ctx->noalloc) != hipSuccess ||
+        launch_zero_u32(ctx->d_err, 4, nullptr) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         dev_free(ctx->d_err);
         dev_free(ctx->d_scratch);
@@ -674,139 +825,17 @@ olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_co
     if (!ctx) return OLAP_E_INVALID;
     if (!plan || !cols || !out) return fail(ctx, OLAP_E_INVALID, "query_create: null argument");
     *out = nullptr;
-    const int nd = plan->dimvec_nums;
-    if (nd < 1 || nd > OLAP_MAX_DIMS) return fail(ctx, OLAP_E_INVALID, "dimvec_nums must be 1..4");
-    if (plan->group_nums < 1 || plan->group_nums > INT32_MAX)
-        return fail(ctx, OLAP_E_INVALID, "group_nums must be in [1, 2^31)");
-    if (cols->rows < 0 || cols->rows > (int64_t)UINT32_MAX - kTileRW)
-        return fail(ctx, OLAP_E_CAPACITY, "rows must be in [0, 2^32 - %d)", kTileRW);
-    bool seen[OLAP_MAX_DIMS] = {false, false, false, false};
-    for (int s = 0; s < nd; s++) {
-        const int t = plan->orders[s];
-        if (t < 0 || t >= OLAP_MAX_DIMS || seen[t])
-            return fail(ctx, OLAP_E_INVALID, "orders must be distinct table slots 0..3");
-        seen[t] = true;
-        if (plan->factor[s] < 0) return fail(ctx, OLAP_E_INVALID, "factor must be >= 0");
-        if (cols->dim_rows[t] < 0 || cols->dim_rows[t] > INT32_MAX)
-            return fail(ctx, OLAP_E_CAPACITY, "dim_rows of table %d out of range", t);
-        if (cols->code_bytes[t] != 1 && cols->code_bytes[t] != 4)
-            return fail(ctx, OLAP_E_INVALID, "code_bytes of table %d must be 1 or 4", t);
-        if (cols->rows > 0 && !cols->fk[t])
-            return fail(ctx, OLAP_E_INVALID, "fk column of table %d is null", t);
-        if (cols->dim_rows[t] > 0 && !cols->dimvec[t])
-            return fail(ctx, OLAP_E_INVALID, "dimension vector of table %d is null", t);
-    }
+    olap_status st = check_query_args(ctx, plan, cols);
+    if (st != OLAP_OK) return st;
     DeviceGuard g(ctx->device);
     olap_query q = new olap_query_s();
     q->ctx = ctx;
     q->plan = *plan;
     q->cols = *cols;
-
-    // validation: max code, negative non-NULL codes and non-NULL codes per step
-    std::vector<int32_t> hstats(3 * nd);
-    for (int s = 0; s < nd; s++) {
-        hstats[3 * s] = -1;
-        hstats[3 * s + 1] = 0;
-        hstats[3 * s + 2] = 0;
-    }
-    int32_t *dstats = (int32_t *)ctx->d_scratch;
-    hipStream_t s0 = nullptr;
-    hipError_t e = hipMemcpyAsync(dstats, hstats.data(), 12 * nd, hipMemcpyHostToDevice, s0);
-    for (int s = 0; s < nd && e == hipSuccess; s++) {
-        const int t = plan->orders[s];
-        e = launch_dim_stats(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t],
-                             dstats + 3 * s, s0);
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(hstats.data(), dstats, 12 * nd, hipMemcpyDeviceToHost, s0);
-    if (e == hipSuccess) e = hipStreamSynchronize(s0);
-    if (e != hipSuccess) {
+    st = query_prepare(q, (int32_t *)ctx->d_scratch, nullptr, 0, nullptr);
+    if (st != OLAP_OK) {
         delete q;
-        return hip_fail(ctx, e, "query_create: validation");
-    }
-    int64_t max_gid = 0;
-    uint32_t lw[OLAP_MAX_DIMS] = {0, 0, 0, 0};
-    uint32_t nwords[OLAP_MAX_DIMS] = {0, 0, 0, 0};
-    size_t total_words = 0;
-    double pass = (double)cols->rows;
-    for (int s = 0; s < nd; s++) {
-        const int t = plan->orders[s];
-        if (hstats[3 * s + 1] != 0) {
-            delete q;
-            return fail(ctx, OLAP_E_INVALID, "table %d holds %d negative non-NULL codes", t,
-                        hstats[3 * s + 1]);
-        }
-        pass *= cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
-        const int64_t mx = hstats[3 * s];
-        if (mx > 0) max_gid += mx * (int64_t)plan->factor[s];
-        // smallest power-of-two width whose all-ones value exceeds every code
-        uint32_t l = 0;
-        while (l < 5 && ((1ull << (1u << l)) - 1ull) <= (uint64_t)(mx < 0 ? 0 : mx)) l++;
-        lw[s] = l;
-        const uint64_t per = 32u >> l;
-        nwords[s] = (uint32_t)(((uint64_t)cols->dim_rows[t] + per - 1) / per);
-        total_words += align_up(nwords[s], 4u);
-    }
-    if (max_gid >= plan->group_nums) {
-        delete q;
-        return fail(ctx, OLAP_E_INVALID,
-                    "largest reachable gid %lld is outside group_nums %lld (factor/codes mismatch)",
-                    (long long)max_gid, (long long)plan->group_nums);
-    }
-    q->max_gid = max_gid;
-    q->est_events = pass;
-    e = hipMalloc(&q->packed, std::max<size_t>(total_words, 4) * 4);
-    if (e != hipSuccess) {
-        delete q;
-        return hip_fail(ctx, e, "query_create: packed vectors");
-    }
-    q->packed_words = total_words;
-    // Probe order. A row passes only if every step passes and its gid is the
-    // sum of code * factor over the steps, so the order changes no result; it
-    // decides how many later foreign-key lines the early exit still fetches
-    // and how many rows reach each probe. Independent filters are best taken
-    // by falling (1 - pass fraction) / cost: a step whose packed vector is
-    // larger than LDS stays in HBM and costs an L2 gather per row that
-    // reaches it (cost 4 against 1 for an LDS probe), so a selective step in
-    // LDS can go before a slightly more selective one in HBM. With every
-    // vector in LDS this is the most selective step first.
-    int perm[OLAP_MAX_DIMS];
-    double frac[OLAP_MAX_DIMS], rank[OLAP_MAX_DIMS];
-    for (int s = 0; s < nd; s++) {
-        const int t = plan->orders[s];
-        perm[s] = s;
-        frac[s] = cols->dim_rows[t] > 0 ? (double)hstats[3 * s + 2] / (double)cols->dim_rows[t] : 0.0;
-        const bool hbm = (uint64_t)nwords[s] * 4u > (uint64_t)kLdsBytes;
-        rank[s] = (1.0 - frac[s]) / (hbm ? 4.0 : 1.0);
-    }
-    std::stable_sort(perm, perm + nd, [&](int x, int y) {
-        return rank[x] != rank[y] ? rank[x] > rank[y] : frac[x] < frac[y];
-    });
-    // step 1 is fetched with step 0 when most of its 128-byte lines (32 keys)
-    // hold a row that passed step 0: a streamed column then costs less than
-    // the dependent partial fetches
-    q->eager1 = nd > 1 && 1.0 - std::pow(1.0 - frac[perm[0]], 32.0) > 0.5 ? 1u : 0u;
-    size_t off = 0;
-    for (int s = 0; s < OLAP_MAX_DIMS; s++) memset(&q->steps[s], 0, sizeof(StepDesc));
-    for (int i = 0; i < nd && e == hipSuccess; i++) {
-        const int s = perm[i];
-        const int t = plan->orders[s];
-        StepDesc &st = q->steps[i];
-        st.fk = cols->fk[t];
-        st.words = q->packed + off;
-        st.dim_rows = (uint32_t)cols->dim_rows[t];
-        st.lw = lw[s];
-        st.mask = lw[s] == 5 ? 0xffffffffu : ((1u << (1u << lw[s])) - 1u);
-        st.factor = (uint32_t)plan->factor[s];
-        st.nwords = nwords[s];
-        e = launch_pack(cols->dimvec[t], cols->code_bytes[t], cols->dim_rows[t], lw[s],
-                        q->packed + off, nwords[s], s0);
-        off += align_up(nwords[s], 4u);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s0);
-    if (e != hipSuccess) {
-        dev_free(q->packed);
-        delete q;
-        return hip_fail(ctx, e, "query_create: pack");
+        return st;
     }
     *out = q;
     return OLAP_OK;
@@ -893,7 +922,7 @@ olap_status olap_query_destroy(olap_query q)
 {
     if (!q) return OLAP_E_INVALID;
     DeviceGuard g(q->ctx->device);
-    if (q->packed) dev_free(q->packed);
+    if (q->packed && q->owns_packed) dev_free(q->packed);
     if (q->list_oid) dev_free(q->list_oid);
     if (q->list_gid) dev_free(q->list_gid);
     if (q->list_cnt) dev_free(q->list_cnt);
@@ -1036,6 +1065,61 @@ olap_status olap_rowwise(olap_ctx ctx, const olap_plan *plan, const olap_columns
 {
     return one_shot(ctx, OLAP_ROWWISE, plan, cols, sum, count, total_count, stream);
 }
+// olap_rowwise with every piece of device scratch in the GPU's
+// allocation-free array: the statistics, the error word, the total and the
+// packed dimension vectors. Group tables beyond LDS take global atomics
+// (never a staged record buffer). Calls that overlap on one GPU share the
+// array and race.
+olap_status olap_rowwise_noalloc(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
+                                 int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (!plan || !cols) return fail(ctx, OLAP_E_INVALID, "rowwise_noalloc: null argument");
+    olap_status st = check_query_args(ctx, plan, cols);
+    if (st != OLAP_OK) return st;
+    if (!count || (!sum && cols->m1)) return fail(ctx, OLAP_E_INVALID, "rowwise_noalloc: null output");
+    if (cols->rows > 0 && !cols->m1 && sum)
+        return fail(ctx, OLAP_E_INVALID, "rowwise_noalloc: sum requested without a measure");
+    if (!ctx->noalloc) return fail(ctx, OLAP_E_HIP, "rowwise_noalloc: no scratch on this device");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    unsigned char *base = (unsigned char *)ctx->noalloc;
+    int32_t *dstats = (int32_t *)base;                                // 12 bytes per step
+    unsigned long long *dtot = (unsigned long long *)(base + 64);    // total count
+    uint32_t *derr = (uint32_t *)(base + 128);                        // error word
+    uint32_t *packed = (uint32_t *)(base + kNoallocHeader);
+    olap_query_s q;
+    q.ctx = ctx;
+    q.plan = *plan;
+    q.cols = *cols;
+    q.agg_layout = 0;
+    st = query_prepare(&q, dstats, packed, (kNoallocScratchBytes - kNoallocHeader) / 4, s);
+    if (st != OLAP_OK) return st;
+    int64_t *sum_p = sum ? sum : count;
+    if (sum) st = zero_outputs(&q, sum, count, s);
+    else HIPC(ctx, launch_zero_u64((unsigned long long *)count, (size_t)plan->group_nums, s));
+    if (st != OLAP_OK) return st;
+    HIPC(ctx, launch_zero_u32(derr, 1, s));
+    HIPC(ctx, launch_zero_u64(dtot, 1, s));
+    Layout L = plan_layout(&q, true, true, false, -1);
+    ScanArgs a;
+    fill_args(a, &q, L, sum_p, count);
+    a.err = derr;
+    hipError_t e = run_rowwise(a, plan->dimvec_nums, ctx->num_cus, L.dyn_lds, columns_vec_ok(&q),
+                               effective_shape(&q), s);
+    if (e == hipSuccess)
+        e = launch_sum_u64((const unsigned long long *)count, plan->group_nums, dtot, ctx->num_cus, s);
+    uint32_t herr = 0;
+    unsigned long long htot = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&htot, dtot, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, derr, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rowwise_noalloc");
+    if (herr) return fail(ctx, OLAP_E_INVALID, "%u rows produced a gid outside group_nums", herr);
+    if (total_count) *total_count = (int64_t)htot;
+    return OLAP_OK;
+}
+
 olap_status olap_vwm_dv(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,
                         int64_t *sum, int64_t *count, int64_t *total_count, void *stream)
 {

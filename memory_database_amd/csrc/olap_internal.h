@@ -21,6 +21,11 @@ constexpr int kPerThreadV = kTileV / kBlock;        // list entries per thread (
 constexpr uint32_t kLdsBytes = 160 * 1024;          // LDS per CU on gfx950
 constexpr uint32_t kHotEmpty = 0xffffffffu;         // an unclaimed slot of the LDS cache of hot groups
 constexpr uint32_t kLdsListBytes = kTileV * 2 + kTileV * 4 + 16;  // u16 off + u32 gid + counter
+// device scratch of the allocation-free row-wise entry point, per GPU:
+// statistics, error word and total in the first kNoallocHeader bytes, the
+// packed dimension vectors after them
+constexpr size_t kNoallocScratchBytes = size_t(8) << 20;
+constexpr size_t kNoallocHeader = 256;
 
 struct ScanShape {
     int bs;      // threads per workgroup
@@ -94,6 +99,8 @@ struct olap_ctx_s {
     uint32_t *d_err = nullptr;     // error counter (4 words)
     void *d_scratch = nullptr;     // small reduction scratch
     size_t scratch_bytes = 0;
+    // device address of this GPU's allocation-free scratch (olap_noalloc.hip)
+    void *noalloc = nullptr;
 };
 
 struct olap_query_s {
@@ -101,6 +108,7 @@ struct olap_query_s {
     olap_plan plan{};
     olap_columns cols{};
     uint32_t *packed = nullptr;        // all steps' packed words
+    bool owns_packed = false;          // packed was allocated for this query
     size_t packed_words = 0;
     olap::StepDesc steps[OLAP_MAX_DIMS]{};
     int64_t max_gid = 0;
@@ -132,6 +140,10 @@ olap_status agg_gid_vector(olap_ctx ctx, const int32_t *gvec, int64_t rows, cons
 olap_status agg_row_list(olap_ctx ctx, const uint32_t *oid, const int32_t *gid, int64_t n,
                          const int32_t *m1, const int32_t *m2, int64_t G, int64_t *sum,
                          int64_t *count, hipStream_t s);
+
+// device address of the allocation-free scratch on `device`, which must be
+// current; taken once per device (olap_noalloc.hip)
+hipError_t noalloc_scratch_init(int device, void **ptr);
 
 // launchers, olap_kernels.hip
 hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s);

@@ -154,6 +154,8 @@ SIGNATURES = {
     "olap_vwm_sv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
     "olap_cwm_dv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
     "olap_cwm_sv": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P, _P]),
+    "olap_rowwise_noalloc": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), _P, _P, _I64P,
+                                       _P]),
     "olap_join_cwm_sv": (C.c_int, [_P, _P, _P]),
     "olap_agg_cwm_sv": (C.c_int, [_P, _P, _P, _P, _P]),
     "olap_join_cwm_dv": (C.c_int, [_P, _P]),
@@ -398,7 +400,8 @@ class Context:
         return Batch(self, star, queries)
 
     def run(self, strategy: str, plan: Plan, cols: Columns, out_sum, out_cnt, stream=None):
-        """One-shot reference-named entry point; returns the total count."""
+        """One-shot reference-named entry point (or "rowwise_noalloc", the
+        allocation-free row-wise form); returns the total count."""
         fn = getattr(self.lib, "olap_" + strategy)
         tot = C.c_int64()
         self.check(fn(self.h, C.byref(plan), C.byref(cols), _ptr(out_sum), _ptr(out_cnt),
