@@ -121,20 +121,53 @@ def cpu_baseline(ds, sample_rows, log_fn=log):
                       f"median of 5 runs after 1 warmup, {cores} pinned threads"}
 
 
-def profiled_traffic(kernel: str, workload: str):
+def source_digest() -> str:
+    """First 16 hex digits of the SHA-256 of the HIP kernel sources; a
+    profile summary records the digest of the sources it was taken on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "memory_database_amd", "csrc")
+    for f in sorted(os.listdir(csrc)):
+        if f.endswith((".hip", ".h")):
+            h.update(open(os.path.join(csrc, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def profiled_traffic(kernel: str, workload: str, key: str = "HBM traffic"):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed
-    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), the
+    summary's path, and whether it was taken on the current kernel sources;
+    (None, None, None) without a summary."""
     import re
 
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
     try:
         text = open(path).read()
     except OSError:
-        return None, None
-    m = re.search(r"HBM traffic ([0-9.]+) GB/dispatch", text)
+        return None, None, None
+    m = re.search(re.escape(key) + r" ([0-9.]+) GB/dispatch", text)
     if not m:
-        return None, None
-    return float(m.group(1)) * 1e9, os.path.relpath(path, ROOT)
+        return None, None, None
+    d = re.search(r"kernel sources digest ([0-9a-f]{16})", text)
+    return float(m.group(1)) * 1e9, os.path.relpath(path, ROOT), bool(d and d.group(1) == source_digest())
+
+
+def roofline(alg_bytes: float, ms: float, traffic, source, current):
+    """Roofline of one launch: achieved = HBM bytes moved (the profile's
+    counters) / the HIP-event kernel time, as a fraction of the HBM peak;
+    the algorithmic bytes over the same time are reported beside it."""
+    moved = traffic if traffic else None
+    gbps = (moved if moved else alg_bytes) / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbps / HBM_PEAK_GBS, "traffic": moved,
+            "basis": "HBM bytes moved per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) / "
+                     "HIP-event kernel time" if moved else
+                     "algorithmic bytes / HIP-event kernel time (no counter profile)",
+            "traffic_source": source, "traffic_on_current_sources": current,
+            "algorithmic_bytes": alg_bytes, "kernel_ms": ms,
+            "effective_GBps": alg_bytes / (ms / 1e3) / 1e9,
+            "effective_frac": alg_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
 def bench_sweep(ctx, stream, rows, strategy="rowwise"):
@@ -160,16 +193,16 @@ def bench_sweep(ctx, stream, rows, strategy="rowwise"):
         q = ctx.query(ds.plan, ds.cols)
         osum = torch.empty(G, dtype=torch.int64, device="cuda")
         ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
-        shape = q.autotune(strategy, osum, ocnt, stream=sh)
+        q.autotune(strategy, osum, ocnt, stream=sh)
         _, ev = time_steps(lambda: q.run(strategy, osum, ocnt, stream=sh), 10, 2, stream)
         ms = statistics.mean(ev)
         alg = ds.fact_bytes()
+        th, rp, agg = q.launch_info()
         row = {"theta": cfg.zipf_theta, "groups": G, "sele_pct": cfg.dims[0].pred_hi,
                "ms": round(ms, 4), "rows_per_s": rows / (ms / 1e3),
-               "GBps": round(alg / (ms / 1e3) / 1e9, 1),
-               "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 3),
+               "effective_GBps": round(alg / (ms / 1e3) / 1e9, 1),
                "passing_rows": int(ocnt.sum().item()),
-               "launch": [shape[1], shape[2]]}
+               "launch": {"threads": th, "rows_per_thread": rp, "aggregation": agg}}
         log(f"sweep {cfg.name}: {row}")
         out.append(row)
         q.close()
@@ -199,8 +232,10 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
     b.capture(osum, ocnt, sh)
     _, ev = time_steps(lambda: b.launch(sh), 5, 1, stream)
     batch_ms = statistics.mean(ev)
+    # every query's slots of the last timed replay must equal its plain scan
+    bsum, bcnt = osum.clone(), ocnt.clone()
     plain_ms = 0.0
-    for steps, factor, G in qs:
+    for i, (steps, factor, G) in enumerate(qs):
         plan, cols = bs.plain_query(steps, factor, G, stream=sh)
         q = ctx.query(plan, cols)
         ps = torch.empty(G, dtype=torch.int64, device="cuda")
@@ -208,6 +243,9 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
         _, e = time_steps(lambda: q.run("rowwise", ps, pc, stream=sh), 2, 1, stream)
         plain_ms += statistics.mean(e)
         q.close()
+        lo, hi = int(b.offsets[i]), int(b.offsets[i + 1])
+        if not (torch.equal(bsum[lo:hi], ps) and torch.equal(bcnt[lo:hi], pc)):
+            raise RuntimeError(f"batched query {i} differs from its plain scan")
     tables = set(t for steps, _, _ in qs for (t, _, _, _) in steps)
     rows = ds.cfg.rows
     union_bytes = rows * 4 * (len(tables) + 1)
@@ -215,8 +253,13 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
     events = b.expected_events()
     b.close()
     log(f"batched: {nq} queries {batch_ms:.2f} ms (config {cfg}: {threads} threads, {rpt} rows/thread, "
-        f"{kib} KiB LDS tables, staged={staged}); plain {plain_ms:.2f} ms; x{plain_ms / batch_ms:.2f}")
+        f"{kib} KiB LDS tables, staged={staged}); plain {plain_ms:.2f} ms; x{plain_ms / batch_ms:.2f}; "
+        f"all {nq} queries equal their plain scans")
+    traffic, src, current = profiled_traffic("k_batch_ev", f"{nq}q_{ds.cfg.name}",
+                                             key="batch HBM traffic")
     return {
+        "verified": f"every slot of the {nq} queries equals its plain row-wise scan",
+        "roofline": roofline(union_bytes, batch_ms, traffic, src, current),
         "queries": nq, "seed": seed, "batch_ms": batch_ms, "plain_ms_total": plain_ms,
         "qps_batched": nq / (batch_ms / 1e3), "qps_plain": nq / (plain_ms / 1e3),
         "speedup": plain_ms / batch_ms,
@@ -258,10 +301,14 @@ def bench_served(ctx, ds, rates, seconds, streams=2):
         srv.wait(t)
     serving.run_open_loop(srv, ids, qs, serving.arrivals(200.0, 60, len(ids), len(qs), seed=99))
     out = []
+    samples = []  # (dataset, template, sum, count) of every 97th arrival
     for r in rates:
         n = int(max(100, min(r * seconds, 6000)))
         srv.reset_stats()
-        res = serving.run_open_loop(srv, ids, qs, serving.arrivals(r, n, len(ids), len(qs), seed=1))
+        plan = serving.arrivals(r, n, len(ids), len(qs), seed=1)
+        res = serving.run_open_loop(srv, ids, qs, plan, keep_results=lambda i: i % 97 == 0)
+        for i, su, co in res.pop("results"):
+            samples.append((plan[i].dataset, plan[i].template, su, co))
         st = srv.stats()
         row = {"offered_qps": round(res["offered_qps"], 1), "requests": n, "qps": round(res["qps"], 1),
                "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
@@ -270,8 +317,23 @@ def bench_served(ctx, ds, rates, seconds, streams=2):
         log(f"served at {r:g} q/s: {row}")
         out.append(row)
     srv.close()
+    # the sampled results against the same requests served alone
+    import numpy as np
+
+    alone = ctx.server(max_batch=1, streams=1, memory_budget=32 << 30)
+    aids = [alone.add_dataset(s.star) for s in stars]
+    per_ds = [0] * len(stars)
+    for d, t, su, co in samples:
+        s1, c1 = alone.wait(alone.submit(aids[d], qs[t]))
+        if not (np.array_equal(su, s1) and np.array_equal(co, c1)):
+            raise RuntimeError(f"served result of template {t} on dataset {d} differs from alone")
+        per_ds[d] += 1
+    alone.close()
+    log(f"served: {len(samples)} sampled results equal the same requests served alone {per_ds}")
     return {"datasets_rows": [s.cfg.rows for s in stars], "templates": len(qs), "workers": streams,
             "max_batch": 64, "rates": out,
+            "verified": f"{len(samples)} sampled results (every 97th arrival, per dataset {per_ds}) "
+                        "equal the same requests served alone",
             "latency": "scheduled arrival to result on the host (submit lag + server time)"}
 
 
@@ -397,6 +459,8 @@ def main():
 
     wall_ms, ev = time_steps(step, args.steps, args.warmup, stream)
     ms = statistics.mean(ev)
+    th, rp, agg = q.launch_info()
+    launch = {"threads": th, "rows_per_thread": rp, "aggregation": agg}
     rows_per_s = cfg.rows / (ms / 1e3)
     alg_bytes = ds.fact_bytes()
     achieved = alg_bytes / (ms / 1e3) / 1e9
@@ -439,7 +503,8 @@ def main():
         cpu = cpu_baseline(ds, args.cpu_rows)
     q.close()
     kernel = "k_rowwise" if args.strategy == "rowwise" else None
-    traffic, traffic_src = profiled_traffic(kernel, cfg.name) if kernel else (None, None)
+    traffic, traffic_src, current = (profiled_traffic(kernel, cfg.name) if kernel
+                                     else (None, None, None))
 
     rec = {
         "metric": METRIC,
@@ -458,16 +523,12 @@ def main():
                    "dims": [{"slot": d.slot, "rows": d.rows, "sele": d.sele, "groups": d.groups}
                             for d in cfg.dims],
                    "measure": [cfg.m_lo, cfg.m_hi], "group_nums": G, "strategy": args.strategy,
-                   "launch": {"threads": shape[1], "rows_per_thread": shape[2]},
+                   "launch": launch,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_src,
-                     "traffic_GBps": traffic / (ms / 1e3) / 1e9 if traffic else None,
-                     "algorithmic_bytes": alg_bytes, "kernel_ms": ms,
-                     "note": "algorithmic bytes count every FK and measure byte; the scan's early "
-                             "exit loads later columns only for live rows, so HBM traffic is "
-                             "below them and frac can exceed 1"},
+        "roofline": dict(roofline(alg_bytes, ms, traffic, traffic_src, current),
+                         note="algorithmic bytes count every FK and measure byte; the scan's early "
+                              "exit loads later columns only for cache lines holding a live row, "
+                              "so the bytes moved are below them"),
         "cpu_baseline": cpu,
         "passing_rows": passing,
         "wall_ms_per_step": wall_ms / args.steps,

@@ -153,11 +153,26 @@ olap_status olap_query_set_shape(olap_query q, int32_t shape);
  * atomics); 3 one-pass staged (one scan writes every match's record into
  * chunks of a record pool, per partition; one reduce; no host
  * synchronisation; OLAP_E_CAPACITY where each workgroup's two open chunks
- * per partition would exceed 1/32 of HBM). The autotuner also tries all
- * four. */
+ * per partition would exceed 1/32 of HBM). Layouts 1 and 3 keep per-
+ * partition state (8,192 groups a partition) in LDS, so both refuse group
+ * tables above about 1.1e8 groups with OLAP_E_CAPACITY; the automatic
+ * choice then takes global atomics (layout 0). The autotuner tries all four
+ * and caches the winning shape and layout for queries of the same class,
+ * whose automatic runs then use them (falling back to the automatic choice
+ * where the cached staged layout does not fit). */
 olap_status olap_query_set_aggregation(olap_query q, int32_t layout);
 olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *sum, int64_t *count,
                                 void *stream, int32_t *shape_out, double *ms_out);
+/* The launch of the query's last run (e.g. the autotuner's winner): threads
+ * per workgroup, rows per thread and the aggregation used. OLAP_E_INVALID
+ * before the first run. */
+#define OLAP_AGG_ATOMICS 0   /* group table beyond LDS: global atomics */
+#define OLAP_AGG_TWOPASS 1   /* two-pass staged */
+#define OLAP_AGG_HOT 2       /* global atomics behind the LDS cache of hot groups */
+#define OLAP_AGG_ONEPASS 3   /* one-pass staged */
+#define OLAP_AGG_LDS 4       /* group table in LDS */
+olap_status olap_query_launch_info(olap_query q, int32_t *threads, int32_t *rows_per_thread,
+                                   int32_t *aggregation);
 /* bytes of device memory the query holds (packed vectors, scratch) */
 int64_t olap_query_device_bytes(olap_query q);
 

@@ -273,20 +273,35 @@ uint64_t query_class(const olap_query_s *q, olap_strategy strat)
            ((uint64_t)L.gt_lds << 31) | rows_log2;
 }
 
+const olap_ctx_s::Tuned *tuned_for(const olap_query_s *q)
+{
+    const uint64_t key = query_class(q, OLAP_ROWWISE);
+    for (const auto &t : q->ctx->tuned)
+        if (t.key == key) return &t;
+    return nullptr;
+}
+
 int effective_shape(const olap_query_s *q)
 {
     if (q->row_shape >= 0) return q->row_shape;
-    const uint64_t key = query_class(q, OLAP_ROWWISE);
-    for (const auto &kv : q->ctx->tuned)
-        if (kv.first == key) return kv.second;
-    return 0;
+    const olap_ctx_s::Tuned *t = tuned_for(q);
+    return t ? t->shape : 0;
+}
+
+// aggregation layout of a row-wise run whose group table exceeds LDS: the
+// query's own setting, else the autotuner's choice for the query's class,
+// else -1 (automatic)
+int effective_layout(const olap_query_s *q)
+{
+    if (q->agg_layout >= 0) return q->agg_layout;
+    const olap_ctx_s::Tuned *t = tuned_for(q);
+    return t ? t->layout : -1;
 }
 
 // Group table beyond LDS: staged aggregation when the expected matches make
 // global atomics (~24 G/s) cost more than a second scan of the key columns.
-bool use_staged(const olap_query_s *q)
+bool auto_staged(const olap_query_s *q)
 {
-    if (q->agg_layout >= 0) return q->agg_layout == 1;
     return q->est_events > 2.0e6 && q->est_events > 0.03 * (double)q->cols.rows;
 }
 
@@ -324,6 +339,10 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     a.stage = S.args(cur_lds, start_lds, (unsigned long long *)sum, (unsigned long long *)count);
     HIPC(ctx, run_rowwise_staged(a, nd, dyn, AGG_SCATTER, grid, s));
     HIPC(ctx, S.reduce((unsigned long long *)sum, (unsigned long long *)count, ctx->num_cus, s));
+    const ScanShape sh = staged_shape();
+    q->ran_threads = sh.bs;
+    q->ran_rows = sh.unroll * kChunk;
+    q->ran_agg = OLAP_AGG_TWOPASS;
     return OLAP_OK;
 }
 
@@ -367,6 +386,10 @@ olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, h
     a.pc_lds = pc_lds;
     HIPC(ctx, run_rowwise_chunk(a, nd, dyn, grid, s));
     HIPC(ctx, S.chunk_reduce((unsigned long long *)sum, (unsigned long long *)count, ctx->num_cus, s));
+    const ScanShape sh = chunk_shape();
+    q->ran_threads = sh.bs;
+    q->ran_rows = sh.unroll * kChunk;
+    q->ran_agg = OLAP_AGG_ONEPASS;
     return OLAP_OK;
 }
 
@@ -865,23 +888,34 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
     case OLAP_VWM_SV:
     case OLAP_VWM_DV: {
         Layout L = plan_layout(q, true, true, strategy == OLAP_VWM_DV, -1);
+        int layout = strategy == OLAP_ROWWISE ? effective_layout(q) : 0;
+        const bool own = q->agg_layout >= 0;  // set on the query, not taken from the cache
         // the one-pass form keeps the dimension vectors in LDS, also where
         // the group table would have displaced them
-        if (strategy == OLAP_ROWWISE && vec && q->agg_layout == 3)
-            return run_rowwise_onepass(q, sum_p, count, s);
-        if (strategy == OLAP_ROWWISE && !L.gt_lds && vec && use_staged(q)) {
-            // automatic choice: one pass where its open chunks fit, else two
-            if (q->agg_layout < 0) {
+        if (layout == 3 && vec) {
+            st = run_rowwise_onepass(q, sum_p, count, s);
+            if (st != OLAP_E_CAPACITY || own) return st;
+            ctx->err.clear();  // a cached choice that does not fit this query
+            layout = -1;
+        }
+        if (layout == 1 && !own && (L.gt_lds || !vec)) layout = -1;
+        if (!L.gt_lds && vec && (layout == 1 || (layout < 0 && auto_staged(q)))) {
+            // automatic choice: one pass where its open chunks fit, else two,
+            // else (partition state beyond LDS) global atomics
+            if (layout < 0) {
                 st = run_rowwise_onepass(q, sum_p, count, s);
                 if (st != OLAP_E_CAPACITY) return st;
                 ctx->err.clear();
             }
-            return run_rowwise_stage(q, sum_p, count, s);
+            st = run_rowwise_stage(q, sum_p, count, s);
+            if (st != OLAP_E_CAPACITY || own) return st;
+            ctx->err.clear();
+            layout = 0;
         }
         ScanArgs a;
         fill_args(a, q, L, sum_p, count);
         uint32_t dyn = L.dyn_lds;
-        if (strategy == OLAP_ROWWISE && !L.gt_lds && q->agg_layout == 2) {
+        if (strategy == OLAP_ROWWISE && !L.gt_lds && layout == 2) {
             // global atomics behind an LDS cache of hot groups: 16 bytes a
             // slot, 256..4096 slots in the LDS the steps leave
             const uint32_t base = align_up(L.dyn_lds, 16u);
@@ -895,11 +929,16 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
             }
         }
         hipError_t e;
+        const ScanShape sh = strategy == OLAP_ROWWISE && vec ? row_shape(effective_shape(q))
+                                                             : ScanShape{kBlock, kUnroll};
         if (strategy == OLAP_ROWWISE)
             e = run_rowwise(a, nd, ctx->num_cus, dyn, vec, effective_shape(q), s);
         else if (strategy == OLAP_VWM_SV) e = run_vwm_sv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         else e = run_vwm_dv(a, nd, ctx->num_cus, L.dyn_lds, vec, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "scan kernel");
+        q->ran_threads = sh.bs;
+        q->ran_rows = sh.unroll * kChunk;
+        q->ran_agg = L.gt_lds ? OLAP_AGG_LDS : a.hot_bits ? OLAP_AGG_HOT : OLAP_AGG_ATOMICS;
         return OLAP_OK;
     }
     case OLAP_CWM_SV: {
@@ -1008,6 +1047,7 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
         if (st == OLAP_E_CAPACITY && (cands[(size_t)i].second == 1 || cands[(size_t)i].second == 3)) {
             // a staged form whose partition state or open chunks do not fit
             st = OLAP_OK;
+            ctx->err.clear();
             continue;
         }
         std::vector<double> t;
@@ -1037,20 +1077,34 @@ olap_status olap_query_autotune(olap_query q, olap_strategy strategy, int64_t *s
     }
     q->row_shape = cands[(size_t)best].first;
     q->agg_layout = cands[(size_t)best].second;
-    const uint64_t key = query_class(q, strategy);
-    bool found = false;
-    for (auto &kv : ctx->tuned)
-        if (kv.first == key) {
-            kv.second = q->row_shape;
-            found = true;
-        }
-    if (!found) ctx->tuned.push_back({key, q->row_shape});
+    if (strategy == OLAP_ROWWISE) {
+        const uint64_t key = query_class(q, strategy);
+        bool found = false;
+        for (auto &t : ctx->tuned)
+            if (t.key == key) {
+                t.shape = q->row_shape;
+                t.layout = q->agg_layout;
+                found = true;
+            }
+        if (!found) ctx->tuned.push_back({key, q->row_shape, q->agg_layout});
+    }
     // shape index; num_shapes(strategy) stands for a staged aggregation
     if (shape_out)
         *shape_out = q->agg_layout == 1 || q->agg_layout == 3 ? olap_num_shapes(strategy) : q->row_shape;
     if (ms_out) *ms_out = best_ms;
     // leave the outputs holding a complete run of the chosen shape
     return olap_query_run(q, strategy, sum, count, stream);
+}
+
+olap_status olap_query_launch_info(olap_query q, int32_t *threads, int32_t *rows_per_thread,
+                                   int32_t *aggregation)
+{
+    if (!q) return OLAP_E_INVALID;
+    if (q->ran_agg < 0) return fail(q->ctx, OLAP_E_INVALID, "launch_info: the query has not run");
+    if (threads) *threads = q->ran_threads;
+    if (rows_per_thread) *rows_per_thread = q->ran_rows;
+    if (aggregation) *aggregation = q->ran_agg;
+    return OLAP_OK;
 }
 
 int64_t olap_query_device_bytes(olap_query q)

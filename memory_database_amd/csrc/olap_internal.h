@@ -94,8 +94,14 @@ struct olap_ctx_s {
     int64_t hbm_bytes = 0;
     char arch[64] = {0};
     std::string err;
-    // autotuner cache: query class -> chosen launch shape
-    std::vector<std::pair<uint64_t, int>> tuned;
+    // autotuner cache: query class -> chosen launch shape and aggregation
+    // layout of the row-wise scan
+    struct Tuned {
+        uint64_t key;
+        int shape;
+        int layout;
+    };
+    std::vector<Tuned> tuned;
     uint32_t *d_err = nullptr;     // error counter (4 words)
     void *d_scratch = nullptr;     // small reduction scratch
     size_t scratch_bytes = 0;
@@ -127,6 +133,9 @@ struct olap_query_s {
     uint32_t eager1 = 0;               // row-wise: stream step 1's keys with step 0's
     olap::Stager *stager = nullptr;
     uint32_t stage_grid = 0;
+    // launch of the last run: threads per workgroup, rows per thread and the
+    // aggregation (OLAP_AGG_*)
+    int32_t ran_threads = 0, ran_rows = 0, ran_agg = -1;
 };
 
 namespace olap {
@@ -172,6 +181,9 @@ hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bo
                        int shape, hipStream_t s);
 int num_row_shapes();
 ScanShape row_shape(int i);
+// launch shapes of the one-pass and the two-pass staged row-wise scans
+ScanShape chunk_shape();
+ScanShape staged_shape();
 // staged row-wise scan (group table beyond LDS): grid shared by both passes
 hipError_t rowwise_staged_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows,
                                uint32_t *grid);

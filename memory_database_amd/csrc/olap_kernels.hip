@@ -1203,6 +1203,8 @@ static const void *chunk_kernel(int ndims)
 }
 
 uint32_t rowwise_chunk_rows() { return kChunkRows; }
+ScanShape chunk_shape() { return ScanShape{kChunkBS, kChunkU}; }
+ScanShape staged_shape() { return ScanShape{kStageBS, kStageU}; }
 
 hipError_t rowwise_chunk_grid(int ndims, int num_cus, uint32_t dyn_lds, uint64_t rows, uint32_t *grid)
 {

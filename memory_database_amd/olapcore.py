@@ -24,6 +24,8 @@ OK, E_INVALID, E_HIP, E_NOMEM, E_CAPACITY, E_NODEVICE, E_BUSY = range(7)
 ROWWISE, VWM_DV, VWM_SV, CWM_DV, CWM_SV = range(5)
 STRATEGIES = {"rowwise": ROWWISE, "vwm_dv": VWM_DV, "vwm_sv": VWM_SV,
               "cwm_dv": CWM_DV, "cwm_sv": CWM_SV}
+AGGREGATIONS = ["global atomics", "two-pass staged", "hot-group cache", "one-pass staged",
+                "LDS table"]
 # table slots as the reference's main() indexes them
 TABLES = {"customer": 0, "supplier": 1, "part": 2, "date": 3}
 
@@ -143,6 +145,8 @@ SIGNATURES = {
     "olap_query_run": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "olap_query_destroy": (C.c_int, [_P]),
     "olap_query_device_bytes": (C.c_int64, [_P]),
+    "olap_query_launch_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]),
     "olap_num_shapes": (C.c_int32, [C.c_int]),
     "olap_shape_info": (C.c_int, [C.c_int, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "olap_query_set_shape": (C.c_int, [_P, C.c_int32]),
@@ -487,6 +491,13 @@ class Query:
         atomics behind an LDS cache of hot groups, 3 one-pass staged (group
         tables beyond LDS)."""
         self.ctx.check(self.ctx.lib.olap_query_set_aggregation(self.h, layout), "set_aggregation")
+
+    def launch_info(self):
+        """(threads, rows per thread, aggregation name) of the last run."""
+        th, rp, ag = C.c_int32(), C.c_int32(), C.c_int32()
+        self.ctx.check(self.ctx.lib.olap_query_launch_info(self.h, C.byref(th), C.byref(rp),
+                                                           C.byref(ag)), "launch_info")
+        return th.value, rp.value, AGGREGATIONS[ag.value]
 
     def device_bytes(self) -> int:
         return int(self.ctx.lib.olap_query_device_bytes(self.h))

@@ -51,12 +51,15 @@ def arrivals(rate: float, n: int, ndatasets: int, ntemplates: int, seed: int = 1
 
 
 def run_open_loop(server: oc.Server, dataset_ids: Sequence[int], queries: Sequence[oc.BatchQuery],
-                  plan: Sequence[Arrival], keep_results: bool = False):
+                  plan: Sequence[Arrival], keep_results=False):
     """Submit `plan` on schedule from this thread while a collector thread
     waits for the results in submission order. Latency of a request = its
     submit lag behind schedule + the server's submit-to-result time.
     Returns a dict of q/s, p50/p99/max latency (ms) and, when asked, the
-    results [(arrival index, sum, count)]."""
+    results [(arrival index, sum, count)]: of every arrival when
+    keep_results is True, of the arrivals i with keep_results(i) when it is
+    a function."""
+    keep = keep_results if callable(keep_results) else (lambda i: bool(keep_results))
     tickets: List[tuple] = []
     cv = threading.Condition()
     done = {"lat": [], "err": [], "results": [], "t_last": 0.0}
@@ -72,7 +75,7 @@ def run_open_loop(server: oc.Server, dataset_ids: Sequence[int], queries: Sequen
             try:
                 s, c, lat_us = server.wait(tk, with_latency=True)
                 done["lat"].append(lag + lat_us * 1e-6)
-                if keep_results:
+                if keep(i):
                     done["results"].append((i, s, c))
             except oc.OlapError as e:  # a failed batch: reported, not hidden
                 done["err"].append((i, str(e)))

@@ -5,8 +5,8 @@ order, with the byte past each dimension vector set to 0. Every strategy must
 give the recorded count.
 
 Generating each SF=100 dataset draws 2.4 G rand() values on one host core
-(tens of seconds per query), so the GPU cases run only when
-OLAP_KNOWN_ANSWERS=1 is set."""
+(tens of seconds per query), so two of the ten cases run by default and the
+other eight only when OLAP_KNOWN_ANSWERS=1 is set."""
 import json
 import os
 import subprocess
@@ -26,11 +26,21 @@ def test_fixture_holds_the_ten_recorded_runs():
     assert all(c["count"] % 10 == 0 for c in CASES)
 
 
+# Q2.1 (the largest group table of the ten, 47,780,820) and Q3.4 (the
+# smallest total, 4,050) always run; the other eight with OLAP_KNOWN_ANSWERS=1.
+ALWAYS = (21, 34)
+
+
+def _gate(case):
+    if case["sqlnum"] in ALWAYS:
+        return pytest.param(case, id=str(case["sqlnum"]))
+    return pytest.param(case, id=str(case["sqlnum"]), marks=pytest.mark.skipif(
+        os.environ.get("OLAP_KNOWN_ANSWERS") != "1",
+        reason="SF=100 host generation takes tens of seconds per query; set OLAP_KNOWN_ANSWERS=1"))
+
+
 @pytest.mark.gpu
-@pytest.mark.skipif(os.environ.get("OLAP_KNOWN_ANSWERS") != "1",
-                    reason="SF=100 host generation takes tens of seconds per query; "
-                           "set OLAP_KNOWN_ANSWERS=1")
-@pytest.mark.parametrize("case", CASES, ids=[str(c["sqlnum"]) for c in CASES])
+@pytest.mark.parametrize("case", [_gate(c) for c in CASES])
 def test_driver_reproduces_recorded_total(case, tmp_path):
     assert os.path.exists(DRIVER), "build the driver first"
     js = tmp_path / "out.jsonl"

@@ -253,6 +253,40 @@ def test_staged_forms_refuse_partitions_beyond_lds(ctx, ncpu):
     q.close()
 
 
+def test_automatic_layout_beyond_partition_state_takes_atomics(ctx, ncpu):
+    """2^27 groups with ~2.4M expected matches (4M rows, 60% pass): the
+    automatic choice would stage, but both staged forms are refused for
+    partition state beyond LDS, so it takes global atomics and still equals
+    the oracle."""
+    import torch
+
+    G = 1 << 27
+    rows = 4_000_000
+    rng = np.random.default_rng(127)
+    _say(f"G=2^27, {rows} rows, automatic layout")
+    D = 100_000
+    dv = po.build_dimvec(rng.integers(0, 100, D, dtype=np.int32), rng.integers(0, G, D, dtype=np.int32),
+                         0, 60, 1, 4)
+    dvs = [dv, None, None, None]
+    fks = [rng.integers(0, D, rows, dtype=np.int32), None, None, None]
+    m1 = rng.integers(1, 1000, rows, dtype=np.int32)
+    p = po.plan([0], [1], G)
+    es, ec, _ = po.run("rowwise", p, po.Star(dvs, fks, m1), nthreads=ncpu)
+    assert ec.sum() > 2_000_000
+    plan = oc.plan_from([0], [1], G)
+    cols = oc.Context.columns([_dev(d, torch) for d in dvs], [_dev(x, torch) for x in fks],
+                              _dev(m1, torch), None, rows=rows)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    q.set_aggregation(-1)
+    q.run("rowwise", osum, ocnt)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
+    np.testing.assert_array_equal(osum.cpu().numpy(), es)
+    q.close()
+
+
 def test_large_dimensions_spill_to_hbm(ctx, ncpu):
     """Dimension vectors larger than LDS are probed from HBM."""
     rng = np.random.default_rng(11)
