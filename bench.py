@@ -435,7 +435,7 @@ def main():
     from memory_database_amd import olapcore as oc
     from memory_database_amd import workloads as wl
 
-    cfg = wl.CONFIGS[args.config]
+    cfg = wl.config_by_name(args.config)
     ctx = oc.Context(0)
     info = ctx.info()
     stream = torch.cuda.Stream()

@@ -106,6 +106,17 @@ def sweep_configs(rows: int = 600_000_000, thetas=SWEEP_THETA, groups=tuple(SWEE
     return out
 
 
+def config_by_name(name: str) -> StarConfig:
+    """A named configuration: the three plain workloads or a point of the
+    600M-row sweep (e.g. sweep_4dim_600M_theta0_G1000000_sel50)."""
+    if name in CONFIGS:
+        return CONFIGS[name]
+    for c in sweep_configs():
+        if c.name == name:
+            return c
+    raise KeyError(f"unknown configuration {name}")
+
+
 def scaled(cfg: StarConfig, rows: int, name: Optional[str] = None, **kw) -> StarConfig:
     d = dict(cfg.__dict__)
     d.update(kw)
