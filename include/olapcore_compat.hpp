@@ -13,7 +13,8 @@
 //  - a foreign key outside [0, dimension rows) finds no row (the reference
 //    reads past the vector); the dimension sizes are therefore required and
 //    are set once with olapcore_compat_set_dim_rows;
-//  - nthreads is accepted and ignored (one GPU);
+//  - nthreads is accepted and ignored (one GPU); the NUMA drivers read the
+//    node shards in row order and run one scan over all of them;
 //  - errors return -1 (message in olapcore_compat_last_error) instead of exit(-1).
 #pragma once
 
@@ -22,6 +23,9 @@
 // Dimension-vector lengths by table slot (0 customer, 1 supplier, 2 part,
 // 3 date), e.g. size_of_table() of the reference's generator.
 void olapcore_compat_set_dim_rows(const int64_t rows[4]);
+// Node shards the NUMA drivers' arrays hold (default 0: the host's NUMA
+// nodes, numa_max_node() + 1, as the reference's main sizes them).
+void olapcore_compat_set_numa_nodes(int nodes);
 // Device used by the compat functions (default 0).
 void olapcore_compat_set_device(int device);
 const char *olapcore_compat_last_error();
@@ -41,6 +45,20 @@ int OLAPcore_vwm_dv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int 
 int OLAPcore_vwm_sv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &dimvec_nums,
                     int *factor, int &size_lineorder, int &group_nums, int32_t *M1, int32_t *M2,
                     int nthreads);
+// The reference's NUMA argument structs (include/metadata.h:272-279), defined
+// there as { int8_t *dimvec[4]; } and { int32_t *fk[4]; }, one per node.
+struct Dimvec_array_numa;
+struct Fk_array_numa;
+// OLAPcore_vwm_dv_numa, :1142 -- node i holds rows [i * (N / n), ...) of the
+// fact columns (fk_array_numa[i], M1_p[i], M2_p[i]; the last node the
+// remainder) and a replica of the dimension vectors (dimvec_array_numa[i])
+int OLAPcore_vwm_dv_numa(Dimvec_array_numa *dimvec_array_numa, Fk_array_numa *fk_array_numa,
+                         int *orders, int &dimvec_nums, int *factor, int &size_lineorder,
+                         int &group_nums, int32_t **M1_p, int32_t **M2_p, int nthreads);
+// OLAPcore_vwm_sv_numa, :1342
+int OLAPcore_vwm_sv_numa(Dimvec_array_numa *dimvec_array_numa, Fk_array_numa *fk_array_numa,
+                         int *orders, int &dimvec_nums, int *factor, int &size_lineorder,
+                         int &group_nums, int32_t **M1_p, int32_t **M2_p, int nthreads);
 // join_cwm_dv, :840 -- fills the per-thread OID/groupID lists and index[]
 // counts exactly as the reference partitions rows (thread j owns rows
 // [j*N/nthreads, ...), the last thread the remainder)

@@ -16,6 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_PATH = os.path.join(HERE, "libolapcore.so")
 DRIVER_PATH = os.path.join(HERE, "olapcore_test")
+HARNESS_SRC = os.path.join(ROOT, "tests", "native", "compat_harness.cpp")
+HARNESS_PATH = os.path.join(ROOT, "tests", "native", "compat_harness")
 
 ARCH = os.environ.get("OLAP_OFFLOAD_ARCH", "gfx950")
 
@@ -73,6 +75,12 @@ def build_library(verbose: bool = False) -> str:
     if os.path.exists(drv):
         cmd = [hipcc, "-O2", "-std=c++17", *inc, "-o", DRIVER_PATH, drv,
                f"-L{HERE}", "-lolapcore", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"]
+        _run(cmd, verbose)
+    # the GPU tests' caller of the reference-signature layer, linked the way a
+    # caller of the reference's operators links the library
+    if os.path.exists(HARNESS_SRC):
+        cmd = [hipcc, "-O2", "-std=c++17", *inc, "-o", HARNESS_PATH, HARNESS_SRC,
+               f"-L{HERE}", "-lolapcore", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../memory_database_amd"]
         _run(cmd, verbose)
     return LIB_PATH
 

@@ -3,6 +3,7 @@
 // results out; every call uploads the columns it reads.
 #include <hip/hip_runtime.h>
 
+#include <dirent.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -13,6 +14,16 @@
 
 #include "olap_internal.h"
 #include "olapcore_compat.hpp"
+
+// The reference's NUMA argument structs (include/metadata.h:272-279): one
+// per node, the node's replica of every dimension vector and its shard of
+// every foreign-key column, by table slot.
+struct Dimvec_array_numa {
+    int8_t *dimvec[4];
+};
+struct Fk_array_numa {
+    int32_t *fk[4];
+};
 
 namespace {
 
@@ -58,6 +69,50 @@ bool upload(DevBuf &b, const void *host, size_t bytes)
     return true;
 }
 
+// Fact rows as the caller holds them: one host array per column, or (NUMA
+// drivers) one array per node shard, the shards in row order.
+struct FactShards {
+    int n = 1;                       // shards
+    std::vector<int64_t> rows;       // rows per shard
+    std::vector<int32_t *const *> fk;  // per shard: fk[table]
+    std::vector<const int32_t *> m1, m2;
+};
+
+// upload one fact column, shard by shard, into one device array
+bool upload_column(DevBuf &b, const FactShards &F, int64_t total,
+                   const int32_t *(*pick)(const FactShards &, int, int), int table)
+{
+    if (olap_dev_alloc(g_ctx, total ? 4 * (size_t)total : 4, &b.p) != OLAP_OK) {
+        g_err = olap_last_error(g_ctx);
+        return false;
+    }
+    int64_t off = 0;
+    for (int i = 0; i < F.n; i++) {
+        const int32_t *src = pick(F, i, table);
+        if (F.rows[(size_t)i] && olap_copy_h2d(g_ctx, (int32_t *)b.p + off, src,
+                                               4 * (size_t)F.rows[(size_t)i], nullptr) != OLAP_OK) {
+            g_err = olap_last_error(g_ctx);
+            return false;
+        }
+        off += F.rows[(size_t)i];
+    }
+    return true;
+}
+
+const int32_t *pick_fk(const FactShards &F, int i, int t) { return F.fk[(size_t)i][t]; }
+const int32_t *pick_m1(const FactShards &F, int i, int) { return F.m1[(size_t)i]; }
+const int32_t *pick_m2(const FactShards &F, int i, int) { return F.m2[(size_t)i]; }
+
+FactShards one_shard(int32_t **fk_array, int rows, const int32_t *M1, const int32_t *M2)
+{
+    FactShards F;
+    F.rows = {rows};
+    F.fk = {fk_array};
+    F.m1 = {M1};
+    F.m2 = {M2};
+    return F;
+}
+
 int32_t legacy(const std::vector<int64_t> &sum)
 {
     return olap_legacy_count(sum.data(), (int64_t)sum.size());
@@ -75,9 +130,10 @@ struct Staged {
     }
 };
 
-bool stage(Staged &S, int8_t **dimvec_array, int32_t **fk_array, const int *orders, int ndims,
-           const int *factor, int rows, int group_nums, const int32_t *M1, const int32_t *M2)
+bool stage(Staged &S, int8_t *const *dimvec_array, const FactShards &F, const int *orders, int ndims,
+           const int *factor, int rows, int group_nums)
 {
+    const bool with_m1 = F.m1[0] != nullptr, with_m2 = F.m2[0] != nullptr;
     if (!ensure_ctx()) return false;
     if (ndims < 1 || ndims > 4 || rows < 0 || group_nums < 1) {
         g_err = "bad dimvec_nums / size_lineorder / group_nums";
@@ -98,7 +154,7 @@ bool stage(Staged &S, int8_t **dimvec_array, int32_t **fk_array, const int *orde
         S.plan.orders[i] = t;
         S.plan.factor[i] = factor[i];
         if (!upload(S.dv[t], dimvec_array[t], (size_t)g_dim_rows[t]) ||
-            !upload(S.fk[t], fk_array[t], sizeof(int32_t) * (size_t)rows))
+            !upload_column(S.fk[t], F, rows, pick_fk, t))
             return false;
         S.cols.dimvec[t] = S.dv[t].p;
         S.cols.dim_rows[t] = g_dim_rows[t];
@@ -107,8 +163,8 @@ bool stage(Staged &S, int8_t **dimvec_array, int32_t **fk_array, const int *orde
     }
     for (int t = 0; t < 4; t++)
         if (!S.cols.code_bytes[t]) S.cols.code_bytes[t] = 1;
-    if (M1 && !upload(S.m1, M1, sizeof(int32_t) * (size_t)rows)) return false;
-    if (M2 && !upload(S.m2, M2, sizeof(int32_t) * (size_t)rows)) return false;
+    if (with_m1 && !upload_column(S.m1, F, rows, pick_m1, 0)) return false;
+    if (with_m2 && !upload_column(S.m2, F, rows, pick_m2, 0)) return false;
     S.cols.m1 = (const int32_t *)S.m1.p;
     S.cols.m2 = (const int32_t *)S.m2.p;
     S.cols.rows = rows;
@@ -132,12 +188,12 @@ bool fetch_results(int64_t G, const DevBuf &sum, const DevBuf &cnt)
     return true;
 }
 
-int run_fused(olap_strategy strat, int8_t **dimvec_array, int32_t **fk_array, int *orders,
-              int dimvec_nums, int *factor, int rows, int group_nums, int32_t *M1, int32_t *M2)
+int run_fused(olap_strategy strat, int8_t *const *dimvec_array, const FactShards &F, int *orders,
+              int dimvec_nums, int *factor, int rows, int group_nums)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     Staged S;
-    if (!stage(S, dimvec_array, fk_array, orders, dimvec_nums, factor, rows, group_nums, M1, M2))
+    if (!stage(S, dimvec_array, F, orders, dimvec_nums, factor, rows, group_nums))
         return -1;
     DevBuf sum, cnt;
     if (olap_dev_alloc(g_ctx, 8 * (size_t)group_nums, &sum.p) != OLAP_OK ||
@@ -148,6 +204,51 @@ int run_fused(olap_strategy strat, int8_t **dimvec_array, int32_t **fk_array, in
     }
     if (!fetch_results(group_nums, sum, cnt)) return -1;
     return legacy(g_sum);
+}
+
+int g_numa_nodes = 0;  // 0: count the host's nodes
+
+// numa_max_node() + 1 as libnuma reports it: the highest node number under
+// /sys/devices/system/node, plus one
+int host_numa_nodes()
+{
+    int hi = -1;
+    if (DIR *d = opendir("/sys/devices/system/node")) {
+        while (dirent *e = readdir(d)) {
+            int k = -1;
+            char tail = 0;
+            if (sscanf(e->d_name, "node%d%c", &k, &tail) == 1 && k > hi) hi = k;
+        }
+        closedir(d);
+    }
+    return hi >= 0 ? hi + 1 : 1;
+}
+
+// The NUMA drivers' fact layout (test_multi_compute_operator.cpp:2017-2100):
+// node i holds rows [i * (N / n), ...) of every fact column, the last node
+// the remainder, and a replica of every dimension vector.
+int run_numa(olap_strategy strat, Dimvec_array_numa *dv, Fk_array_numa *fk, int *orders,
+             int dimvec_nums, int *factor, int rows, int group_nums, int32_t **M1_p,
+             int32_t **M2_p)
+{
+    int n;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        n = g_numa_nodes > 0 ? g_numa_nodes : host_numa_nodes();
+        if (!dv || !fk || !M1_p) {
+            g_err = "NUMA driver: null dimension, foreign-key or measure array";
+            return -1;
+        }
+    }
+    FactShards F;
+    F.n = n;
+    for (int i = 0; i < n; i++) {
+        F.rows.push_back(i == n - 1 ? rows - (int64_t)(n - 1) * (rows / n) : rows / n);
+        F.fk.push_back(fk[i].fk);
+        F.m1.push_back(M1_p[i]);
+        F.m2.push_back(M2_p ? M2_p[i] : nullptr);
+    }
+    return run_fused(strat, dv[0].dimvec, F, orders, dimvec_nums, factor, rows, group_nums);
 }
 
 // thread partition of the reference drivers: thread j owns rows
@@ -166,6 +267,12 @@ void olapcore_compat_set_dim_rows(const int64_t rows[4])
 {
     std::lock_guard<std::mutex> lk(g_mu);
     for (int t = 0; t < 4; t++) g_dim_rows[t] = rows[t];
+}
+
+void olapcore_compat_set_numa_nodes(int nodes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_numa_nodes = nodes > 0 ? nodes : 0;
 }
 
 void olapcore_compat_set_device(int device)
@@ -187,8 +294,8 @@ int OLAPcore_rowwise(int8_t **dimvec_array, int32_t **fk_array, int *orders, int
                      int nthreads)
 {
     (void)nthreads;
-    return run_fused(OLAP_ROWWISE, dimvec_array, fk_array, orders, dimvec_nums, factor,
-                     size_lineorder, group_nums, M1, M2);
+    return run_fused(OLAP_ROWWISE, dimvec_array, one_shard(fk_array, size_lineorder, M1, M2), orders,
+                     dimvec_nums, factor, size_lineorder, group_nums);
 }
 
 int OLAPcore_vwm_dv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &dimvec_nums,
@@ -196,8 +303,8 @@ int OLAPcore_vwm_dv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int 
                     int nthreads)
 {
     (void)nthreads;
-    return run_fused(OLAP_VWM_DV, dimvec_array, fk_array, orders, dimvec_nums, factor,
-                     size_lineorder, group_nums, M1, M2);
+    return run_fused(OLAP_VWM_DV, dimvec_array, one_shard(fk_array, size_lineorder, M1, M2), orders,
+                     dimvec_nums, factor, size_lineorder, group_nums);
 }
 
 int OLAPcore_vwm_sv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &dimvec_nums,
@@ -205,8 +312,26 @@ int OLAPcore_vwm_sv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int 
                     int nthreads)
 {
     (void)nthreads;
-    return run_fused(OLAP_VWM_SV, dimvec_array, fk_array, orders, dimvec_nums, factor,
-                     size_lineorder, group_nums, M1, M2);
+    return run_fused(OLAP_VWM_SV, dimvec_array, one_shard(fk_array, size_lineorder, M1, M2), orders,
+                     dimvec_nums, factor, size_lineorder, group_nums);
+}
+
+int OLAPcore_vwm_dv_numa(Dimvec_array_numa *dimvec_array_numa, Fk_array_numa *fk_array_numa,
+                         int *orders, int &dimvec_nums, int *factor, int &size_lineorder,
+                         int &group_nums, int32_t **M1_p, int32_t **M2_p, int nthreads)
+{
+    (void)nthreads;
+    return run_numa(OLAP_VWM_DV, dimvec_array_numa, fk_array_numa, orders, dimvec_nums, factor,
+                    size_lineorder, group_nums, M1_p, M2_p);
+}
+
+int OLAPcore_vwm_sv_numa(Dimvec_array_numa *dimvec_array_numa, Fk_array_numa *fk_array_numa,
+                         int *orders, int &dimvec_nums, int *factor, int &size_lineorder,
+                         int &group_nums, int32_t **M1_p, int32_t **M2_p, int nthreads)
+{
+    (void)nthreads;
+    return run_numa(OLAP_VWM_SV, dimvec_array_numa, fk_array_numa, orders, dimvec_nums, factor,
+                    size_lineorder, group_nums, M1_p, M2_p);
 }
 
 void join_cwm_dv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &dimvec_nums,
@@ -218,8 +343,8 @@ void join_cwm_dv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &di
     for (int j = 0; j < nthreads; j++) index[j] = 0;
     Staged S;
     // group_nums only bounds the gid check here; the lists carry raw gids
-    if (!stage(S, dimvec_array, fk_array, orders, dimvec_nums, factor, size_lineorder, INT32_MAX,
-               nullptr, nullptr))
+    if (!stage(S, dimvec_array, one_shard(fk_array, size_lineorder, nullptr, nullptr), orders,
+               dimvec_nums, factor, size_lineorder, INT32_MAX))
         return;
     if (olap_join_cwm_dv(S.q, nullptr) != OLAP_OK || olap_stream_sync(g_ctx, nullptr) != OLAP_OK) {
         g_err = olap_last_error(g_ctx);
@@ -292,8 +417,8 @@ void join_cwm_sv(int8_t **dimvec_array, int32_t **fk_array, int *orders, int &di
     (void)nthreads;
     std::lock_guard<std::mutex> lk(g_mu);
     Staged S;
-    if (!stage(S, dimvec_array, fk_array, orders, dimvec_nums, factor, size_lineorder, INT32_MAX,
-               nullptr, nullptr))
+    if (!stage(S, dimvec_array, one_shard(fk_array, size_lineorder, nullptr, nullptr), orders,
+               dimvec_nums, factor, size_lineorder, INT32_MAX))
         return;
     DevBuf g;
     if (olap_dev_alloc(g_ctx, 4 * (size_t)size_lineorder + 16, &g.p) != OLAP_OK ||
