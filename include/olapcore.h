@@ -399,6 +399,21 @@ olap_status olap_vector_group(olap_ctx ctx, int64_t rows, const int32_t *key1,
                               const int32_t *m1, const int32_t *m2, int64_t *group_vector,
                               int64_t group_num, int64_t *total, int64_t *bad_keys,
                               void *stream);
+/* vecjoin build (code/join/test_join_option.cpp:30-36): vector[key[i]] =
+ * payload[i] over the build relation (device columns, rows < 2^31); a later
+ * row overwrites an earlier row with the same key, as the reference's
+ * sequential loop does; entries no key names keep their contents. Keys
+ * outside [0, vector_len) are skipped and counted in bad_keys (host,
+ * nullable). Synchronises. */
+olap_status olap_join_build(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *payload,
+                            int32_t *vector, int64_t vector_len, int64_t *bad_keys, void *stream);
+/* vecjoin probe (:58-64; probe_prefetch :72-82 sums the same): total (host)
+ * = sum of vector[key[i]] over the probe relation, in int64 (the reference's
+ * int count is its low 32 bits); keys outside [0, vector_len) add nothing
+ * and are counted in bad_keys (host, nullable). A vector of up to 32,768
+ * entries is probed from LDS. Synchronises. */
+olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *vector,
+                            int64_t vector_len, int64_t *total, int64_t *bad_keys, void *stream);
 /* aggalgo_rowwise (code/aggregation/test_aggregation_option.cpp:28-37) in fixed
  * point: SUM(p*q*t + 100*p*q) with t the tax in percent; host result. */
 olap_status olap_agg_expr(olap_ctx ctx, int64_t rows, const int32_t *p, const int32_t *q,

@@ -687,6 +687,7 @@ olap_status olap_ctx_destroy(olap_ctx ctx)
         DeviceGuard g(ctx->device);
         dev_free(ctx->d_err);
         dev_free(ctx->d_scratch);
+        dev_free(ctx->join_win);
     }
     delete ctx;
     return OLAP_OK;
@@ -1322,6 +1323,57 @@ olap_status olap_vector_group(olap_ctx ctx, int64_t rows, const int32_t *key1, c
     HIPC(ctx, hipStreamSynchronize(s));
     *total = (int64_t)ht;
     if (bad_keys) *bad_keys = hb;
+    return OLAP_OK;
+}
+
+olap_status olap_join_build(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *payload,
+                            int32_t *vector, int64_t vector_len, int64_t *bad_keys, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || rows > INT32_MAX || vector_len < 0 || vector_len > INT32_MAX ||
+        (rows && (!key || !payload)) || (vector_len && !vector))
+        return fail(ctx, OLAP_E_INVALID, "join_build: rows and vector_len must be in [0, 2^31)");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    if (vector_len > 0 && (size_t)vector_len > ctx->join_win_len) {
+        dev_free(ctx->join_win);
+        ctx->join_win = nullptr;
+        ctx->join_win_len = 0;
+        HIPC(ctx, hipMalloc(&ctx->join_win, (size_t)vector_len * 4));
+        ctx->join_win_len = (size_t)vector_len;
+    }
+    uint32_t *dbad = (uint32_t *)ctx->d_scratch;
+    HIPC(ctx, launch_zero_u32(dbad, 1, s));
+    HIPC(ctx, launch_join_build(rows, key, payload, ctx->join_win, vector, vector_len, dbad,
+                                ctx->num_cus, s));
+    uint32_t hbad = 0;
+    HIPC(ctx, hipMemcpyAsync(&hbad, dbad, 4, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    if (bad_keys) *bad_keys = hbad;
+    return OLAP_OK;
+}
+
+olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *vector,
+                            int64_t vector_len, int64_t *total, int64_t *bad_keys, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || vector_len < 0 || vector_len > INT32_MAX || (rows && !key) ||
+        (vector_len && !vector))
+        return fail(ctx, OLAP_E_INVALID, "join_probe: bad arguments");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    unsigned long long *dtot = (unsigned long long *)ctx->d_scratch;
+    uint32_t *dbad = (uint32_t *)((char *)ctx->d_scratch + 8);
+    HIPC(ctx, launch_zero_u64(dtot, 1, s));
+    HIPC(ctx, launch_zero_u32(dbad, 1, s));
+    HIPC(ctx, launch_join_probe(rows, key, vector, vector_len, dtot, dbad, ctx->num_cus, s));
+    unsigned long long htot = 0;
+    uint32_t hbad = 0;
+    HIPC(ctx, hipMemcpyAsync(&htot, dtot, 8, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipMemcpyAsync(&hbad, dbad, 4, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    if (total) *total = (int64_t)htot;
+    if (bad_keys) *bad_keys = hbad;
     return OLAP_OK;
 }
 

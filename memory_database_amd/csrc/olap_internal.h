@@ -107,6 +107,9 @@ struct olap_ctx_s {
     size_t scratch_bytes = 0;
     // device address of this GPU's allocation-free scratch (olap_noalloc.hip)
     void *noalloc = nullptr;
+    // join build: the row claiming each key, grown on demand
+    int32_t *join_win = nullptr;
+    size_t join_win_len = 0;
 };
 
 struct olap_query_s {
@@ -202,6 +205,12 @@ hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t 
                                const int32_t *m2, unsigned long long *gv, int64_t G,
                                unsigned long long *total, uint32_t *bad, int num_cus,
                                hipStream_t s);
+// dense-vector join sub-step (olap_join.hip); winner: vlen int32 of scratch
+hipError_t launch_join_build(int64_t rows, const int32_t *key, const int32_t *payload,
+                             int32_t *winner, int32_t *vec, int64_t vlen, uint32_t *bad,
+                             int num_cus, hipStream_t s);
+hipError_t launch_join_probe(int64_t rows, const int32_t *key, const int32_t *vec, int64_t vlen,
+                             unsigned long long *total, uint32_t *bad, int num_cus, hipStream_t s);
 hipError_t launch_sum_u64(const unsigned long long *v, int64_t n, unsigned long long *out,
                           int num_cus, hipStream_t s);
 hipError_t launch_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t,

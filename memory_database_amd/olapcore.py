@@ -198,6 +198,8 @@ SIGNATURES = {
     "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
                                     _P]),
     "olap_agg_expr": (C.c_int, [_P, C.c_int64, _P, _P, _P, _I64P, _P]),
+    "olap_join_build": (C.c_int, [_P, C.c_int64, _P, _P, _P, C.c_int64, _I64P, _P]),
+    "olap_join_probe": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, _I64P, _I64P, _P]),
 }
 
 _lib = None
@@ -429,6 +431,24 @@ class Context:
                                               _ptr(m2), _ptr(group_vector), group_vector.numel(),
                                               C.byref(tot), C.byref(bad), stream),
                    "vector_group")
+        return tot.value, bad.value
+
+    def join_build(self, key, payload, vector, stream=None) -> int:
+        """vecjoin build into the int32 device vector; returns the keys
+        outside it."""
+        bad = C.c_int64()
+        self.check(self.lib.olap_join_build(self.h, key.numel(), _ptr(key), _ptr(payload),
+                                            _ptr(vector), vector.numel(), C.byref(bad), stream),
+                   "join_build")
+        return bad.value
+
+    def join_probe(self, key, vector, stream=None):
+        """(int64 sum of vector[key], keys outside the vector)."""
+        tot = C.c_int64()
+        bad = C.c_int64()
+        self.check(self.lib.olap_join_probe(self.h, key.numel(), _ptr(key), _ptr(vector),
+                                            vector.numel(), C.byref(tot), C.byref(bad), stream),
+                   "join_probe")
         return tot.value, bad.value
 
     def agg_expr(self, p, q, t, stream=None) -> int:

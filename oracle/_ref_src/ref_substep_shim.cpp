@@ -1,5 +1,6 @@
-// Recipe piece for oracle/_ref: the reference's star-join and group-by
-// sub-step operators, compiled from their own source files where they lie.
+// Recipe piece for oracle/_ref: the reference's star-join, vector-join and
+// group-by sub-step operators, compiled from their own source files where
+// they lie.
 // Two translation units would each define the header's non-inline helpers, so
 // this shim is built twice, once per operator file (SUBSTEP selects it).
 #if SUBSTEP == 1
@@ -18,6 +19,27 @@ extern "C" int ref_starjoin_cwm_sv(int n, const int8_t *c, const int8_t *s, cons
                                    const int32_t *fp, const int32_t *fd, int *result)
 {
     return starjoinalgo_cwm_sv(n, c, s, p, d, fc, fs, fp, fd, result);
+}
+#elif SUBSTEP == 3
+#define main ref_join_main
+#include "code/join/test_join_option.cpp"
+#undef main
+// vecjoin build (:30-36) and probe (:58-64) on a relation of n rows
+extern "C" void ref_join_build(long long n, int32_t *key, int32_t *payload, int *vector)
+{
+    relation_t r;
+    r.key = key;
+    r.payload = payload;
+    r.num_tuples = (uint64_t)n;
+    build(&r, vector);
+}
+extern "C" int ref_join_probe(long long n, int32_t *key, int *vector, int prefetch)
+{
+    relation_t s;
+    s.key = key;
+    s.payload = nullptr;
+    s.num_tuples = (uint64_t)n;
+    return prefetch ? probe_prefetch(&s, vector) : probe(&s, vector);
 }
 #else
 #define main ref_group_main

@@ -610,6 +610,37 @@ int64_t orc_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
     return total;
 }
 
+/* vecjoin build, code/join/test_join_option.cpp:30-36: vector[key[i]] =
+ * payload[i] over the build relation, in row order (a later row with the same
+ * key overwrites an earlier one); entries no key names are left as they are.
+ * Keys outside [0, vector_len) are skipped and counted in bad_keys. */
+void orc_join_build(int64_t rows, const int32_t *key, const int32_t *payload, int32_t *vector,
+                    int64_t vector_len, int64_t *bad_keys)
+{
+    int64_t bad = 0;
+    for (int64_t i = 0; i < rows; i++) {
+        if (key[i] < 0 || key[i] >= vector_len) { bad++; continue; }
+        vector[key[i]] = payload[i];
+    }
+    if (bad_keys) *bad_keys = bad;
+}
+
+/* vecjoin probe, code/join/test_join_option.cpp:58-64 (and probe_prefetch,
+ * :72-82, the same sum): count += vector[key[i]] over the probe relation,
+ * here in int64. Keys outside [0, vector_len) add nothing and are counted in
+ * bad_keys. */
+int64_t orc_join_probe(int64_t rows, const int32_t *key, const int32_t *vector,
+                       int64_t vector_len, int64_t *bad_keys)
+{
+    int64_t bad = 0, s = 0;
+    for (int64_t i = 0; i < rows; i++) {
+        if (key[i] < 0 || key[i] >= vector_len) { bad++; continue; }
+        s += vector[key[i]];
+    }
+    if (bad_keys) *bad_keys = bad;
+    return s;
+}
+
 /* aggalgo_rowwise, code/aggregation/test_aggregation_option.cpp:28-37:
  * SUM(price*qty*tax + price*qty) with tax a percentage (gendata_util.hpp:757,
  * tax = rand_x(L_TAX_MIN, L_TAX_MAX)/100). Restated in fixed point with scale

@@ -113,6 +113,10 @@ int64_t orc_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
                          const int32_t *m2, int64_t *group_vector, int64_t group_num,
                          int64_t *bad_keys);
 int64_t orc_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t);
+void orc_join_build(int64_t rows, const int32_t *key, const int32_t *payload, int32_t *vector,
+                    int64_t vector_len, int64_t *bad_keys);
+int64_t orc_join_probe(int64_t rows, const int32_t *key, const int32_t *vector,
+                       int64_t vector_len, int64_t *bad_keys);
 
 /* threads actually usable by this process (sched_getaffinity) */
 int orc_available_cpus(void);

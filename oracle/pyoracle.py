@@ -95,6 +95,9 @@ def lib() -> C.CDLL:
         L.orc_vector_group.argtypes = [C.c_int64, P, P, P, P, C.c_int64, P]
         L.orc_agg_expr.restype = C.c_int64
         L.orc_agg_expr.argtypes = [C.c_int64, P, P, P]
+        L.orc_join_build.argtypes = [C.c_int64, P, P, P, C.c_int64, P]
+        L.orc_join_probe.restype = C.c_int64
+        L.orc_join_probe.argtypes = [C.c_int64, P, P, C.c_int64, P]
         L.orc_available_cpus.restype = C.c_int
         _lib = L
     return _lib
@@ -264,6 +267,26 @@ def vector_group(key1, m1, m2, group_num):
     return int(tot), gv, bad.value
 
 
+def join_build(key, payload, vector):
+    """vecjoin build into `vector` (int32 numpy array, updated in place);
+    returns the keys outside the vector."""
+    k = np.ascontiguousarray(key, dtype=np.int32)
+    v = np.ascontiguousarray(payload, dtype=np.int32)
+    assert vector.dtype == np.int32 and vector.flags.c_contiguous
+    bad = C.c_int64()
+    lib().orc_join_build(k.size, _p(k), _p(v), _p(vector), vector.size, C.byref(bad))
+    return bad.value
+
+
+def join_probe(key, vector):
+    """(int64 sum of vector[key], keys outside the vector)."""
+    k = np.ascontiguousarray(key, dtype=np.int32)
+    v = np.ascontiguousarray(vector, dtype=np.int32)
+    bad = C.c_int64()
+    s = lib().orc_join_probe(k.size, _p(k), _p(v), v.size, C.byref(bad))
+    return int(s), bad.value
+
+
 def agg_expr(p, q, t):
     p = np.ascontiguousarray(p, dtype=np.int32)
     q = np.ascontiguousarray(q, dtype=np.int32)
@@ -309,6 +332,12 @@ class Reference:
             self.SJ.ref_starjoin_rowwise.argtypes = [C.c_int] + [P] * 8
             self.SJ.ref_starjoin_cwm_sv.restype = C.c_int
             self.SJ.ref_starjoin_cwm_sv.argtypes = [C.c_int] + [P] * 9
+        jn = os.path.join(REF_DIR, "libref_join.so")
+        self.JN = C.CDLL(jn) if os.path.exists(jn) else None
+        if self.JN is not None:
+            self.JN.ref_join_build.argtypes = [C.c_longlong, P, P, P]
+            self.JN.ref_join_probe.restype = C.c_int
+            self.JN.ref_join_probe.argtypes = [C.c_longlong, P, P, C.c_int]
         self.GP = C.CDLL(gp) if os.path.exists(gp) else None
         if self.GP is not None:
             self.GP.ref_vector_group.restype = C.c_longlong

@@ -11,7 +11,8 @@ the reference's single-thread workers OLAPcore_row_thread,
 OLAPcore_vwm_dv_thread, OLAPcore_vwm_sv_thread, join/agg_cwm_dv_thread and
 join/agg_cwm_sv_thread: their uint32 group vectors for the measures and for
 unit measures (= counts), and the int16 static gid vector of join_cwm_sv.
-starjoin_*.npz and group_*.npz hold the sub-step operators' results.
+starjoin_*.npz, join_*.npz and group_*.npz hold the sub-step operators'
+results (`python tests/golden/make_golden.py join` writes only join_*.npz).
 """
 from __future__ import annotations
 
@@ -58,10 +59,39 @@ def star_case(seed: int, ndims: int, rows: int):
     return dimvecs, fks, m1, m2, plan, groups, bitmap, sele
 
 
+def join_cases(ref):
+    """vecjoin build + probe (code/join/test_join_option.cpp:30-36, :58-82)
+    through the reference's own functions: unique keys 1..maxid shuffled as
+    its generator makes them (gendata_util.hpp:259-300, payload 1), the same
+    with random payloads, and a build relation with repeated keys (the later
+    row wins)."""
+    rng = np.random.default_rng(23)
+    for name, maxid, s_rows, payload_kind, dup in (("unique_ones", 1000, 50_000, "ones", False),
+                                                  ("unique_rand", 4096, 40_000, "rand", False),
+                                                  ("repeated", 777, 30_000, "rand", True)):
+        if dup:
+            rkey = rng.integers(1, maxid + 1, 3 * maxid, dtype=np.int32)
+        else:
+            rkey = rng.permutation(np.arange(1, maxid + 1, dtype=np.int32))
+        rpay = (np.ones(rkey.size, np.int32) if payload_kind == "ones"
+                else rng.integers(-1000, 1000, rkey.size, dtype=np.int32))
+        skey = rng.integers(1, maxid + 1, s_rows, dtype=np.int32)
+        vec = np.zeros(maxid + 1, np.int32)  # the reference leaves unnamed entries as they are
+        ref.JN.ref_join_build(rkey.size, rkey.ctypes.data, rpay.ctypes.data, vec.ctypes.data)
+        cnt = ref.JN.ref_join_probe(skey.size, skey.ctypes.data, vec.ctypes.data, 0)
+        cnt_pf = ref.JN.ref_join_probe(skey.size, skey.ctypes.data, vec.ctypes.data, 1)
+        path = os.path.join(HERE, f"join_{name}.npz")
+        np.savez_compressed(path, r_key=rkey, r_payload=rpay, s_key=skey, vector=vec,
+                            count=np.int64(cnt), count_prefetch=np.int64(cnt_pf))
+        print("wrote", path)
+
+
 def main():
     ref = po.Reference()
     if not ref.ok:
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["join"]:
+        return join_cases(ref)
     cases = [(11, 1, 5000), (12, 2, 7001), (13, 3, 6000), (14, 4, 9000), (15, 4, 3333),
              (16, 2, 1025)]
     for seed, nd, rows in cases:
@@ -107,6 +137,9 @@ def main():
                             fk0=fk[0], fk1=fk[1], fk2=fk[2], fk3=fk[3], count=np.int64(cnt),
                             count_cwm_sv=np.int64(cnt2), result=res)
         print("wrote", path)
+
+    if ref.JN is not None:
+        join_cases(ref)
 
     # dense-vector group-by sub-step, code/group/test_group_option.cpp:44-60
     if ref.GP is not None:

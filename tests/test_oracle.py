@@ -313,6 +313,59 @@ def test_vector_group_matches_golden(G):
     np.testing.assert_array_equal(gv.astype(np.int32), z["group_vector"])
 
 
+@pytest.mark.parametrize("name", ["unique_ones", "unique_rand", "repeated"])
+def test_join_matches_golden(name):
+    z = np.load(os.path.join(GOLDEN, f"join_{name}.npz"))
+    vec = np.zeros(z["vector"].size, np.int32)
+    assert po.join_build(z["r_key"], z["r_payload"], vec) == 0
+    np.testing.assert_array_equal(vec, z["vector"])
+    total, bad = po.join_probe(z["s_key"], vec)
+    assert bad == 0
+    # the reference sums into an int; the oracle's int64 sum wraps to it
+    assert np.int64(total).astype(np.int32) == np.int32(z["count"]) == np.int32(z["count_prefetch"])
+
+
+def test_join_closed_forms():
+    rng = np.random.default_rng(31)
+    maxid = 5000
+    rkey = rng.permutation(np.arange(1, maxid + 1, dtype=np.int32))
+    skey = rng.integers(1, maxid + 1, 100_000, dtype=np.int32)
+    # payload 1 (the reference's generator): every probe key hits once
+    vec = np.zeros(maxid + 1, np.int32)
+    po.join_build(rkey, np.ones(maxid, np.int32), vec)
+    assert po.join_probe(skey, vec) == (skey.size, 0)
+    # payload = key: the probe sums the probe keys
+    po.join_build(rkey, rkey, vec)
+    assert po.join_probe(skey, vec)[0] == int(skey.astype(np.int64).sum())
+    # keys outside the vector are skipped on both sides
+    bad_r = np.array([-1, maxid + 1, 10**9], np.int32)
+    assert po.join_build(bad_r, bad_r, vec) == 3
+    assert po.join_probe(np.concatenate([skey, bad_r]), vec) == (int(skey.astype(np.int64).sum()), 3)
+    # repeated build keys: the later row wins, like a sequential overwrite
+    k = np.array([7, 3, 7, 7, 3], np.int32)
+    v = np.array([1, 2, 3, 4, 5], np.int32)
+    vec = np.zeros(10, np.int32)
+    po.join_build(k, v, vec)
+    assert vec[7] == 4 and vec[3] == 5
+
+
+@needs_ref
+def test_join_matches_compiled_reference():
+    rng = np.random.default_rng(32)
+    for maxid, dup in ((64, False), (3000, True), (1 << 15, False)):
+        rkey = (rng.integers(1, maxid + 1, 2 * maxid, dtype=np.int32) if dup
+                else rng.permutation(np.arange(1, maxid + 1, dtype=np.int32)))
+        rpay = rng.integers(-(2 ** 31), 2 ** 31 - 1, rkey.size, dtype=np.int32)
+        skey = rng.integers(1, maxid + 1, 200_000, dtype=np.int32)
+        want = np.zeros(maxid + 1, np.int32)
+        REF.JN.ref_join_build(rkey.size, rkey.ctypes.data, rpay.ctypes.data, want.ctypes.data)
+        got = np.zeros(maxid + 1, np.int32)
+        po.join_build(rkey, rpay, got)
+        np.testing.assert_array_equal(got, want)
+        cnt = REF.JN.ref_join_probe(skey.size, skey.ctypes.data, want.ctypes.data, 0)
+        assert np.int64(po.join_probe(skey, got)[0]).astype(np.int32) == np.int32(cnt)
+
+
 def test_agg_expr_closed_form():
     rng = np.random.default_rng(8)
     p = rng.integers(5, 21, 10_000, dtype=np.int32)
