@@ -122,6 +122,67 @@ olap_status olap_build_dimvec(olap_ctx ctx, const int32_t *filter, const int32_t
                               int64_t rows, int32_t lo, int32_t hi, int32_t bitmap,
                               int32_t group_mod, int32_t code_bytes, void *out, void *stream);
 
+/* General dimension predicates: a dimension vector from a program of up to
+ * OLAP_MAX_TERMS terms over the dimension's columns, combined left to right
+ * (acc = acc AND term, or acc OR term), as the reference chains its select
+ * nodes (code/TPCH_Q5_operator/q5_operator.cpp:1288-1596, each node ANDed or
+ * ORed with the one before) and builds snowflake dimension vectors through a
+ * second table (code/TPCH_Q5/Q5.cpp:921-944, q5_operator.cpp:1472-1506):
+ *   RANGE  lo <= col[r] < hi (the <, <=, >, >=, == and != forms of the
+ *          reference's value selects, with negate for !=)
+ *   IN     col[r] is one of values[0..nvalues) (host array, any order); a
+ *          dictionary-coded string equality is IN over the string's code
+ *   HOP    parent[col[r] + key_bias] is a non-NULL code of the parent
+ *          dimension vector (int8 NULL 127 or int32 NULL -1; keys outside
+ *          [0, parent_rows) find no row) -- one hop of a snowflake; the
+ *          parent may itself be built by a program (two hops)
+ * negate inverts a term. The row's code: OLAP_GROUP_NONE 0 (filter only);
+ * OLAP_GROUP_COLUMN group_col[r] % group_mod (group_mod 0: the raw value);
+ * OLAP_GROUP_HOP the parent code that term group_term found (a row whose
+ * group term found none fails), e.g. customer -> nation grouped by nation. */
+#define OLAP_MAX_TERMS 8
+#define OLAP_MAX_IN_VALUES 4096
+typedef enum { OLAP_TERM_RANGE = 0, OLAP_TERM_IN = 1, OLAP_TERM_HOP = 2 } olap_term_kind;
+typedef enum { OLAP_AND = 0, OLAP_OR = 1 } olap_combine;
+typedef enum { OLAP_GROUP_NONE = 0, OLAP_GROUP_COLUMN = 1, OLAP_GROUP_HOP = 2 } olap_group_kind;
+
+typedef struct {
+    int32_t kind;               /* olap_term_kind */
+    int32_t combine;            /* olap_combine with the terms before; ignored for term 0 */
+    int32_t negate;             /* 1: the term passes where its test fails */
+    int32_t lo, hi;             /* RANGE */
+    int32_t nvalues;            /* IN: 1..OLAP_MAX_IN_VALUES */
+    int32_t parent_code_bytes;  /* HOP: 1 or 4 */
+    int32_t key_bias;           /* HOP: added to col[r] (e.g. -1 for 1-based keys) */
+    const int32_t *col;         /* device, dimension rows: the tested column or the hop key */
+    const int32_t *values;      /* IN: host */
+    const void *parent;         /* HOP: device parent dimension vector */
+    int64_t parent_rows;        /* HOP */
+} olap_term;
+
+typedef struct {
+    int32_t nterms;             /* 1..OLAP_MAX_TERMS */
+    int32_t group;              /* olap_group_kind */
+    int32_t group_mod;          /* OLAP_GROUP_COLUMN */
+    int32_t group_term;         /* OLAP_GROUP_HOP: index of a HOP term, not negated */
+    const int32_t *group_col;   /* OLAP_GROUP_COLUMN: device */
+    olap_term term[OLAP_MAX_TERMS];
+} olap_dim_program;
+
+/* out[r] = program passes row r ? code : NULL, code_bytes 1 (codes must fit
+ * 0..126, else OLAP_E_INVALID) or 4. Synchronises. */
+olap_status olap_build_dimvec_program(olap_ctx ctx, int64_t rows, const olap_dim_program *prog,
+                                      int32_t code_bytes, void *out, void *stream);
+
+/* Dictionary coding of a string column (host): codes[i] = rank of strs[i]
+ * among the column's distinct strings in byte order, so equal strings get
+ * equal codes and the codes do not depend on row order or threads (the
+ * reference's Group_Option numbers values first-seen under a mutex,
+ * q5_operator.cpp:1639-1692). ndistinct (nullable) receives the count. */
+olap_status olap_dict_encode(const char *const *strs, int64_t n, int32_t *codes, int64_t *ndistinct);
+/* The code of `value` in a column coded by olap_dict_encode, -1 when absent. */
+int32_t olap_dict_code_of(const char *const *strs, const int32_t *codes, int64_t n, const char *value);
+
 /* ---- steps 2-4: probe, compose, group-by SUM ----------------------------- */
 /* Validate and pack the dimension vectors once (synchronises). Rejects
  * negative non-NULL codes and plans whose largest gid reaches group_nums. */
@@ -239,6 +300,11 @@ typedef struct {
     int32_t group_mod[OLAP_MAX_DIMS];   /* code = attr % group_mod; 0: filter only */
     int32_t factor[OLAP_MAX_DIMS];      /* multiplier of step i's code */
     int64_t group_nums;                 /* output slots of the query */
+    /* optional general predicate of step i (host, NULL: none), ANDed with
+     * the range above; its group output is not used (the code is
+     * attr % group_mod). Read at batch creation (a server reads it when the
+     * request's batch is created: keep it valid until the wait returns). */
+    const olap_dim_program *program[OLAP_MAX_DIMS];
 } olap_batch_query;
 
 /* The shared star schema: per table slot a filter and a group attribute

@@ -89,6 +89,7 @@ struct MaskPred {
     int32_t lo;
     int32_t hi;
     int32_t pad;
+    const int8_t *pass;  // a general predicate's pass vector (NULL 127 fails), or null
 };
 
 // explicit address spaces keep the LDS and the HBM atomics apart (a generic
@@ -126,7 +127,9 @@ __global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32
         const int32_t f = in ? filter[r] : 0;
         unsigned long long m = base;
         for (int i = 0; i < np; i++) {
-            const bool ok = in && f >= preds[i].lo && f < preds[i].hi;
+            const int8_t *pv = preds[i].pass;
+            const bool ok = in && f >= preds[i].lo && f < preds[i].hi &&
+                            (!pv || pv[r] != (int8_t)OLAP_DIM_NULL_I8);
             if (ok) m |= 1ull << preds[i].q;
             const unsigned long long b = __ballot(ok);
             if ((threadIdx.x & 63) == 0 && b) atomicAdd(&pass[i], (uint32_t)__popcll(b));
@@ -898,6 +901,8 @@ struct BatchWorkspace {
     MaskPred *preds = nullptr;          // up to 4 x OLAP_BATCH_MAX predicates (one per query and table)
     uint32_t *pass = nullptr;           // their pass counts
     int32_t *minmax = nullptr;          // [2] device scratch of measure_range
+    int8_t *progvec = nullptr;          // pass vectors of the steps' general predicates
+    size_t progvec_cap = 0;
     const int32_t *range_col = nullptr; // the measure column whose range is kept
     int64_t range_rows = -1;
     int32_t range[2] = {0, 0};          // its min and max
@@ -914,6 +919,16 @@ struct BatchWorkspace {
         hipError_t e = hipMalloc(&mask[t], 8 * rows);
         if (e == hipSuccess) e = hipMalloc(&rec[t], 16 * rows);
         if (e == hipSuccess) dim_cap[t] = rows;
+        return e;
+    }
+    hipError_t fit_progvec(size_t bytes)
+    {
+        if (bytes <= progvec_cap) return hipSuccess;
+        if (progvec) (void)hipFree(progvec);
+        progvec = nullptr;
+        progvec_cap = 0;
+        hipError_t e = hipMalloc(&progvec, bytes);
+        if (e == hipSuccess) progvec_cap = bytes;
         return e;
     }
     hipError_t fit_fixed()
@@ -937,6 +952,7 @@ struct BatchWorkspace {
         if (preds) (void)hipFree(preds);
         if (pass) (void)hipFree(pass);
         if (minmax) (void)hipFree(minmax);
+        if (progvec) (void)hipFree(progvec);
     }
 };
 
@@ -1301,16 +1317,42 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
     // read-back of the pass counts and one synchronisation
     b->sel.assign((size_t)nq, 1.0);
     hipError_t e = ws->fit_fixed();
+    // general predicates: each step's program is evaluated once into an
+    // int8 pass vector of its table, which the mask kernel ANDs in
+    size_t prog_bytes = 0;
+    for (int q = 0; q < nq; q++)
+        for (int s = 0; s < queries[q].ndims; s++)
+            if (queries[q].program[s]) prog_bytes += ((size_t)star->dim_rows[queries[q].table[s]] + 15) & ~(size_t)15;
+    if (e == hipSuccess && prog_bytes) e = ws->fit_progvec(prog_bytes);
+    if (e != hipSuccess) {
+        delete b;
+        return bhip(ctx, e, "batch predicates");
+    }
     std::vector<MaskPred> preds;
     std::vector<int> pq;
     uint32_t pbeg[5] = {0, 0, 0, 0, 0};
+    size_t poff = 0;
     for (int t = 0; t < 4; t++) {
         pbeg[t] = (uint32_t)preds.size();
         if ((b->used >> t) & 1u)
             for (int q = 0; q < nq; q++)
                 for (int s = 0; s < queries[q].ndims; s++)
                     if (queries[q].table[s] == t) {
-                        preds.push_back({q, queries[q].lo[s], queries[q].hi[s], 0});
+                        const int8_t *pv = nullptr;
+                        if (const olap_dim_program *prog = queries[q].program[s]) {
+                            DevProgram dp;
+                            olap_status pst = prepare_program(ctx, prog, star->dim_rows[t], &dp, strm);
+                            if (pst != OLAP_OK) {
+                                delete b;
+                                return pst;
+                            }
+                            dp.group = OLAP_GROUP_NONE;  // only the pass matters here
+                            e = launch_dim_program(dp, star->dim_rows[t], 1, ws->progvec + poff,
+                                                   (uint32_t *)ctx->d_scratch, strm);
+                            pv = ws->progvec + poff;
+                            poff += ((size_t)star->dim_rows[t] + 15) & ~(size_t)15;
+                        }
+                        preds.push_back({q, queries[q].lo[s], queries[q].hi[s], 0, pv});
                         pq.push_back(q);
                     }
         if (e == hipSuccess && ((b->used >> t) & 1u)) e = ws->fit_table(t, (size_t)star->dim_rows[t]);

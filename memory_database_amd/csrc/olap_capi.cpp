@@ -639,6 +639,85 @@ olap_status agg_row_list(olap_ctx ctx, const uint32_t *oid, const int32_t *gid, 
     return st;
 }
 
+// Validate a dimension program and stage its IN values (sorted, one
+// device copy for all terms) on stream s.
+olap_status prepare_program(olap_ctx ctx, const olap_dim_program *prog, int64_t rows,
+                            DevProgram *out, hipStream_t s)
+{
+    if (!prog) return fail(ctx, OLAP_E_INVALID, "dimension program: null");
+    if (prog->nterms < 1 || prog->nterms > OLAP_MAX_TERMS)
+        return fail(ctx, OLAP_E_INVALID, "dimension program: 1..%d terms", OLAP_MAX_TERMS);
+    memset(out, 0, sizeof(*out));
+    out->nterms = prog->nterms;
+    out->group = prog->group;
+    out->group_mod = prog->group_mod;
+    out->group_term = -1;
+    out->group_col = prog->group_col;
+    if (prog->group < OLAP_GROUP_NONE || prog->group > OLAP_GROUP_HOP)
+        return fail(ctx, OLAP_E_INVALID, "dimension program: unknown group kind %d", prog->group);
+    if (prog->group == OLAP_GROUP_COLUMN && ((rows && !prog->group_col) || prog->group_mod < 0))
+        return fail(ctx, OLAP_E_INVALID, "dimension program: group column and group_mod >= 0 required");
+    if (prog->group == OLAP_GROUP_HOP) {
+        const int gt = prog->group_term;
+        if (gt < 0 || gt >= prog->nterms || prog->term[gt].kind != OLAP_TERM_HOP || prog->term[gt].negate)
+            return fail(ctx, OLAP_E_INVALID, "dimension program: group_term must name a HOP term that is not negated");
+        out->group_term = gt;
+    }
+    std::vector<int32_t> vals;
+    std::vector<size_t> voff(OLAP_MAX_TERMS, 0);
+    for (int i = 0; i < prog->nterms; i++) {
+        const olap_term &t = prog->term[i];
+        DevTerm &d = out->term[i];
+        d.kind = t.kind;
+        d.combine = t.combine;
+        d.negate = t.negate ? 1 : 0;
+        d.lo = t.lo;
+        d.hi = t.hi;
+        d.key_bias = t.key_bias;
+        d.col = t.col;
+        if (t.combine != OLAP_AND && t.combine != OLAP_OR)
+            return fail(ctx, OLAP_E_INVALID, "term %d: combine must be OLAP_AND or OLAP_OR", i);
+        if (rows && !t.col) return fail(ctx, OLAP_E_INVALID, "term %d: null column", i);
+        if (t.kind == OLAP_TERM_RANGE) {
+            continue;
+        } else if (t.kind == OLAP_TERM_IN) {
+            if (t.nvalues < 1 || t.nvalues > OLAP_MAX_IN_VALUES || !t.values)
+                return fail(ctx, OLAP_E_INVALID, "term %d: IN needs 1..%d host values", i, OLAP_MAX_IN_VALUES);
+            std::vector<int32_t> v(t.values, t.values + t.nvalues);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            voff[(size_t)i] = vals.size();
+            d.nvalues = (int32_t)v.size();
+            vals.insert(vals.end(), v.begin(), v.end());
+        } else if (t.kind == OLAP_TERM_HOP) {
+            if ((t.parent_code_bytes != 1 && t.parent_code_bytes != 4) || t.parent_rows < 0 ||
+                (t.parent_rows > 0 && !t.parent))
+                return fail(ctx, OLAP_E_INVALID, "term %d: HOP needs a parent vector of 1- or 4-byte codes", i);
+            d.parent = t.parent;
+            d.parent_rows = t.parent_rows;
+            d.parent_code_bytes = t.parent_code_bytes;
+        } else {
+            return fail(ctx, OLAP_E_INVALID, "term %d: unknown kind %d", i, t.kind);
+        }
+    }
+    if (!vals.empty()) {
+        if (vals.size() > ctx->in_vals_len) {
+            HIPC(ctx, hipStreamSynchronize(s));  // earlier programs on s may still read the old copy
+            dev_free(ctx->in_vals);
+            ctx->in_vals = nullptr;
+            ctx->in_vals_len = 0;
+            HIPC(ctx, hipMalloc(&ctx->in_vals, vals.size() * 4));
+            ctx->in_vals_len = vals.size();
+        }
+        HIPC(ctx, hipMemcpyAsync(ctx->in_vals, vals.data(), vals.size() * 4, hipMemcpyHostToDevice, s));
+        for (int i = 0; i < prog->nterms; i++)
+            if (prog->term[i].kind == OLAP_TERM_IN) out->term[i].values = ctx->in_vals + voff[(size_t)i];
+        // the host copy must outlive the upload
+        HIPC(ctx, hipStreamSynchronize(s));
+    }
+    return OLAP_OK;
+}
+
 }  // namespace olap
 
 extern "C" {
@@ -688,6 +767,7 @@ olap_status olap_ctx_destroy(olap_ctx ctx)
         dev_free(ctx->d_err);
         dev_free(ctx->d_scratch);
         dev_free(ctx->join_win);
+        dev_free(ctx->in_vals);
     }
     delete ctx;
     return OLAP_OK;
@@ -841,6 +921,57 @@ olap_status olap_build_dimvec(olap_ctx ctx, const int32_t *filter, const int32_t
     HIPC(ctx, launch_build_dimvec(filter, group, rows, lo, hi, bitmap, group_mod, code_bytes, out,
                                   as_stream(stream)));
     return OLAP_OK;
+}
+
+olap_status olap_build_dimvec_program(olap_ctx ctx, int64_t rows, const olap_dim_program *prog,
+                                      int32_t code_bytes, void *out, void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (rows < 0 || (rows && !out)) return fail(ctx, OLAP_E_INVALID, "build_dimvec_program: bad arguments");
+    if (code_bytes != 1 && code_bytes != 4)
+        return fail(ctx, OLAP_E_INVALID, "build_dimvec_program: code_bytes must be 1 or 4");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    DevProgram dp;
+    olap_status st = prepare_program(ctx, prog, rows, &dp, s);
+    if (st != OLAP_OK) return st;
+    uint32_t *derr = (uint32_t *)ctx->d_scratch;
+    HIPC(ctx, launch_zero_u32(derr, 1, s));
+    HIPC(ctx, launch_dim_program(dp, rows, code_bytes, out, derr, s));
+    uint32_t herr = 0;
+    HIPC(ctx, hipMemcpyAsync(&herr, derr, 4, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    if (herr)
+        return fail(ctx, OLAP_E_INVALID, "build_dimvec_program: %u passing rows have codes outside 0..126 "
+                                         "for an int8 vector", herr);
+    return OLAP_OK;
+}
+
+olap_status olap_dict_encode(const char *const *strs, int64_t n, int32_t *codes, int64_t *ndistinct)
+{
+    if (n < 0 || (n && (!strs || !codes))) return OLAP_E_INVALID;
+    for (int64_t i = 0; i < n; i++)
+        if (!strs[i]) return OLAP_E_INVALID;
+    std::vector<int64_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; i++) idx[(size_t)i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int64_t a, int64_t b) { return strcmp(strs[a], strs[b]) < 0; });
+    int64_t rank = -1;
+    for (size_t j = 0; j < idx.size(); j++) {
+        if (j == 0 || strcmp(strs[idx[j - 1]], strs[idx[j]]) != 0) rank++;
+        if (rank > INT32_MAX) return OLAP_E_CAPACITY;
+        codes[idx[j]] = (int32_t)rank;
+    }
+    if (ndistinct) *ndistinct = rank + 1;
+    return OLAP_OK;
+}
+
+int32_t olap_dict_code_of(const char *const *strs, const int32_t *codes, int64_t n, const char *value)
+{
+    if (!strs || !codes || !value) return -1;
+    for (int64_t i = 0; i < n; i++)
+        if (strs[i] && strcmp(strs[i], value) == 0) return codes[i];
+    return -1;
 }
 
 olap_status olap_query_create(olap_ctx ctx, const olap_plan *plan, const olap_columns *cols,

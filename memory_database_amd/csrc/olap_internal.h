@@ -27,6 +27,21 @@ constexpr uint32_t kLdsListBytes = kTileV * 2 + kTileV * 4 + 16;  // u16 off + u
 constexpr size_t kNoallocScratchBytes = size_t(8) << 20;
 constexpr size_t kNoallocHeader = 256;
 
+// A dimension program as the kernel reads it (olap_dim_program with IN
+// values in a sorted device copy).
+struct DevTerm {
+    int32_t kind, combine, negate, lo, hi, nvalues, parent_code_bytes, key_bias;
+    const int32_t *col;
+    const int32_t *values;
+    const void *parent;
+    int64_t parent_rows;
+};
+struct DevProgram {
+    int32_t nterms, group, group_mod, group_term;
+    const int32_t *group_col;
+    DevTerm term[OLAP_MAX_TERMS];
+};
+
 struct ScanShape {
     int bs;      // threads per workgroup
     int unroll;  // 128-bit chunks per thread per tile
@@ -110,6 +125,9 @@ struct olap_ctx_s {
     // join build: the row claiming each key, grown on demand
     int32_t *join_win = nullptr;
     size_t join_win_len = 0;
+    // dimension programs: sorted IN values, grown on demand
+    int32_t *in_vals = nullptr;
+    size_t in_vals_len = 0;
 };
 
 struct olap_query_s {
@@ -205,6 +223,14 @@ hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t 
                                const int32_t *m2, unsigned long long *gv, int64_t G,
                                unsigned long long *total, uint32_t *bad, int num_cus,
                                hipStream_t s);
+// dimension programs (olap_predicate.hip); err counts rows whose code does
+// not fit an int8 vector
+hipError_t launch_dim_program(const DevProgram &p, int64_t rows, int32_t code_bytes, void *out,
+                              uint32_t *err, hipStream_t s);
+// validate a program and stage its IN values on stream s (olap_capi.cpp);
+// the staged values stay valid until the next call on the context
+olap_status prepare_program(olap_ctx ctx, const olap_dim_program *prog, int64_t rows,
+                            DevProgram *out, hipStream_t s);
 // dense-vector join sub-step (olap_join.hip); winner: vlen int32 of scratch
 hipError_t launch_join_build(int64_t rows, const int32_t *key, const int32_t *payload,
                              int32_t *winner, int32_t *vec, int64_t vlen, uint32_t *bad,

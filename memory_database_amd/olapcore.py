@@ -57,6 +57,71 @@ class Columns(C.Structure):
                 ("rows", C.c_int64)]
 
 
+MAX_TERMS = 8
+MAX_IN_VALUES = 4096
+TERM_RANGE, TERM_IN, TERM_HOP = range(3)
+AND, OR = range(2)
+GROUP_NONE, GROUP_COLUMN, GROUP_HOP = range(3)
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("combine", C.c_int32), ("negate", C.c_int32),
+                ("lo", C.c_int32), ("hi", C.c_int32), ("nvalues", C.c_int32),
+                ("parent_code_bytes", C.c_int32), ("key_bias", C.c_int32),
+                ("col", C.c_void_p), ("values", C.c_void_p), ("parent", C.c_void_p),
+                ("parent_rows", C.c_int64)]
+
+
+class DimProgram(C.Structure):
+    """olap_dim_program. Build with DimProgram.of(terms, group=...): terms are
+    ("range", col, lo, hi), ("in", col, values), ("hop", col, parent_vector,
+    key_bias) tuples, each optionally preceded by "and"/"or" (combine with the
+    terms before, default "and") and "not" (negate)."""
+    _fields_ = [("nterms", C.c_int32), ("group", C.c_int32), ("group_mod", C.c_int32),
+                ("group_term", C.c_int32), ("group_col", C.c_void_p),
+                ("term", Term * MAX_TERMS)]
+
+    @classmethod
+    def of(cls, terms, group=GROUP_NONE, group_col=None, group_mod=0, group_term=0):
+        import numpy as np
+        p = cls()
+        p.nterms = len(terms)
+        p.group, p.group_mod, p.group_term = group, group_mod, group_term
+        p.group_col = _ptr(group_col)
+        keep = [group_col]
+        for i, t in enumerate(terms):
+            t = list(t)
+            comb, neg = AND, 0
+            while t[0] in ("and", "or", "not"):
+                w = t.pop(0)
+                if w == "not":
+                    neg = 1
+                else:
+                    comb = AND if w == "and" else OR
+            x = p.term[i]
+            x.combine, x.negate = comb, neg
+            kind = t[0]
+            x.col = _ptr(t[1])
+            keep.append(t[1])
+            if kind == "range":
+                x.kind, x.lo, x.hi = TERM_RANGE, int(t[2]), int(t[3])
+            elif kind == "in":
+                v = np.ascontiguousarray(t[2], dtype=np.int32)
+                keep.append(v)
+                x.kind, x.values, x.nvalues = TERM_IN, v.ctypes.data, v.size
+            elif kind == "hop":
+                parent = t[2]
+                keep.append(parent)
+                x.kind, x.parent = TERM_HOP, _ptr(parent)
+                x.parent_rows = parent.numel()
+                x.parent_code_bytes = parent.element_size()
+                x.key_bias = int(t[3]) if len(t) > 3 else 0
+            else:
+                raise ValueError(f"unknown term kind {kind}")
+        p._keep = keep
+        return p
+
+
 class BatchQuery(C.Structure):
     _fields_ = [("ndims", C.c_int32),
                 ("table", C.c_int32 * MAX_DIMS),
@@ -64,17 +129,24 @@ class BatchQuery(C.Structure):
                 ("hi", C.c_int32 * MAX_DIMS),
                 ("group_mod", C.c_int32 * MAX_DIMS),
                 ("factor", C.c_int32 * MAX_DIMS),
-                ("group_nums", C.c_int64)]
+                ("group_nums", C.c_int64),
+                ("program", C.POINTER(DimProgram) * MAX_DIMS)]
 
     @classmethod
-    def make(cls, steps, factor, group_nums):
-        """steps: [(table, lo, hi, group_mod)] in probe order."""
+    def make(cls, steps, factor, group_nums, programs=None):
+        """steps: [(table, lo, hi, group_mod)] in probe order; programs:
+        optional DimProgram (or None) per step, ANDed with the range."""
         q = cls()
         q.ndims = len(steps)
         for i, (t, lo, hi, g) in enumerate(steps):
             q.table[i], q.lo[i], q.hi[i], q.group_mod[i] = t, lo, hi, g
             q.factor[i] = factor[i]
         q.group_nums = group_nums
+        if programs:
+            q._programs = list(programs)
+            for i, pr in enumerate(programs):
+                if pr is not None:
+                    q.program[i] = C.pointer(pr)
         return q
 
     def as_dict(self):
@@ -141,6 +213,10 @@ SIGNATURES = {
     "olap_make_plan": (C.c_int, [_P, _P, _P, C.POINTER(Plan)]),
     "olap_build_dimvec": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int32, _P, _P]),
+    "olap_build_dimvec_program": (C.c_int, [_P, C.c_int64, C.POINTER(DimProgram), C.c_int32, _P,
+                                            _P]),
+    "olap_dict_encode": (C.c_int, [_P, C.c_int64, _P, _I64P]),
+    "olap_dict_code_of": (C.c_int32, [_P, _P, C.c_int64, C.c_char_p]),
     "olap_query_create": (C.c_int, [_P, C.POINTER(Plan), C.POINTER(Columns), C.POINTER(_P)]),
     "olap_query_run": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "olap_query_destroy": (C.c_int, [_P]),
@@ -285,6 +361,28 @@ def config_info(cfg: int):
     return th.value, rp.value, kib.value, st.value
 
 
+def dict_encode(strings):
+    """olap_dict_encode: int32 codes (rank among the distinct strings in byte
+    order) and the number of distinct strings."""
+    import numpy as np
+    enc = [s.encode() if isinstance(s, str) else bytes(s) for s in strings]
+    arr = (C.c_char_p * max(len(enc), 1))(*enc)
+    codes = np.zeros(max(len(enc), 1), dtype=np.int32)
+    nd = C.c_int64()
+    st = load().olap_dict_encode(arr, len(enc), codes.ctypes.data, C.byref(nd))
+    if st != OK:
+        raise OlapError(st, "dict_encode")
+    return codes[: len(enc)], nd.value
+
+
+def dict_code_of(strings, codes, value: str) -> int:
+    import numpy as np
+    enc = [s.encode() if isinstance(s, str) else bytes(s) for s in strings]
+    arr = (C.c_char_p * max(len(enc), 1))(*enc)
+    c = np.ascontiguousarray(codes, dtype=np.int32)
+    return int(load().olap_dict_code_of(arr, c.ctypes.data, len(enc), value.encode()))
+
+
 def zipf_table(d: int, theta: float):
     import numpy as np
     thr = np.zeros(d, dtype=np.uint64)
@@ -350,6 +448,13 @@ class Context:
         self.check(self.lib.olap_build_dimvec(self.h, _ptr(filter_col), _ptr(group_col),
                                               filter_col.numel(), lo, hi, bitmap, group_mod, cb,
                                               _ptr(out), stream), "build_dimvec")
+        return out
+
+    def build_dimvec_program(self, prog: "DimProgram", rows: int, out, stream=None):
+        """Dimension vector of a general predicate program (int8 or int32 out)."""
+        self.check(self.lib.olap_build_dimvec_program(self.h, rows, C.byref(prog),
+                                                      out.element_size(), _ptr(out), stream),
+                   "build_dimvec_program")
         return out
 
     # ---- star join ---------------------------------------------------

@@ -208,6 +208,85 @@ void orc_build_dimvec_i32(const int32_t *filter, const int32_t *group, int64_t r
     }
 }
 
+/* General dimension predicates. The reference chains select nodes over one
+ * table, each writing a result vector that the next ANDs or ORs in
+ * (code/TPCH_Q5_operator/q5_operator.cpp:1288-1550: less / less_equal /
+ * more / more_equal / equal / non_equal value tests, string equality, and
+ * equal_col, a lookup of the row's key in another table's result vector,
+ * :1472-1506; Select_Option runs the chain, :1551-1596), and builds
+ * snowflake dimension vectors through a second table
+ * (code/TPCH_Q5/Q5.cpp:921-944: supplier -> nation -> region, and orders ->
+ * customer -> nation with a date range). Restated: a term is a range
+ * lo <= v < hi (each value test is one: v < c is [INT_MIN, c), v == c is
+ * [c, c+1), v != c its negation), an IN list (string equality over
+ * dictionary codes), or a hop parent[col + key_bias] != NULL; terms combine
+ * left to right with AND / OR. Deliberate difference: the reference's OR
+ * nodes keep a row when the previous node stored 0 (pre_bmp == 0,
+ * :1329-1331), which is the previous node's pass flag only for string tests;
+ * here OR is the boolean OR of the passes. The group code is 0, a group
+ * column (mod group_mod), or the parent code of a hop term (equal_col_storecol2,
+ * :1487-1506, and DimVec_S/o storing the nation key). */
+static int orc_in_list(const int32_t *v, int32_t n, int32_t x)
+{
+    for (int32_t i = 0; i < n; i++)
+        if (v[i] == x) return 1;
+    return 0;
+}
+
+int64_t orc_build_dimvec_program(const orc_program *p, int64_t rows, int32_t code_bytes, void *out)
+{
+    int64_t bad = 0;
+    for (int64_t r = 0; r < rows; r++) {
+        int acc = 0, gfound = 0;
+        int32_t gcode = 0;
+        for (int i = 0; i < p->nterms; i++) {
+            const orc_term *t = &p->term[i];
+            int ok;
+            if (t->kind == ORC_TERM_RANGE) {
+                ok = t->col[r] >= t->lo && t->col[r] < t->hi;
+            } else if (t->kind == ORC_TERM_IN) {
+                ok = orc_in_list(t->values, t->nvalues, t->col[r]);
+            } else {
+                int64_t k = (int64_t)t->col[r] + t->key_bias;
+                int32_t code = -1;
+                if (k >= 0 && k < t->parent_rows) {
+                    if (t->parent_code_bytes == 1) {
+                        int8_t c = ((const int8_t *)t->parent)[k];
+                        code = c == ORC_DIM_NULL_I8 ? -1 : c;
+                    } else {
+                        code = ((const int32_t *)t->parent)[k];
+                    }
+                }
+                ok = code != -1;
+                if (i == p->group_term && p->group == ORC_GROUP_HOP) {
+                    gfound = ok;
+                    gcode = code;
+                }
+            }
+            if (t->negate) ok = !ok;
+            acc = i == 0 ? ok : (t->combine == ORC_OR ? (acc || ok) : (acc && ok));
+        }
+        int32_t code = 0;
+        if (p->group == ORC_GROUP_COLUMN) {
+            int32_t g = p->group_col[r];
+            code = p->group_mod > 0 ? (int32_t)((uint32_t)g % (uint32_t)p->group_mod) : g;
+        } else if (p->group == ORC_GROUP_HOP) {
+            acc = acc && gfound;
+            code = gcode;
+        }
+        if (code_bytes == 1) {
+            if (acc && (code < 0 || code >= ORC_DIM_NULL_I8)) {
+                bad++;
+                acc = 0;
+            }
+            ((int8_t *)out)[r] = acc ? (int8_t)code : (int8_t)ORC_DIM_NULL_I8;
+        } else {
+            ((int32_t *)out)[r] = acc ? code : ORC_DIM_NULL_I32;
+        }
+    }
+    return bad;
+}
+
 /* ------------------------------------------------------------------------ */
 /* shared helpers                                                            */
 /* ------------------------------------------------------------------------ */

@@ -84,6 +84,24 @@ int orc_make_plan(const int32_t groups[4], const int32_t bitmap[4],
 /* ---- filter -> dimension vector (north-star step 1) */
 void orc_build_dimvec_i8(const int32_t *filter, const int32_t *group, int64_t rows,
                          int32_t lo, int32_t hi, int32_t bitmap, int8_t *out);
+/* general dimension predicates: terms combined left to right */
+enum { ORC_TERM_RANGE = 0, ORC_TERM_IN = 1, ORC_TERM_HOP = 2 };
+enum { ORC_AND = 0, ORC_OR = 1 };
+enum { ORC_GROUP_NONE = 0, ORC_GROUP_COLUMN = 1, ORC_GROUP_HOP = 2 };
+typedef struct {
+    int32_t kind, combine, negate, lo, hi, nvalues, parent_code_bytes, key_bias;
+    const int32_t *col;       /* host */
+    const int32_t *values;    /* IN, any order */
+    const void *parent;       /* HOP */
+    int64_t parent_rows;
+} orc_term;
+typedef struct {
+    int32_t nterms, group, group_mod, group_term;
+    const int32_t *group_col;
+    orc_term term[8];
+} orc_program;
+/* returns the rows whose code does not fit an int8 vector (written NULL) */
+int64_t orc_build_dimvec_program(const orc_program *p, int64_t rows, int32_t code_bytes, void *out);
 void orc_build_dimvec_i32(const int32_t *filter, const int32_t *group, int64_t rows,
                           int32_t lo, int32_t hi, int32_t bitmap, int32_t group_mod,
                           int32_t *out);

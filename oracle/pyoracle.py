@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
         L.orc_agg_expr.restype = C.c_int64
         L.orc_agg_expr.argtypes = [C.c_int64, P, P, P]
         L.orc_join_build.argtypes = [C.c_int64, P, P, P, C.c_int64, P]
+        L.orc_build_dimvec_program.restype = C.c_int64
+        L.orc_build_dimvec_program.argtypes = [P, C.c_int64, C.c_int32, P]
         L.orc_join_probe.restype = C.c_int64
         L.orc_join_probe.argtypes = [C.c_int64, P, P, C.c_int64, P]
         L.orc_available_cpus.restype = C.c_int
@@ -265,6 +267,63 @@ def vector_group(key1, m1, m2, group_num):
     bad = C.c_int64()
     tot = lib().orc_vector_group(k.size, _p(k), _p(a), _p(b), _p(gv), group_num, C.byref(bad))
     return int(tot), gv, bad.value
+
+
+class _OrcTerm(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("combine", C.c_int32), ("negate", C.c_int32),
+                ("lo", C.c_int32), ("hi", C.c_int32), ("nvalues", C.c_int32),
+                ("parent_code_bytes", C.c_int32), ("key_bias", C.c_int32),
+                ("col", C.c_void_p), ("values", C.c_void_p), ("parent", C.c_void_p),
+                ("parent_rows", C.c_int64)]
+
+
+class _OrcProgram(C.Structure):
+    _fields_ = [("nterms", C.c_int32), ("group", C.c_int32), ("group_mod", C.c_int32),
+                ("group_term", C.c_int32), ("group_col", C.c_void_p), ("term", _OrcTerm * 8)]
+
+
+def build_dimvec_program(terms, rows, code_bytes=1, group=0, group_col=None, group_mod=0,
+                         group_term=0):
+    """The oracle's general-predicate dimension vector. terms as
+    DimProgram.of takes them, over numpy columns: ("range", col, lo, hi),
+    ("in", col, values), ("hop", col, parent, key_bias), optionally preceded
+    by "and" / "or" / "not". Returns (vector, rows refused for int8)."""
+    p = _OrcProgram()
+    p.nterms, p.group, p.group_mod, p.group_term = len(terms), group, group_mod, group_term
+    keep = []
+    if group_col is not None:
+        gc = np.ascontiguousarray(group_col, dtype=np.int32)
+        keep.append(gc)
+        p.group_col = _p(gc)
+    for i, t in enumerate(terms):
+        t = list(t)
+        comb, neg = 0, 0
+        while t[0] in ("and", "or", "not"):
+            w = t.pop(0)
+            if w == "not":
+                neg = 1
+            else:
+                comb = 0 if w == "and" else 1
+        x = p.term[i]
+        x.combine, x.negate = comb, neg
+        col = np.ascontiguousarray(t[1], dtype=np.int32)
+        keep.append(col)
+        x.col = _p(col)
+        if t[0] == "range":
+            x.kind, x.lo, x.hi = 0, int(t[2]), int(t[3])
+        elif t[0] == "in":
+            v = np.ascontiguousarray(t[2], dtype=np.int32)
+            keep.append(v)
+            x.kind, x.values, x.nvalues = 1, _p(v), v.size
+        else:
+            par = np.ascontiguousarray(t[2])
+            keep.append(par)
+            x.kind, x.parent, x.parent_rows = 2, _p(par), par.size
+            x.parent_code_bytes = par.dtype.itemsize
+            x.key_bias = int(t[3]) if len(t) > 3 else 0
+    out = np.empty(rows, dtype=np.int8 if code_bytes == 1 else np.int32)
+    bad = lib().orc_build_dimvec_program(C.byref(p), rows, code_bytes, _p(out))
+    return out, int(bad)
 
 
 def join_build(key, payload, vector):
