@@ -83,7 +83,7 @@ def cpu_baseline(ds, sample_rows, log_fn=log):
     orders = list(plan.orders)[: plan.dimvec_nums]
     factor = list(plan.factor)[: plan.dimvec_nums]
     kind = "port"
-    ref = po.Reference()
+    ref = po.Reference(avx512=True)
     allowed = os.sched_getaffinity(0)
     use_ref = ref.ok and allowed == set(range(len(allowed)))
     if use_ref:
@@ -115,10 +115,100 @@ def cpu_baseline(ds, sample_rows, log_fn=log):
         ts.append(time.perf_counter() - t0)
     t = statistics.median(ts)
     log_fn(f"cpu baseline ({kind}, {cores} threads): {n} rows in {t * 1e3:.1f} ms")
+    build = ("reference sources built -O3 with the reference's AVX-512 flags (its makefile "
+             "builds -O0)" if kind == "reference" and ref.avx512 else
+             "reference sources built -O2 (this host lacks AVX-512; the reference builds -O0)"
+             if kind == "reference" else "C restatement -O3")
     return {"value": n / t, "unit": "fact rows/s", "cores": cores, "kind": kind,
-            "sample": f"first {n} fact rows of the workload, OLAPcore_vwm_dv "
-                      f"({'reference sources built -O2 (the reference builds -O0)' if kind == 'reference' else 'C restatement -O3'}), "
+            "sample": f"first {n} fact rows of the workload, OLAPcore_vwm_dv ({build}), "
                       f"median of 5 runs after 1 warmup, {cores} pinned threads"}
+
+
+def cpu_configs(ctx, stream, c4_rows=60_000_000):
+    """The CPU path on the other configurations, on this host's cores: the
+    1-dimension 6M-row star and the 4-dimension 60M-row star in full through
+    the reference's own OLAPcore_vwm_dv and OLAPcore_rowwise drivers, and the
+    64 batched queries run one after the other by the C restatement over the
+    first c4_rows rows of the 600M-row star (their group codes exceed the
+    reference's int8 vectors)."""
+    import numpy as np
+    import torch
+
+    from memory_database_amd import workloads as wl
+    from oracle import pyoracle as po
+
+    cores = po.available_cpus()
+    ref = po.Reference(avx512=True)
+    use_ref = ref.ok and os.sched_getaffinity(0) == set(range(len(os.sched_getaffinity(0))))
+    out = {}
+    for cfg in (wl.STAR_1DIM_6M, wl.STAR_4DIM_60M):
+        ds = wl.DeviceStar(ctx, cfg, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        dvs, fks = [None] * 4, [None] * 4
+        for d in cfg.dims:
+            dvs[d.slot] = ds.dimvecs[d.slot].cpu().numpy()
+            fks[d.slot] = ds.fks[d.slot].cpu().numpy()
+        m1 = ds.m1.cpu().numpy()
+        m2 = np.zeros_like(m1)
+        plan = ds.plan
+        orders = list(plan.orders)[: plan.dimvec_nums]
+        factor = list(plan.factor)[: plan.dimvec_nums]
+        del ds
+        row = {"rows": cfg.rows, "cores": cores}
+        for strat in ("vwm_dv", "rowwise"):
+            if use_ref:
+                dvr = [np.concatenate([d, np.array([127], np.int8)]) if d is not None
+                       else np.full(1, 127, np.int8) for d in dvs]
+                fkr = [f if f is not None else np.zeros(1, np.int32) for f in fks]
+                fn = getattr(ref, "olapcore_" + strat)
+
+                def run():
+                    return fn(dvr, fkr, orders, factor, int(plan.group_nums), m1, m2, cores)
+            else:
+                star = po.Star(dvs, fks, m1, None)
+                p = po.plan(orders, factor, int(plan.group_nums))
+
+                def run():
+                    return po.run(strat, p, star, nthreads=cores)
+            run()
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                run()
+                ts.append(time.perf_counter() - t0)
+            t = statistics.median(ts)
+            row[strat] = {"ms": t * 1e3, "rows_per_s": cfg.rows / t}
+        row["kind"] = "reference" if use_ref else "port"
+        row["build"] = ("-O3 + the reference's AVX-512 flags" if use_ref and ref.avx512 else "-O2" if use_ref
+                        else "C restatement -O3")
+        log(f"cpu {cfg.name}: {row}")
+        out[cfg.name] = row
+    # C4: 64 queries in sequence (the plain scans' CPU equivalent)
+    cfg = wl.scaled(wl.STAR_4DIM_600M, c4_rows, name=f"c4_sample_{c4_rows}")
+    bs = wl.DeviceBatchStar(ctx, cfg)
+    torch.cuda.synchronize()
+    filt = [None if f is None else f.cpu().numpy() for f in bs.filters]
+    attr = [None if a is None else a.cpu().numpy() for a in bs.attrs]
+    fks = [None if f is None else f.cpu().numpy() for f in bs.fks]
+    m1 = bs.m1.cpu().numpy()
+    del bs
+    qs = wl.batch_queries(wl.STAR_4DIM_600M, 64, seed=7)
+    t_all = 0.0
+    for steps, factor, G in qs:
+        dvs = [None] * 4
+        for (t, lo, hi, g) in steps:
+            dvs[t] = po.build_dimvec(filt[t], attr[t], lo, hi, 1 if g > 0 else 0, 4, group_mod=g)
+        p = po.plan([s[0] for s in steps], factor, G)
+        star = po.Star(dvs, fks, m1, None)
+        t0 = time.perf_counter()
+        po.run("rowwise", p, star, nthreads=cores)
+        t_all += time.perf_counter() - t0
+    out["batch_64q_sequential"] = {"rows": c4_rows, "queries": 64, "ms": t_all * 1e3,
+                                   "qps": 64 / t_all, "cores": cores, "kind": "port",
+                                   "note": "C restatement (row-wise, int32 vectors), one query after "
+                                           f"another over the first {c4_rows} rows of the 600M-row star"}
+    log(f"cpu 64 queries over {c4_rows} rows: {t_all * 1e3:.0f} ms")
+    return out
 
 
 def source_digest() -> str:
@@ -337,6 +427,95 @@ def bench_served(ctx, ds, rates, seconds, streams=2):
             "latency": "scheduled arrival to result on the host (submit lag + server time)"}
 
 
+def bench_q5(ctx, stream, sf=100.0, region="ASIA", date=19940101):
+    """TPC-H Q5 over a TPC-H-shaped snowflake at scale factor sf (6M x sf
+    lineitems): the dimension vectors built with general predicates, then
+    the fused probe (Tri_table_join), timed apart."""
+    import torch
+
+    from memory_database_amd import workloads as wl
+
+    sh = stream.cuda_stream
+    cfg = wl.Q5Config(sf)
+    with torch.cuda.stream(stream):
+        dq = wl.DeviceQ5(ctx, cfg, stream=sh)
+    torch.cuda.synchronize()
+    out_s = torch.empty(25, dtype=torch.int64, device="cuda")
+    out_c = torch.empty(25, dtype=torch.int64, device="cuda")
+    vecs = {}
+
+    def build():
+        vecs["v"] = dq.vectors(ctx, region, date, code_bytes=1, stream=sh)
+
+    _, evb = time_steps(build, 5, 1, stream)
+    sv, ov = vecs["v"]
+    _, evp = time_steps(lambda: dq.run(ctx, sv, ov, out_s, out_c, stream=sh), 10, 2, stream)
+    ms = statistics.mean(evp)
+    alg = dq.fact_bytes()
+    rows = cfg.lineitems
+    nat = {n: int(c) for n, c in zip(range(25), out_c.cpu().tolist()) if c}
+    log(f"Q5 sf={sf:g}: vectors {statistics.mean(evb):.3f} ms, probe {ms:.3f} ms over {rows} lineitems "
+        f"({rows / ms / 1e6:.1f} G rows/s), {sum(nat.values())} matching rows in {len(nat)} nations")
+    return {"sf": sf, "lineitems": rows, "region": region, "date": date,
+            "vectors_ms": statistics.mean(evb), "probe_ms": ms, "rows_per_s": rows / (ms / 1e3),
+            "effective_GBps": alg / (ms / 1e3) / 1e9, "algorithmic_bytes": alg,
+            "matching_rows": sum(nat.values()), "nations": len(nat),
+            "revenue_units": "cents x percent (10^4 x the reference's double revenue)",
+            "timing": "HIP events, mean of 10 probes after 2 warmups; vectors mean of 5"}
+
+
+def bench_substeps(ctx, stream):
+    """The sub-step operators at the reference's own sizes: vector_group over
+    600M rows for 2^5..2^26 groups (code/group, log/group/groupalgo_test.tsv
+    times beside), the dense-vector join probe of 2^30 keys into 2^10..2^26
+    entries (code/join)."""
+    import torch
+
+    sh = stream.cuda_stream
+    rows = 600_000_000
+    key = torch.empty(rows, dtype=torch.int32, device="cuda")
+    m1 = torch.empty(rows, dtype=torch.int32, device="cuda")
+    m2 = torch.empty(rows, dtype=torch.int32, device="cuda")
+    ctx.gen_uniform(m1, 3, 1, 0, 1000, stream=sh)
+    ctx.gen_uniform(m2, 3, 2, 0, 1000, stream=sh)
+    # the reference's published single-thread times (ms, -O0), log/group/groupalgo_test.tsv
+    published = {5: 7077, 8: 2838, 10: 2746, 12: 2414, 16: 2457, 18: 3423, 20: 4292, 22: 5014,
+                 24: 11102, 26: 14438}
+    group = []
+    for k in sorted(published):
+        G = 1 << k
+        ctx.gen_uniform(key, 3, 10 + k, 1, G + 1, stream=sh)
+        out = torch.empty(G, dtype=torch.int64, device="cuda")
+        _, ev = time_steps(lambda: ctx.vector_group(key, m1, m2, out, stream=sh), 3, 1, stream)
+        ms = statistics.mean(ev)
+        group.append({"log2_groups": k, "ms": round(ms, 3), "rows_per_s": rows / (ms / 1e3),
+                      "reference_cpu_ms": published[k]})
+        log(f"vector_group 600M rows, 2^{k} groups: {ms:.3f} ms (reference CPU {published[k]} ms)")
+    del key, m1, m2
+    srows = 1 << 30
+    skey = torch.empty(srows, dtype=torch.int32, device="cuda")
+    join = []
+    for k in (10, 15, 20, 26):
+        maxid = 1 << k
+        ctx.gen_uniform(skey, 4, k, 1, maxid + 1, stream=sh)
+        rkey = torch.randperm(maxid, device="cuda", dtype=torch.int64).to(torch.int32) + 1
+        vec = torch.zeros(maxid + 1, dtype=torch.int32, device="cuda")
+        ctx.join_build(rkey, torch.ones_like(rkey), vec, stream=sh)
+        res = {}
+
+        def probe():
+            res["t"] = ctx.join_probe(skey, vec, stream=sh)
+
+        _, ev = time_steps(probe, 3, 1, stream)
+        ms = statistics.mean(ev)
+        assert res["t"] == (srows, 0)
+        join.append({"log2_build_rows": k, "probe_rows": srows, "ms": round(ms, 3),
+                     "rows_per_s": srows / (ms / 1e3)})
+        log(f"join probe 2^30 keys into 2^{k} entries: {ms:.3f} ms")
+    return {"vector_group": group, "join_probe": join,
+            "timing": "HIP events around each synchronising call, mean of 3 after 1 warmup"}
+
+
 def host_info():
     info = {}
     try:
@@ -405,6 +584,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the skew/groups/selectivity sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
+    ap.add_argument("--no-q5", action="store_true", help="skip TPC-H Q5 at scale factor 100")
+    ap.add_argument("--no-substeps", action="store_true", help="skip the sub-step operators")
     ap.add_argument("--serve-rates", default="50,500,1000,2000,3000,5000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -497,11 +678,24 @@ def main():
     if not args.no_serve:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         extra["served"] = bench_served(ctx, ds, rates, args.serve_seconds)
-
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, args.cpu_rows)
     q.close()
+    if not args.no_cpu and not args.no_extra:
+        del ds
+        torch.cuda.empty_cache()
+        ds = None
+        cpu["configs"] = cpu_configs(ctx, stream)
+    if not args.no_q5 or not args.no_substeps:
+        # the sub-steps and Q5 need room: the star's fact columns are done with
+        ds = None
+        torch.cuda.empty_cache()
+    if not args.no_q5:
+        extra["tpch_q5"] = bench_q5(ctx, stream)
+    if not args.no_substeps:
+        extra["substeps"] = bench_substeps(ctx, stream)
+
     kernel = "k_rowwise" if args.strategy == "rowwise" else None
     traffic, traffic_src, current = (profiled_traffic(kernel, cfg.name) if kernel
                                      else (None, None, None))

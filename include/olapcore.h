@@ -480,6 +480,31 @@ olap_status olap_join_build(olap_ctx ctx, int64_t rows, const int32_t *key, cons
  * entries is probed from LDS. Synchronises. */
 olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *vector,
                             int64_t vector_len, int64_t *total, int64_t *bad_keys, void *stream);
+/* TPC-H Q5's fused snowflake probe, Tri_table_join (code/TPCH_Q5/Q5.cpp:945-978,
+ * driver :1000-1020): per fact (lineitem) row i, s = dimvec[0][key[0][i] +
+ * key_bias[0]] (the supplier's nation code, NULL outside the region) and
+ * o = dimvec[1][key[1][i] + key_bias[1]] (the ordering customer's nation code,
+ * NULL outside the region or date range); the row counts when s is non-NULL
+ * and s == o, and sum[s] += price[i] * (100 - discount[i]), count[s] += 1.
+ * The reference sums l_extendedprice * (1 - l_discount) in double; here the
+ * price is in cents and the discount in percent, so the int64 sum is exactly
+ * 10^4 times the revenue. Outputs (device, group_nums int64 each) are zeroed
+ * by the call; codes outside [0, group_nums) are refused (OLAP_E_INVALID).
+ * Keys outside a vector find no row. Synchronises. */
+#define OLAP_TRI_MAX_GROUPS 1024
+typedef struct {
+    int64_t rows;                 /* fact rows */
+    const int32_t *key[2];        /* device: supplier key, order key per row */
+    int32_t key_bias[2];          /* added to the keys (e.g. -1 for 1-based keys) */
+    const void *dimvec[2];        /* device: supplier and order dimension vectors */
+    int64_t dim_rows[2];
+    int32_t code_bytes[2];        /* 1 (int8, NULL 127) or 4 (int32, NULL -1) */
+    const int32_t *price;         /* device, cents */
+    const int32_t *discount;      /* device, percent 0..100 */
+    int32_t group_nums;           /* 1..OLAP_TRI_MAX_GROUPS */
+} olap_tri_join;
+olap_status olap_tri_table_join(olap_ctx ctx, const olap_tri_join *j, int64_t *sum, int64_t *count,
+                                void *stream);
 /* aggalgo_rowwise (code/aggregation/test_aggregation_option.cpp:28-37) in fixed
  * point: SUM(p*q*t + 100*p*q) with t the tax in percent; host result. */
 olap_status olap_agg_expr(olap_ctx ctx, int64_t rows, const int32_t *p, const int32_t *q,

@@ -22,7 +22,7 @@ HARNESS_PATH = os.path.join(ROOT, "tests", "native", "compat_harness")
 ARCH = os.environ.get("OLAP_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["olap_kernels.hip", "olap_stage.hip", "olap_batch.hip", "olap_noalloc.hip",
-           "olap_join.hip", "olap_predicate.hip",
+           "olap_join.hip", "olap_predicate.hip", "olap_snowflake.hip",
            "olap_capi.cpp",
            "olapcore_compat.cpp",
            "olap_server.cpp"]

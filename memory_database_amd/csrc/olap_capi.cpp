@@ -1508,6 +1508,41 @@ olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, cons
     return OLAP_OK;
 }
 
+olap_status olap_tri_table_join(olap_ctx ctx, const olap_tri_join *j, int64_t *sum, int64_t *count,
+                                void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (!j || !sum || !count) return fail(ctx, OLAP_E_INVALID, "tri_table_join: null argument");
+    if (j->group_nums < 1 || j->group_nums > OLAP_TRI_MAX_GROUPS)
+        return fail(ctx, OLAP_E_CAPACITY, "tri_table_join: group_nums must be in [1, %d]", OLAP_TRI_MAX_GROUPS);
+    if (j->rows < 0) return fail(ctx, OLAP_E_INVALID, "tri_table_join: rows < 0");
+    for (int d = 0; d < 2; d++) {
+        if (j->code_bytes[d] != 1 && j->code_bytes[d] != 4)
+            return fail(ctx, OLAP_E_INVALID, "tri_table_join: code_bytes must be 1 or 4");
+        if (j->dim_rows[d] < 0 || (j->dim_rows[d] && !j->dimvec[d]) || (j->rows && !j->key[d]))
+            return fail(ctx, OLAP_E_INVALID, "tri_table_join: keys and dimension vectors required");
+    }
+    if (j->rows && (!j->price || !j->discount))
+        return fail(ctx, OLAP_E_INVALID, "tri_table_join: price and discount columns required");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = as_stream(stream);
+    uint32_t *dbad = (uint32_t *)ctx->d_scratch;
+    HIPC(ctx, launch_zero_u64((unsigned long long *)sum, (size_t)j->group_nums, s));
+    HIPC(ctx, launch_zero_u64((unsigned long long *)count, (size_t)j->group_nums, s));
+    HIPC(ctx, launch_zero_u32(dbad, 1, s));
+    HIPC(ctx, launch_tri_join(j->key[0], j->key_bias[0], j->dimvec[0], j->dim_rows[0], j->code_bytes[0],
+                              j->key[1], j->key_bias[1], j->dimvec[1], j->dim_rows[1], j->code_bytes[1],
+                              j->price, j->discount, j->rows, j->group_nums,
+                              (unsigned long long *)sum, (unsigned long long *)count, dbad,
+                              ctx->num_cus, s));
+    uint32_t hbad = 0;
+    HIPC(ctx, hipMemcpyAsync(&hbad, dbad, 4, hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    if (hbad)
+        return fail(ctx, OLAP_E_INVALID, "tri_table_join: %u matching rows carry a code outside group_nums", hbad);
+    return OLAP_OK;
+}
+
 olap_status olap_agg_expr(olap_ctx ctx, int64_t rows, const int32_t *p, const int32_t *q,
                           const int32_t *t, int64_t *result, void *stream)
 {

@@ -231,6 +231,12 @@ hipError_t launch_dim_program(const DevProgram &p, int64_t rows, int32_t code_by
 // the staged values stay valid until the next call on the context
 olap_status prepare_program(olap_ctx ctx, const olap_dim_program *prog, int64_t rows,
                             DevProgram *out, hipStream_t s);
+// TPC-H Q5's fused snowflake probe (olap_snowflake.hip)
+hipError_t launch_tri_join(const int32_t *skey, int32_t sbias, const void *svec, int64_t srows,
+                           int32_t sbytes, const int32_t *okey, int32_t obias, const void *ovec,
+                           int64_t orows, int32_t obytes, const int32_t *price, const int32_t *disc,
+                           int64_t rows, int32_t G, unsigned long long *sum, unsigned long long *cnt,
+                           uint32_t *bad, int num_cus, hipStream_t s);
 // dense-vector join sub-step (olap_join.hip); winner: vlen int32 of scratch
 hipError_t launch_join_build(int64_t rows, const int32_t *key, const int32_t *payload,
                              int32_t *winner, int32_t *vec, int64_t vlen, uint32_t *bad,

@@ -122,6 +122,12 @@ class DimProgram(C.Structure):
         return p
 
 
+class TriJoin(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("key", C.c_void_p * 2), ("key_bias", C.c_int32 * 2),
+                ("dimvec", C.c_void_p * 2), ("dim_rows", C.c_int64 * 2), ("code_bytes", C.c_int32 * 2),
+                ("price", C.c_void_p), ("discount", C.c_void_p), ("group_nums", C.c_int32)]
+
+
 class BatchQuery(C.Structure):
     _fields_ = [("ndims", C.c_int32),
                 ("table", C.c_int32 * MAX_DIMS),
@@ -274,6 +280,7 @@ SIGNATURES = {
     "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
                                     _P]),
     "olap_agg_expr": (C.c_int, [_P, C.c_int64, _P, _P, _P, _I64P, _P]),
+    "olap_tri_table_join": (C.c_int, [_P, C.POINTER(TriJoin), _P, _P, _P]),
     "olap_join_build": (C.c_int, [_P, C.c_int64, _P, _P, _P, C.c_int64, _I64P, _P]),
     "olap_join_probe": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, _I64P, _I64P, _P]),
 }
@@ -555,6 +562,22 @@ class Context:
                                             vector.numel(), C.byref(tot), C.byref(bad), stream),
                    "join_probe")
         return tot.value, bad.value
+
+    def tri_table_join(self, supp_key, supp_vec, order_key, order_vec, price, discount,
+                       group_nums, out_sum, out_cnt, supp_bias=-1, order_bias=0, stream=None):
+        """TPC-H Q5's fused snowflake probe into out_sum / out_cnt (int64,
+        group_nums each): revenue in cents x percent."""
+        j = TriJoin()
+        j.rows = supp_key.numel()
+        j.key[0], j.key[1] = _ptr(supp_key), _ptr(order_key)
+        j.key_bias[0], j.key_bias[1] = supp_bias, order_bias
+        j.dimvec[0], j.dimvec[1] = _ptr(supp_vec), _ptr(order_vec)
+        j.dim_rows[0], j.dim_rows[1] = supp_vec.numel(), order_vec.numel()
+        j.code_bytes[0], j.code_bytes[1] = supp_vec.element_size(), order_vec.element_size()
+        j.price, j.discount = _ptr(price), _ptr(discount)
+        j.group_nums = group_nums
+        self.check(self.lib.olap_tri_table_join(self.h, C.byref(j), _ptr(out_sum), _ptr(out_cnt),
+                                                stream), "tri_table_join")
 
     def agg_expr(self, p, q, t, stream=None) -> int:
         r = C.c_int64()

@@ -275,3 +275,121 @@ class DeviceBatchStar:
         plan = oc.plan_from([s[0] for s in steps], factor, G)
         cols = oc.Context.columns(dvs, self.fks, self.m1, None, rows=self.cfg.rows)
         return plan, cols
+
+
+# ---------------------------------------------------------------- TPC-H Q5
+
+REGIONS = ["AFRICA", "AMERICA", "ASIA", "EUROPE", "MIDDLE EAST"]
+# n_regionkey of the 25 TPC-H nations in n_nationkey order (the TPC-H
+# specification's NATION table)
+NATION_REGION = [0, 1, 1, 1, 4, 0, 3, 3, 2, 2, 4, 4, 2, 4, 0, 0, 0, 1, 2, 3, 4, 2, 3, 3, 1]
+LINES_PER_ORDER = 4
+
+
+@dataclass
+class Q5Config:
+    """A TPC-H-shaped snowflake for Q5 at scale factor sf: 10,000 suppliers,
+    150,000 customers and 1,500,000 orders per unit of sf, 4 lineitems per
+    order (6M per unit of sf), lineitems in order-key order as TPC-H lays
+    them out. Columns are uniform draws of the counter-based generator:
+    s_nationkey, c_nationkey U[0,25); o_custkey U[1,C]; o_orderdate
+    yyyymmdd over 1992-1998 (days 1-28); l_suppkey U[1,S]; l_extendedprice
+    in cents U[90000, 10494950]; l_discount in percent U[0,10]."""
+    sf: float
+    seed: int = DEFAULT_SEED
+    name: str = ""
+
+    @property
+    def suppliers(self) -> int:
+        return int(10_000 * self.sf)
+
+    @property
+    def customers(self) -> int:
+        return int(150_000 * self.sf)
+
+    @property
+    def orders(self) -> int:
+        return int(1_500_000 * self.sf)
+
+    @property
+    def lineitems(self) -> int:
+        return LINES_PER_ORDER * self.orders
+
+
+class DeviceQ5:
+    """Q5Config's tables in HBM."""
+
+    def __init__(self, ctx, cfg: Q5Config, device: str = "cuda", stream=None):
+        import torch
+
+        self.cfg = cfg
+        i32 = torch.int32
+        k = cfg.seed
+
+        def col(n, sid, lo, hi):
+            t = torch.empty(n, dtype=i32, device=device)
+            ctx.gen_uniform(t, k, sid, lo, hi, stream=stream)
+            return t
+
+        self.r_code = torch.arange(len(REGIONS), dtype=i32, device=device)  # sorted names: rank = index
+        self.n_regionkey = torch.tensor(NATION_REGION, dtype=i32, device=device)
+        self.n_nationkey = torch.arange(25, dtype=i32, device=device)
+        self.s_nationkey = col(cfg.suppliers, 600, 0, 25)
+        self.c_nationkey = col(cfg.customers, 601, 0, 25)
+        self.o_custkey = col(cfg.orders, 602, 1, cfg.customers + 1)
+        y = col(cfg.orders, 603, 1992, 1999)
+        m = col(cfg.orders, 604, 1, 13)
+        dd = col(cfg.orders, 605, 1, 29)
+        self.o_orderdate = (y * 10000 + m * 100 + dd).to(i32)
+        del y, m, dd
+        L = cfg.lineitems
+        self.l_orderkey_new = (torch.arange(L, dtype=torch.int64, device=device)
+                               // LINES_PER_ORDER).to(i32)
+        self.l_suppkey = col(L, 606, 1, cfg.suppliers + 1)
+        self.l_extendedprice = col(L, 607, 90_000, 10_494_951)
+        self.l_discount = col(L, 608, 0, 11)
+
+    def fact_bytes(self) -> int:
+        return self.cfg.lineitems * 16
+
+    def vectors(self, ctx, region: str, date: int, code_bytes: int = 1, stream=None):
+        """Q5's dimension vectors for `region` and orders in [date, date +
+        10000) (one year of yyyymmdd), built with general predicates as the
+        reference builds them (code/TPCH_Q5/Q5.cpp:915-944): region by name
+        -> nation (its key) -> supplier and customer (their nation's key,
+        through one hop) -> orders (date range AND the customer's nation,
+        through a second hop). Returns (supplier vector, orders vector)."""
+        import torch
+
+        from . import olapcore as oc
+
+        dev = self.n_regionkey.device
+        codes = {n: i for i, n in enumerate(sorted(REGIONS))}
+        i32, dt = torch.int32, torch.int8 if code_bytes == 1 else torch.int32
+        region_v = torch.empty(len(REGIONS), dtype=i32, device=dev)
+        ctx.build_dimvec_program(oc.DimProgram.of([("in", self.r_code, [codes[region]])]),
+                                 len(REGIONS), region_v, stream=stream)
+        nation_v = torch.empty(25, dtype=i32, device=dev)
+        ctx.build_dimvec_program(oc.DimProgram.of([("hop", self.n_regionkey, region_v, 0)],
+                                                  group=oc.GROUP_COLUMN, group_col=self.n_nationkey),
+                                 25, nation_v, stream=stream)
+        supp_v = torch.empty(self.cfg.suppliers, dtype=dt, device=dev)
+        ctx.build_dimvec_program(oc.DimProgram.of([("hop", self.s_nationkey, nation_v, 0)],
+                                                  group=oc.GROUP_HOP, group_term=0),
+                                 self.cfg.suppliers, supp_v, stream=stream)
+        cust_v = torch.empty(self.cfg.customers, dtype=i32, device=dev)
+        ctx.build_dimvec_program(oc.DimProgram.of([("hop", self.c_nationkey, nation_v, 0)],
+                                                  group=oc.GROUP_HOP, group_term=0),
+                                 self.cfg.customers, cust_v, stream=stream)
+        order_v = torch.empty(self.cfg.orders, dtype=dt, device=dev)
+        ctx.build_dimvec_program(oc.DimProgram.of([("range", self.o_orderdate, date, date + 10000),
+                                                   ("and", "hop", self.o_custkey, cust_v, -1)],
+                                                  group=oc.GROUP_HOP, group_term=1),
+                                 self.cfg.orders, order_v, stream=stream)
+        return supp_v, order_v
+
+    def run(self, ctx, supp_v, order_v, out_sum, out_cnt, stream=None):
+        """The fused probe over lineitem (Tri_table_join, Q5.cpp:945-978)."""
+        ctx.tri_table_join(self.l_suppkey, supp_v, self.l_orderkey_new, order_v,
+                           self.l_extendedprice, self.l_discount, 25, out_sum, out_cnt,
+                           supp_bias=-1, order_bias=0, stream=stream)

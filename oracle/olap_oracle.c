@@ -720,6 +720,34 @@ int64_t orc_join_probe(int64_t rows, const int32_t *key, const int32_t *vector,
     return s;
 }
 
+/* Tri_table_join_vector_thread, code/TPCH_Q5/Q5.cpp:945-978: a lineitem row
+ * counts when its supplier's code bitmap_S[l_suppkey - 1] is not -1 and
+ * equals its order's code bitmap_o[l_orderkey_new] (the customer's nation),
+ * and GrpVex[code] += l_extendedprice * (1 - l_discount). Restated in fixed
+ * point: price in cents, discount in percent, sum[code] += price * (100 -
+ * discount) in int64 (10^4 x the reference's double), count[code] += 1. Keys
+ * (plus their bias) outside a vector find no row. Returns the matching rows
+ * whose code is outside [0, G) (not aggregated). */
+int64_t orc_tri_table_join(int64_t rows, const int32_t *skey, int32_t sbias, const int32_t *svec,
+                           int64_t srows, const int32_t *okey, int32_t obias, const int32_t *ovec,
+                           int64_t orows, const int32_t *price, const int32_t *disc, int32_t G,
+                           int64_t *sum, int64_t *cnt)
+{
+    int64_t bad = 0;
+    memset(sum, 0, sizeof(int64_t) * (size_t)G);
+    memset(cnt, 0, sizeof(int64_t) * (size_t)G);
+    for (int64_t i = 0; i < rows; i++) {
+        int64_t ks = (int64_t)skey[i] + sbias, ko = (int64_t)okey[i] + obias;
+        if (ks < 0 || ks >= srows || svec[ks] == -1) continue;
+        if (ko < 0 || ko >= orows || ovec[ko] != svec[ks]) continue;
+        int32_t g = svec[ks];
+        if (g < 0 || g >= G) { bad++; continue; }
+        sum[g] += (int64_t)price[i] * (int64_t)(100 - disc[i]);
+        cnt[g] += 1;
+    }
+    return bad;
+}
+
 /* aggalgo_rowwise, code/aggregation/test_aggregation_option.cpp:28-37:
  * SUM(price*qty*tax + price*qty) with tax a percentage (gendata_util.hpp:757,
  * tax = rand_x(L_TAX_MIN, L_TAX_MAX)/100). Restated in fixed point with scale

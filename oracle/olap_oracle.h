@@ -131,6 +131,10 @@ int64_t orc_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
                          const int32_t *m2, int64_t *group_vector, int64_t group_num,
                          int64_t *bad_keys);
 int64_t orc_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t);
+int64_t orc_tri_table_join(int64_t rows, const int32_t *skey, int32_t sbias, const int32_t *svec,
+                           int64_t srows, const int32_t *okey, int32_t obias, const int32_t *ovec,
+                           int64_t orows, const int32_t *price, const int32_t *disc, int32_t G,
+                           int64_t *sum, int64_t *cnt);
 void orc_join_build(int64_t rows, const int32_t *key, const int32_t *payload, int32_t *vector,
                     int64_t vector_len, int64_t *bad_keys);
 int64_t orc_join_probe(int64_t rows, const int32_t *key, const int32_t *vector,

@@ -96,6 +96,9 @@ def lib() -> C.CDLL:
         L.orc_agg_expr.restype = C.c_int64
         L.orc_agg_expr.argtypes = [C.c_int64, P, P, P]
         L.orc_join_build.argtypes = [C.c_int64, P, P, P, C.c_int64, P]
+        L.orc_tri_table_join.restype = C.c_int64
+        L.orc_tri_table_join.argtypes = [C.c_int64, P, C.c_int32, P, C.c_int64, P, C.c_int32, P,
+                                         C.c_int64, P, P, C.c_int32, P, P]
         L.orc_build_dimvec_program.restype = C.c_int64
         L.orc_build_dimvec_program.argtypes = [P, C.c_int64, C.c_int32, P]
         L.orc_join_probe.restype = C.c_int64
@@ -326,6 +329,17 @@ def build_dimvec_program(terms, rows, code_bytes=1, group=0, group_col=None, gro
     return out, int(bad)
 
 
+def tri_table_join(skey, svec, okey, ovec, price, disc, G, sbias=-1, obias=0):
+    """TPC-H Q5's fused probe in fixed point: (sum, count, bad) over int32
+    vectors (NULL -1)."""
+    a = [np.ascontiguousarray(x, dtype=np.int32) for x in (skey, svec, okey, ovec, price, disc)]
+    s = np.zeros(G, np.int64)
+    c = np.zeros(G, np.int64)
+    bad = lib().orc_tri_table_join(a[0].size, _p(a[0]), sbias, _p(a[1]), a[1].size, _p(a[2]), obias,
+                                   _p(a[3]), a[3].size, _p(a[4]), _p(a[5]), G, _p(s), _p(c))
+    return s, c, int(bad)
+
+
 def join_build(key, payload, vector):
     """vecjoin build into `vector` (int32 numpy array, updated in place);
     returns the keys outside the vector."""
@@ -359,13 +373,26 @@ def available_cpus() -> int:
 
 # ---------------------------------------------------------------- compiled reference
 
+def host_has_avx512() -> bool:
+    """The AVX-512 subsets the reference's makefile asks for (F, CD, BW, DQ, VL)."""
+    try:
+        flags = set(open("/proc/cpuinfo").read().split())
+    except OSError:
+        return False
+    return {"avx512f", "avx512cd", "avx512bw", "avx512dq", "avx512vl"} <= flags
+
+
 class Reference:
     """The reference's own operator workers, compiled from its sources into
     oracle/_ref. Single-threaded workers over [0, n); uint32 group vectors."""
 
-    def __init__(self):
+    def __init__(self, avx512: bool = False):
+        """avx512: the build with the reference's AVX-512 flags at -O3
+        (used only where this host's CPU has those instructions)."""
         self.ok = False
-        p = os.path.join(REF_DIR, "libref_olapcore.so")
+        self.avx512 = avx512 and host_has_avx512() and os.path.exists(
+            os.path.join(REF_DIR, "libref_olapcore_avx512.so"))
+        p = os.path.join(REF_DIR, "libref_olapcore_avx512.so" if self.avx512 else "libref_olapcore.so")
         if not os.path.exists(p):
             return
         P = C.c_void_p
@@ -445,11 +472,19 @@ class Reference:
         fn(dvp, fkp, o, len(orders), f, n, _p(a), _p(b), _p(gv))
         return gv
 
+    def olapcore_rowwise(self, dimvecs, fks, orders, factor, group_nums, m1, m2, nthreads):
+        return self._driver("ref_OLAPcore_rowwise", dimvecs, fks, orders, factor, group_nums, m1,
+                            m2, nthreads)
+
     def olapcore_vwm_dv(self, dimvecs, fks, orders, factor, group_nums, m1, m2, nthreads):
+        return self._driver("ref_OLAPcore_vwm_dv", dimvecs, fks, orders, factor, group_nums, m1,
+                            m2, nthreads)
+
+    def _driver(self, name, dimvecs, fks, orders, factor, group_nums, m1, m2, nthreads):
         dv = [np.ascontiguousarray(d, dtype=np.int8) for d in dimvecs]
         dvp = (C.c_void_p * 4)(*[_p(a) for a in dv])
         fkp = (C.c_void_p * 4)(*[_p(a) for a in fks])
         o = (C.c_int * 4)(*list(orders) + [0] * (4 - len(orders)))
         f = (C.c_int * 4)(*list(factor) + [1] * (4 - len(factor)))
-        return int(self.L.ref_OLAPcore_vwm_dv(dvp, fkp, o, len(orders), f, int(m1.size),
-                                              int(group_nums), _p(m1), _p(m2), nthreads))
+        return int(getattr(self.L, name)(dvp, fkp, o, len(orders), f, int(m1.size),
+                                         int(group_nums), _p(m1), _p(m2), nthreads))

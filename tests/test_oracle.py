@@ -376,3 +376,27 @@ def test_agg_expr_closed_form():
     # 100x the reference's double formula, exactly
     ref = float(((p * q) * (t / 100.0) + p * q).astype(np.float64).sum())
     assert abs(po.agg_expr(p, q, t) / 100.0 - ref) < 1e-6 * ref
+
+
+@needs_ref
+def test_avx512_reference_build_agrees():
+    """The CPU baseline's build (the reference's AVX-512 flags at -O3) returns
+    the same totals as the -O2 build the oracle is checked against."""
+    fast = po.Reference(avx512=True)
+    if not fast.avx512:
+        pytest.skip("this host lacks AVX-512 or the build is absent")
+    rng = np.random.default_rng(64)
+    rows = 200_000
+    dv = [np.concatenate([rng.integers(0, 7, D).astype(np.int8), np.array([127], np.int8)])
+          for D in (3000, 200, 1000, 2556)]
+    for d in dv:
+        d[rng.random(d.size) < 0.4] = 127
+    fk = [rng.integers(0, d.size, rows, dtype=np.int32) for d in dv]
+    m1 = rng.integers(1, 100, rows, dtype=np.int32)
+    m2 = np.zeros(rows, np.int32)
+    orders, factor, G = [0, 1, 2, 3], [343, 49, 7, 1], 2401
+    # OLAPcore_vwm_dv zeroes its group vectors (the row-wise driver does not)
+    for fn in ("olapcore_vwm_dv",):
+        a = getattr(fast, fn)(dv, fk, orders, factor, G, m1, m2, 4)
+        b = getattr(REF, fn)(dv, fk, orders, factor, G, m1, m2, 4)
+        assert a == b, fn
