@@ -278,6 +278,12 @@ olap_status olap_agg_cwm_dv(olap_query q, int64_t *sum, int64_t *count, void *st
 /* host helper: the int every reference driver returns -- the uint32 group
  * vector summed into an int (e.g. :1027-1036) -- from int64 sums (host). */
 int32_t olap_legacy_count(const int64_t *sum_host, int64_t group_nums);
+/* The reference's raw result, its uint32 group vector (summed over threads,
+ * :1022-1034): u32_out[g] = (uint32_t)sum[g] for g < group_nums (device
+ * arrays). Where the reference's vector wraps, this wraps the same way;
+ * asynchronous. */
+olap_status olap_legacy_u32(olap_ctx ctx, const int64_t *sum, int64_t group_nums, uint32_t *u32_out,
+                            void *stream);
 
 /* ---- batched form: many star-join group-by queries in one scan ------------ */
 /* No counterpart in the reference, which runs one query per scan. A batch

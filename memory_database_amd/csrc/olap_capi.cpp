@@ -1508,6 +1508,17 @@ olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, cons
     return OLAP_OK;
 }
 
+olap_status olap_legacy_u32(olap_ctx ctx, const int64_t *sum, int64_t group_nums, uint32_t *u32_out,
+                            void *stream)
+{
+    if (!ctx) return OLAP_E_INVALID;
+    if (group_nums < 0 || (group_nums && (!sum || !u32_out)))
+        return fail(ctx, OLAP_E_INVALID, "legacy_u32: bad arguments");
+    DeviceGuard g(ctx->device);
+    HIPC(ctx, launch_truncate_u32((const unsigned long long *)sum, group_nums, u32_out, as_stream(stream)));
+    return OLAP_OK;
+}
+
 olap_status olap_tri_table_join(olap_ctx ctx, const olap_tri_join *j, int64_t *sum, int64_t *count,
                                 void *stream)
 {

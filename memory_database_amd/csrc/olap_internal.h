@@ -243,6 +243,7 @@ hipError_t launch_join_build(int64_t rows, const int32_t *key, const int32_t *pa
                              int num_cus, hipStream_t s);
 hipError_t launch_join_probe(int64_t rows, const int32_t *key, const int32_t *vec, int64_t vlen,
                              unsigned long long *total, uint32_t *bad, int num_cus, hipStream_t s);
+hipError_t launch_truncate_u32(const unsigned long long *v, int64_t n, uint32_t *out, hipStream_t s);
 hipError_t launch_sum_u64(const unsigned long long *v, int64_t n, unsigned long long *out,
                           int num_cus, hipStream_t s);
 hipError_t launch_agg_expr(int64_t rows, const int32_t *p, const int32_t *q, const int32_t *t,

@@ -1362,6 +1362,21 @@ hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t 
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_truncate_u32(const unsigned long long *v, int64_t n,
+                                                      uint32_t *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)v[i];
+}
+
+hipError_t launch_truncate_u32(const unsigned long long *v, int64_t n, uint32_t *out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_truncate_u32<<<grid_for(n, 256 * 4, 4096), 256, 0, s>>>(v, n, out);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long *v, int64_t n,
                                                  unsigned long long *out)
 {

@@ -276,6 +276,7 @@ SIGNATURES = {
     "olap_server_request_bytes": (C.c_int64, [C.POINTER(Star), C.POINTER(BatchQuery)]),
     "olap_server_plan": (C.c_int32, [_P, _P, C.c_int32, C.c_int64, C.c_int32, _P]),
     "olap_legacy_count": (C.c_int32, [_P, C.c_int64]),
+    "olap_legacy_u32": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "olap_starjoin": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P, _P, _P, _I64P, _P]),
     "olap_vector_group": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _I64P, _I64P,
                                     _P]),
@@ -584,6 +585,12 @@ class Context:
         self.check(self.lib.olap_agg_expr(self.h, p.numel(), _ptr(p), _ptr(q), _ptr(t),
                                           C.byref(r), stream), "agg_expr")
         return r.value
+
+    def legacy_u32(self, sums, out, stream=None):
+        """The reference's uint32 group vector from int64 sums (device)."""
+        self.check(self.lib.olap_legacy_u32(self.h, _ptr(sums), sums.numel(), _ptr(out), stream),
+                   "legacy_u32")
+        return out
 
     def sync(self, stream=None):
         self.check(self.lib.olap_stream_sync(self.h, stream), "sync")

@@ -108,3 +108,35 @@ def test_reference_signatures_from_cpp(path, nthreads, nodes, tmp_path, ncpu):
     sv = np.fromfile(tmp_path / "cwm_sv_gid.bin", dtype=np.int16)
     np.testing.assert_array_equal(sv, d["cwm_sv_gid"])
     assert (sv == GROUP_NULL).sum() == rows - int(ec.sum())
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_raw_uint32_vectors_equal_the_references(ctx, path):
+    """Every strategy through the C ABI on device-resident columns, then
+    olap_legacy_u32: the reference workers' uint32 group vectors, slot by
+    slot (sums of M1 + M2, and counts)."""
+    import torch
+
+    from memory_database_amd import olapcore as oc
+
+    d = dict(np.load(path))
+    n = int(d["orders"].size)
+    G = int(d["group_nums"])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    cols = oc.Context.columns([dev(d[f"dimvec{t}"]) for t in range(4)],
+                              [dev(d[f"fk{t}"]) for t in range(4)], dev(d["m1"]), dev(d["m2"]),
+                              rows=int(d["m1"].size))
+    plan = oc.plan_from(list(d["orders"])[:n], list(d["factor"])[:n], G)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    u32 = torch.empty(G, dtype=torch.int32, device="cuda")
+    for s in ("rowwise", "vwm_dv", "vwm_sv", "cwm_dv", "cwm_sv"):
+        q.run(s, osum, ocnt)
+        ctx.legacy_u32(osum, u32)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(u32.cpu().numpy().view(np.uint32), d[f"{s}_sum_u32"], err_msg=s)
+        ctx.legacy_u32(ocnt, u32)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(u32.cpu().numpy().view(np.uint32), d[f"{s}_count_u32"], err_msg=s)
+    q.close()
