@@ -71,7 +71,7 @@ struct BatchArgs {
     uint32_t *err;
     StageArgs stage;
     // event-grouped kernel: LDS word offsets of the per-query counters
-    // (then cursors, starts and the total), the staged attributes [4][R],
+    // (then cursors, starts and the total), the staged attributes [R][4],
     // the staged measures [R] (m1, then m2) and the event list of ev_cap
     uint32_t cnt_lds;
     uint32_t row_lds;
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     l_u32 *qcur = qcnt + 64u;             // [64] list cursors per query
     l_u32 *qreal = qcur + 64u;            // [64] matches per query (kept for the padding)
     l_u32 *seg = qreal + 64u;             // [64] padded query starts, [64] the padded total
-    l_u32 *rattr = L + a.row_lds;         // [4][R]
+    l_u32 *rattr = L + a.row_lds;         // [R] the row's four attributes (16 bytes a row)
     l_u32 *rm = L + a.m_lds;              // [R] m1, then [R] m2
     l_u32 *ev = L + a.ev_lds;             // [ev_cap]
     stage_l_u32 *scur = (stage_l_u32 *)lds + a.stage.cur_lds;
@@ -534,6 +534,7 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
         // 1. rows
         unsigned long long al0 = r0 < rows ? ~0ull : 0ull;
         unsigned long long al1 = r0 + 1 < rows ? ~0ull : 0ull;
+        uint32_t atr[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             uint32_t at0 = 0u, at1 = 0u;
@@ -560,9 +561,18 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                         }
                     }
                 }
-                rattr[(uint32_t)t * R + 2u * tid] = at0;
-                rattr[(uint32_t)t * R + 2u * tid + 1u] = at1;
+                atr[0][t] = at0;
+                atr[1][t] = at1;
             }
+        }
+        {
+            // one 16-byte word a row: the events step reads a row's four
+            // attributes with one LDS read
+            typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+            typedef __attribute__((address_space(3))) v4u l_u4;
+            l_u4 *r4 = (l_u4 *)rattr;
+            r4[2u * tid] = v4u{atr[0][0], atr[0][1], atr[0][2], atr[0][3]};
+            r4[2u * tid + 1u] = v4u{atr[1][0], atr[1][1], atr[1][2], atr[1][3]};
         }
         if (al0 | al1) {
             rm[2u * tid] = (uint32_t)cm1.x;
@@ -653,9 +663,10 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 // loads), then the group-by terms of every table (a table
                 // the query does not group by adds 0, whatever the staged
                 // word holds)
-                uint32_t av[4];
-#pragma unroll
-                for (int t = 0; t < 4; t++) av[t] = rattr[(uint32_t)t * R + row];
+                typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+                typedef __attribute__((address_space(3))) const v4u l_cu4;
+                const v4u a4 = ((l_cu4 *)rattr)[row];
+                const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
                 const uint32_t m1v = rm[row];
                 const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
                 const uint32_t G = d.G;
