@@ -5,8 +5,9 @@
 // ordering customer's nation code for the region and date range; the row
 // counts when both are non-NULL and equal (customer and supplier in the same
 // nation), and its revenue price x (100 - discount%) goes to that nation's
-// int64 slot. Both keys stream through 128-bit loads; the order key, price
-// and discount are read only for 4-row chunks with a live row (early exit);
+// int64 slot. Both keys stream through 128-bit loads and both vectors are
+// probed at once; price and discount are read only for 4-row chunks with a
+// matching row;
 // the per-nation table (up to kSnowSlots slots) sits in LDS.
 #include "olap_internal.h"
 
@@ -76,11 +77,12 @@ __global__ __launch_bounds__(kSnowBS) void k_tri_join(SnowArgs a)
         const v4i *pr4 = (const v4i *)a.price;
         const v4i *ds4 = (const v4i *)a.disc;
         for (int64_t c0 = blockIdx.x * (int64_t)kSnowBS + threadIdx.x; c0 < chunks; c0 += stride * kSnowU) {
-            v4i sk[kSnowU];
+            v4i sk[kSnowU], ok_[kSnowU];
 #pragma unroll
             for (int u = 0; u < kSnowU; u++) {
                 const int64_t c = c0 + u * stride;
                 sk[u] = c < chunks ? __builtin_nontemporal_load(&sk4[c]) : v4i{-1, -1, -1, -1};
+                ok_[u] = c < chunks ? __builtin_nontemporal_load(&ok4[c]) : v4i{-1, -1, -1, -1};
             }
 #pragma unroll
             for (int u = 0; u < kSnowU; u++) {
@@ -91,13 +93,16 @@ __global__ __launch_bounds__(kSnowBS) void k_tri_join(SnowArgs a)
                 s[1] = code_of(a.svec, a.srows, (int64_t)sk[u].y + a.sbias, a.sbytes);
                 s[2] = code_of(a.svec, a.srows, (int64_t)sk[u].z + a.sbias, a.sbytes);
                 s[3] = code_of(a.svec, a.srows, (int64_t)sk[u].w + a.sbias, a.sbytes);
-                if ((s[0] & s[1] & s[2] & s[3]) == -1) continue;  // every row NULL
-                const v4i ok = __builtin_nontemporal_load(&ok4[c]);
+                // both probes at once: the order keys stream like the supplier
+                // keys (a fifth of the rows pass the supplier probe, so almost
+                // every line of them is needed), and the order vector is read
+                // in key order
+                const v4i ok = ok_[u];
                 int32_t o[4];
-                o[0] = s[0] == -1 ? -1 : code_of(a.ovec, a.orows, (int64_t)ok.x + a.obias, a.obytes);
-                o[1] = s[1] == -1 ? -1 : code_of(a.ovec, a.orows, (int64_t)ok.y + a.obias, a.obytes);
-                o[2] = s[2] == -1 ? -1 : code_of(a.ovec, a.orows, (int64_t)ok.z + a.obias, a.obytes);
-                o[3] = s[3] == -1 ? -1 : code_of(a.ovec, a.orows, (int64_t)ok.w + a.obias, a.obytes);
+                o[0] = code_of(a.ovec, a.orows, (int64_t)ok.x + a.obias, a.obytes);
+                o[1] = code_of(a.ovec, a.orows, (int64_t)ok.y + a.obias, a.obytes);
+                o[2] = code_of(a.ovec, a.orows, (int64_t)ok.z + a.obias, a.obytes);
+                o[3] = code_of(a.ovec, a.orows, (int64_t)ok.w + a.obias, a.obytes);
                 bool m[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) m[j] = s[j] != -1 && s[j] == o[j];
