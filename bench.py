@@ -292,7 +292,7 @@ def bench_sweep(ctx, stream, rows, strategy="rowwise"):
                "ms": round(ms, 4), "rows_per_s": rows / (ms / 1e3),
                "effective_GBps": round(alg / (ms / 1e3) / 1e9, 1),
                "passing_rows": int(ocnt.sum().item()),
-               "launch": {"threads": th, "rows_per_thread": rp, "aggregation": agg}}
+               "launch": {"threads": th, "rows_per_thread": rp, "aggregation": agg, **q.pipeline()}}
         log(f"sweep {cfg.name}: {row}")
         out.append(row)
         q.close()
@@ -641,7 +641,7 @@ def main():
     wall_ms, ev = time_steps(step, args.steps, args.warmup, stream)
     ms = statistics.mean(ev)
     th, rp, agg = q.launch_info()
-    launch = {"threads": th, "rows_per_thread": rp, "aggregation": agg}
+    launch = {"threads": th, "rows_per_thread": rp, "aggregation": agg, **q.pipeline()}
     rows_per_s = cfg.rows / (ms / 1e3)
     alg_bytes = ds.fact_bytes()
     achieved = alg_bytes / (ms / 1e3) / 1e9

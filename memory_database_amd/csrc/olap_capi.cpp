@@ -343,6 +343,7 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     q->ran_threads = sh.bs;
     q->ran_rows = sh.unroll * kChunk;
     q->ran_agg = OLAP_AGG_TWOPASS;
+    q->ran_pipe = q->eager1 ? OLAP_PIPE_STREAM2 : 0;
     return OLAP_OK;
 }
 
@@ -390,6 +391,7 @@ olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, h
     q->ran_threads = sh.bs;
     q->ran_rows = sh.unroll * kChunk;
     q->ran_agg = OLAP_AGG_ONEPASS;
+    q->ran_pipe = q->eager1 ? OLAP_PIPE_STREAM2 : 0;
     return OLAP_OK;
 }
 
@@ -1071,6 +1073,11 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
         q->ran_threads = sh.bs;
         q->ran_rows = sh.unroll * kChunk;
         q->ran_agg = L.gt_lds ? OLAP_AGG_LDS : a.hot_bits ? OLAP_AGG_HOT : OLAP_AGG_ATOMICS;
+        q->ran_pipe = 0;
+        if (strategy == OLAP_ROWWISE && vec) {
+            const bool stream2 = nd > 1 && q->eager1 && (sh.pipe == 0 || sh.pipe == 1);
+            q->ran_pipe = (sh.pipe ? OLAP_PIPE_AHEAD : 0) | (stream2 ? OLAP_PIPE_STREAM2 : 0);
+        }
         return OLAP_OK;
     }
     case OLAP_CWM_SV: {
@@ -1237,6 +1244,18 @@ olap_status olap_query_launch_info(olap_query q, int32_t *threads, int32_t *rows
     if (rows_per_thread) *rows_per_thread = q->ran_rows;
     if (aggregation) *aggregation = q->ran_agg;
     return OLAP_OK;
+}
+
+int32_t olap_query_pipeline(olap_query q)
+{
+    if (!q || q->ran_agg < 0) return -1;
+    return q->ran_pipe;
+}
+
+int32_t olap_shape_pipeline(olap_strategy strategy, int32_t shape)
+{
+    if (strategy != OLAP_ROWWISE || shape < 0 || shape >= (int32_t)num_row_shapes()) return 0;
+    return row_shape(shape).pipe;
 }
 
 int64_t olap_query_device_bytes(olap_query q)

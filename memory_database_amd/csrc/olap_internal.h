@@ -45,6 +45,7 @@ struct DevProgram {
 struct ScanShape {
     int bs;      // threads per workgroup
     int unroll;  // 128-bit chunks per thread per tile
+    int pipe;    // row-wise: 1 = every dependent fetch issued one tile ahead
 };
 
 // One probe step as the kernels see it. Codes of the step's dimension are
@@ -157,6 +158,7 @@ struct olap_query_s {
     // launch of the last run: threads per workgroup, rows per thread and the
     // aggregation (OLAP_AGG_*)
     int32_t ran_threads = 0, ran_rows = 0, ran_agg = -1;
+    int32_t ran_pipe = 0;              // OLAP_PIPE_* bits of the last row-wise run
 };
 
 namespace olap {

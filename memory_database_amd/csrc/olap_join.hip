@@ -15,7 +15,7 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kJoinBS = 512;
-constexpr int64_t kJoinLdsEntries = 32768;  // 128 KiB of LDS
+constexpr int64_t kJoinLdsEntries = kLdsBytes / 4;  // the whole LDS (160 KiB)
 
 __device__ __forceinline__ void add_u64(unsigned long long *p, unsigned long long v)
 {
@@ -77,14 +77,14 @@ __global__ __launch_bounds__(256) void k_join_write(int64_t rows, const int32_t 
 // Probe: sum of vec[key] over the probe keys, in int64. LDS = the vector is
 // copied into LDS first. VEC = keys are 16-byte aligned: 4 keys per lane per
 // 128-bit load, 4 loads in flight per lane.
-template <bool LDS, bool VEC>
-__global__ __launch_bounds__(kJoinBS) void k_join_probe(int64_t rows, const int32_t *key,
-                                                        const int32_t *vec, int64_t vlen,
-                                                        unsigned long long *total, uint32_t *bad)
+template <bool LDS, bool VEC, int BS = kJoinBS>
+__global__ __launch_bounds__(BS) void k_join_probe(int64_t rows, const int32_t *key,
+                                                   const int32_t *vec, int64_t vlen,
+                                                   unsigned long long *total, uint32_t *bad)
 {
     extern __shared__ __align__(16) int32_t lvec[];
     if (LDS) {
-        for (int64_t e = threadIdx.x; e < vlen; e += kJoinBS) lvec[e] = vec[e];
+        for (int64_t e = threadIdx.x; e < vlen; e += BS) lvec[e] = vec[e];
         __syncthreads();
     }
     auto look = [&](int32_t k, long long &acc, uint32_t &nbad) {
@@ -107,8 +107,8 @@ __global__ __launch_bounds__(kJoinBS) void k_join_probe(int64_t rows, const int3
         constexpr int U = 4;
         const int64_t chunks = rows / 4;
         const v4i *k4 = (const v4i *)key;
-        const int64_t stride = (int64_t)gridDim.x * kJoinBS;
-        for (int64_t c0 = blockIdx.x * (int64_t)kJoinBS + threadIdx.x; c0 < chunks; c0 += stride * U) {
+        const int64_t stride = (int64_t)gridDim.x * BS;
+        for (int64_t c0 = blockIdx.x * (int64_t)BS + threadIdx.x; c0 < chunks; c0 += stride * U) {
             v4i kk[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -126,8 +126,8 @@ __global__ __launch_bounds__(kJoinBS) void k_join_probe(int64_t rows, const int3
         }
         done = chunks * 4;
     }
-    for (int64_t i = done + blockIdx.x * (int64_t)kJoinBS + threadIdx.x; i < rows;
-         i += (int64_t)gridDim.x * kJoinBS)
+    for (int64_t i = done + blockIdx.x * (int64_t)BS + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * BS)
         look(key[i], acc, nbad);
     for (int o = 32; o > 0; o >>= 1) {
         acc += __shfl_xor(acc, o);
@@ -160,16 +160,21 @@ hipError_t launch_join_probe(int64_t rows, const int32_t *key, const int32_t *ve
     const bool vec4 = ((uintptr_t)key & 15u) == 0;
     if (vlen <= kJoinLdsEntries) {
         // one workgroup per CU pass copies the vector once; 2 per CU overlap
-        // one's copy with the other's probes
+        // one's copy with the other's probes. A vector above 80 KiB leaves
+        // room for one workgroup per CU, which then takes 1024 threads.
         const uint32_t lds = (uint32_t)std::max<int64_t>(vlen, 4) * 4u;
-        const void *k = vec4 ? (const void *)k_join_probe<true, true> : (const void *)k_join_probe<true, false>;
+        const bool big = lds > 80 * 1024;
+        const void *k = big ? (vec4 ? (const void *)k_join_probe<true, true, 1024> : (const void *)k_join_probe<true, false, 1024>)
+                            : (vec4 ? (const void *)k_join_probe<true, true> : (const void *)k_join_probe<true, false>);
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
-        const int grid = grid_of(rows, kJoinBS * 64, num_cus * (lds > 80 * 1024 ? 1 : 2));
-        if (vec4) k_join_probe<true, true><<<grid, kJoinBS, lds, s>>>(rows, key, vec, vlen, total, bad);
-        else k_join_probe<true, false><<<grid, kJoinBS, lds, s>>>(rows, key, vec, vlen, total, bad);
+        const int bs = big ? 1024 : kJoinBS;
+        const int grid = grid_of(rows, bs * 64, num_cus * (big ? 1 : 2));
+        void *args[] = {&rows, &key, &vec, &vlen, &total, &bad};
+        hipError_t e = hipLaunchKernel(k, dim3(grid), dim3(bs), args, lds, s);
+        if (e != hipSuccess) return e;
     } else {
         const int grid = grid_of(rows, kJoinBS * 16, num_cus * 8);
         if (vec4) k_join_probe<false, true><<<grid, kJoinBS, 0, s>>>(rows, key, vec, vlen, total, bad);

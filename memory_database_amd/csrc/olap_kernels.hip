@@ -678,6 +678,178 @@ __global__ __launch_bounds__(BS) void k_rowwise(ScanArgs a)
 }
 
 // ------------------------------------------------------------------------
+// row-wise scan with every dependent fetch one iteration ahead (direct
+// aggregation). Iteration i probes the streamed first step(s) of the
+// workgroup's i-th tile and issues the next step's key loads for its live
+// chunks; that tile's later steps and its measures follow in iterations
+// i+1, i+2, ..., each probing what the previous iteration loaded and issuing
+// the next loads. So every load is in flight for a whole iteration, and a
+// wave waits once per iteration instead of once per step. The loads are
+// buffer loads over the tile: a dead chunk's lane carries an offset past the
+// buffer's end and reads 0 without touching memory, so every load
+// instruction issues unconditionally and the compiler's waits stay exact.
+// Full tiles only; the rows after the last full tile take a plain loop.
+// ------------------------------------------------------------------------
+
+constexpr int kBufWord3 = 0x00020000;      // buffer resource word 3 on gfx9: 32-bit data
+constexpr uint32_t kDeadOff = 0x40000000u;  // past the end of any tile's buffer
+constexpr int kBufNT = 2;                   // nontemporal: the columns stream once
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_buf(const int32_t *col, uint64_t row, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(col + row), (short)0, (int)bytes, kBufWord3);
+}
+
+__device__ __forceinline__ v4i buf_load4(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return (v4i)__builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, kBufNT);
+}
+
+template <int NDIM, bool GLDS, int BS, int U, bool EAGER>
+__global__ __launch_bounds__(BS) void k_rowwise_pipe(ScanArgs a)
+{
+    constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
+    constexpr uint32_t TB = (uint32_t)TILE * 4u;       // bytes of one column of a tile
+    constexpr int FIRST = NDIM > 1 && EAGER ? 2 : 1;    // steps probed from the streamed keys
+    constexpr int D = NDIM - FIRST + 1;                 // stages: the later steps, then the measures
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<NDIM, GLDS>(a, lds);
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint64_t full = a.rows / TILE;
+    const uint64_t mine = blockIdx.x < full ? (full - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
+    const bool load_m = !a.count_only;
+    const bool has_m2 = a.m2 != nullptr;
+    uint32_t off[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) off[u] = (uint32_t)(u * BS + tid) * 16u;
+
+    // the streamed keys of the next tile
+    v4i nf0[U], nf1[U];
+    auto issue_first = [&](uint64_t i) {
+        const bool ok = i < mine;
+        const uint64_t row = ok ? (blockIdx.x + i * gridDim.x) * TILE : 0;
+        const __amdgpu_buffer_rsrc_t r0 = tile_buf(a.st[0].fk, row, ok ? TB : 0u);
+#pragma unroll
+        for (int u = 0; u < U; u++) nf0[u] = buf_load4(r0, off[u]);
+        if (FIRST == 2) {
+            const __amdgpu_buffer_rsrc_t r1 = tile_buf(a.st[NDIM > 1 ? 1 : 0].fk, row, ok ? TB : 0u);
+#pragma unroll
+            for (int u = 0; u < U; u++) nf1[u] = buf_load4(r1, off[u]);
+        }
+    };
+
+    // stage s holds a tile whose steps before FIRST + s are probed: its live
+    // rows, partial gids, first row, and the loads in flight for it (the keys
+    // of step FIRST + s, or the measures in the last stage)
+    v4i ld[D][U], ld2[U];
+    uint32_t al[D][U];
+    uint32_t gd[D][U][4];
+    uint64_t sb[D];
+#pragma unroll
+    for (int s = 0; s < D; s++) {
+        sb[s] = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ld[s][u] = v4i{0, 0, 0, 0};
+            al[s][u] = 0u;
+            gd[s][u][0] = gd[s][u][1] = gd[s][u][2] = gd[s][u][3] = 0u;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) ld2[u] = v4i{0, 0, 0, 0};
+
+    issue_first(0);
+    for (uint64_t it = 0; it < mine + D; it++) {
+        // 1. the oldest stage's measures have arrived: aggregate its matches
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (al[D - 1][u]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((al[D - 1][u] >> k) & 1u) {
+                        long long m = (long long)ld[D - 1][u][k];
+                        if (has_m2) m += (long long)ld2[u][k];
+                        aggregate<GLDS>(a, lds, gd[D - 1][u][k], (unsigned long long)m);
+                    }
+            }
+        }
+        // 2. probe what the other stages loaded, each moving one stage on
+#pragma unroll
+        for (int s = D - 2; s >= 0; s--) {
+            sb[s + 1] = sb[s];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                uint32_t alive = al[s][u];
+                uint32_t g[4] = {gd[s][u][0], gd[s][u][1], gd[s][u][2], gd[s][u][3]};
+                if (alive) probe4(a.st[FIRST + s], lds, ld[s][u], g, alive, false);
+                al[s + 1][u] = alive;
+#pragma unroll
+                for (int k = 0; k < 4; k++) gd[s + 1][u][k] = g[k];
+            }
+        }
+        // ... and the new tile's streamed steps
+        const bool ok = it < mine;
+        sb[0] = ok ? (blockIdx.x + it * gridDim.x) * TILE : 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint32_t alive = ok ? 0xFu : 0u;
+            uint32_t g[4] = {0u, 0u, 0u, 0u};
+            probe4(a.st[0], lds, nf0[u], g, alive, true);
+            if (FIRST == 2 && alive) probe4(a.st[NDIM > 1 ? 1 : 0], lds, nf1[u], g, alive, false);
+            al[0][u] = alive;
+#pragma unroll
+            for (int k = 0; k < 4; k++) gd[0][u][k] = g[k];
+        }
+        // 3. issue this iteration's loads in the order the next one uses
+        // them, then the next tile's streamed keys
+#pragma unroll
+        for (int s = D - 1; s >= 0; s--) {
+            if (s == D - 1) {
+                const __amdgpu_buffer_rsrc_t r1 = tile_buf(a.m1, sb[s], TB);
+#pragma unroll
+                for (int u = 0; u < U; u++) ld[s][u] = buf_load4(r1, al[s][u] && load_m ? off[u] : kDeadOff);
+                if (has_m2) {
+                    const __amdgpu_buffer_rsrc_t r2 = tile_buf(a.m2, sb[s], TB);
+#pragma unroll
+                    for (int u = 0; u < U; u++) ld2[u] = buf_load4(r2, al[s][u] && load_m ? off[u] : kDeadOff);
+                }
+            } else {
+                const __amdgpu_buffer_rsrc_t rk = tile_buf(a.st[FIRST + s].fk, sb[s], TB);
+#pragma unroll
+                for (int u = 0; u < U; u++) ld[s][u] = buf_load4(rk, al[s][u] ? off[u] : kDeadOff);
+            }
+        }
+        issue_first(it + 1);
+    }
+
+    // rows after the last full tile: every step in turn
+    for (uint64_t r = full * TILE + ((uint64_t)blockIdx.x * BS + tid) * 4u; r < a.rows;
+         r += (uint64_t)gridDim.x * BS * 4u) {
+        uint32_t alive, dummy;
+        uint32_t g[4] = {0u, 0u, 0u, 0u};
+        v4i f = load4<true>(a.st[0].fk, r, a.rows, alive);
+        probe4(a.st[0], lds, f, g, alive, true);
+#pragma unroll
+        for (int s = 1; s < NDIM; s++) {
+            if (!alive) break;
+            f = load4<true>(a.st[s].fk, r, a.rows, dummy);
+            probe4(a.st[s], lds, f, g, alive, false);
+        }
+        if (!alive) continue;
+        v4i m1 = v4i{0, 0, 0, 0}, m2 = v4i{0, 0, 0, 0};
+        if (load_m) {
+            m1 = load4<true>(a.m1, r, a.rows, dummy);
+            if (has_m2) m2 = load4<true>(a.m2, r, a.rows, dummy);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if ((alive >> k) & 1u) aggregate<GLDS>(a, lds, g[k], measure4(a, m1, m2, k));
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
 // vector-wise static vector (OLAPcore_vwm_sv_thread semantics): every step is
 // probed for every row; all columns of a tile are loaded up front.
 // ------------------------------------------------------------------------
@@ -1080,7 +1252,11 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
 
 // Row-wise launch shapes: (threads per workgroup, 128-bit chunks per thread
 // per tile). The autotuner times them on a live query.
-const ScanShape kRowShapes[] = {{512, 4}, {512, 2}, {1024, 2}, {1024, 4}, {256, 8}, {512, 8}};
+// The last four run the pipelined kernel (every dependent fetch one tile
+// ahead): pipe 1 streams the second step's keys with the first's when most
+// of their lines are live, pipe 2 never does.
+const ScanShape kRowShapes[] = {{512, 4, 0}, {512, 2, 0}, {1024, 2, 0}, {1024, 4, 0}, {256, 8, 0},
+                                {512, 8, 0}, {1024, 2, 1}, {512, 2, 1},  {1024, 2, 2}, {512, 2, 2}};
 const int kNumRowShapes = (int)(sizeof(kRowShapes) / sizeof(kRowShapes[0]));
 
 template <int NDIM, bool VEC, bool GLDS>
@@ -1090,6 +1266,15 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
     const ScanShape sh = kRowShapes[shape];
     a.ntiles = (a.rows + (uint64_t)sh.bs * 4u * sh.unroll - 1) / ((uint64_t)sh.bs * 4u * sh.unroll);
     if (a.ntiles == 0) return hipSuccess;
+    if (VEC && sh.pipe) {
+        const bool eager = NDIM > 1 && a.eager1 && sh.pipe == 1;
+#define OLAP_PIPE(BSX, UX)                                                                                  \
+    return eager ? launch_persistent(k_rowwise_pipe<NDIM, GLDS, BSX, UX, true>, a, num_cus, dyn_lds, a.ntiles, s, BSX) \
+                 : launch_persistent(k_rowwise_pipe<NDIM, GLDS, BSX, UX, false>, a, num_cus, dyn_lds, a.ntiles, s, BSX)
+        if (sh.bs == 1024) OLAP_PIPE(1024, 2);
+        OLAP_PIPE(512, 2);
+#undef OLAP_PIPE
+    }
     if (VEC && NDIM > 1 && a.eager1) {
         // step 1's keys streamed with step 0's
         switch (shape) {

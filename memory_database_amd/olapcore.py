@@ -231,6 +231,8 @@ SIGNATURES = {
                                          C.POINTER(C.c_int32)]),
     "olap_num_shapes": (C.c_int32, [C.c_int]),
     "olap_shape_info": (C.c_int, [C.c_int, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "olap_shape_pipeline": (C.c_int32, [C.c_int, C.c_int32]),
+    "olap_query_pipeline": (C.c_int32, [C.c_void_p]),
     "olap_query_set_shape": (C.c_int, [_P, C.c_int32]),
     "olap_query_set_aggregation": (C.c_int, [_P, C.c_int32]),
     "olap_query_autotune": (C.c_int, [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_int32),
@@ -653,6 +655,14 @@ class Query:
         self.ctx.check(self.ctx.lib.olap_query_launch_info(self.h, C.byref(th), C.byref(rp),
                                                            C.byref(ag)), "launch_info")
         return th.value, rp.value, AGGREGATIONS[ag.value]
+
+    def pipeline(self):
+        """How the last row-wise run fetched: dependent fetches one tile
+        ahead, and the second step's keys streamed with the first's."""
+        p = int(self.ctx.lib.olap_query_pipeline(self.h))
+        if p < 0:
+            raise OlapError(E_INVALID, "pipeline: the query has not run")
+        return {"fetch_ahead": bool(p & 1), "second_step_streamed": bool(p & 2)}
 
     def device_bytes(self) -> int:
         return int(self.ctx.lib.olap_query_device_bytes(self.h))

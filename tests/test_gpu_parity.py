@@ -509,3 +509,76 @@ def test_every_row_shape_matches_oracle(ctx, ncpu):
     np.testing.assert_array_equal(ocnt.cpu().numpy(), ec)
     np.testing.assert_array_equal(osum.cpu().numpy(), es)
     q.close()
+
+
+PIPE_CASES = {
+    # first step passes 1% (its later keys are not streamed), two measures
+    "sparse_first": dict(rows=1_234_567, thr=[1, 40, 60, 30], groups=[3, 4, 5, 2], m2=True),
+    # every step passes about half (step 1's keys streamed with step 0's)
+    "dense": dict(rows=1_000_003, thr=[50, 55, 45, 60], groups=[7, 3, 5, 4], m2=False),
+    # fewer rows than one tile, then a few tiles and a ragged end
+    "tiny": dict(rows=5, thr=[70, 80, 90, 60], groups=[2, 3, 2, 2], m2=True),
+    "few_tiles": dict(rows=3 * 8192 + 3, thr=[30, 80, 90, 60], groups=[2, 3, 2, 2], m2=False),
+    # no measure column: counts only
+    "count_only": dict(rows=777_777, thr=[20, 50, 50, 50], groups=[3, 3, 3, 3], m2=None),
+    # group table beyond LDS: global atomics, then behind the hot-group cache
+    "wide_groups": dict(rows=900_001, thr=[60, 70, 80, 90], groups=[200, 150, 30, 4], m2=False),
+}
+
+
+@pytest.mark.parametrize("ndims", [1, 2, 3, 4])
+@pytest.mark.parametrize("case", list(PIPE_CASES))
+def test_row_shapes_every_pipeline_depth(ctx, ncpu, ndims, case):
+    """Every row-wise launch shape (the pipelined ones issue each dependent
+    fetch a tile ahead, with a depth of ndims - 1 or ndims stages) against
+    the oracle: streamed or unstreamed second step, one or two measures,
+    counts only, ragged ends, group tables in LDS or in HBM."""
+    import torch
+
+    c = PIPE_CASES[case]
+    rng = np.random.default_rng(700 + 10 * ndims + len(case))
+    rows = c["rows"]
+    slots = list(rng.permutation(4)[:ndims])
+    groups, bitmap, sele = [0] * 4, [0] * 4, [0.0] * 4
+    dv, fk = [None] * 4, [None] * 4
+    for i, t in enumerate(slots):
+        D = int(rng.integers(1000, 40_000))
+        g = c["groups"][i]
+        groups[t], bitmap[t], sele[t] = g, 1, c["thr"][i] / 100.0
+        f = rng.integers(0, 100, D, dtype=np.int32)
+        dv[t] = po.build_dimvec(f, rng.integers(0, g, D, dtype=np.int32), -(2 ** 31), c["thr"][i], 1,
+                                1 if g < 127 else 4)
+        fk[t] = rng.integers(-2, D + D // 16, rows, dtype=np.int32)
+    plan_np = po.make_plan(groups, bitmap, sele)
+    count_only = c["m2"] is None
+    m1 = None if count_only else rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    m2 = rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32) if c["m2"] else None
+    ref_m1 = np.zeros(rows, np.int32) if count_only else m1
+    es, ec, _ = po.run("rowwise", plan_np, po.Star(dv, fk, ref_m1, m2), nthreads=ncpu)
+    plan = oc.plan_from(list(plan_np.orders)[: plan_np.dimvec_nums],
+                        list(plan_np.factor)[: plan_np.dimvec_nums], int(plan_np.group_nums))
+    cols = oc.Context.columns([None if d is None else _dev(d, torch) for d in dv],
+                              [None if f is None else _dev(f, torch) for f in fk],
+                              None if m1 is None else _dev(m1, torch),
+                              None if m2 is None else _dev(m2, torch), rows=rows)
+    G = int(plan.group_nums)
+    q = ctx.query(plan, cols)
+    osum = torch.empty(G, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty(G, dtype=torch.int64, device="cuda")
+    layouts = [0, 2] if case == "wide_groups" else [-1]
+    for layout in layouts:
+        q.set_aggregation(layout)
+        for shape in range(ctx.lib.olap_num_shapes(oc.ROWWISE)):
+            _say(f"  {case} ndims={ndims} G={G} layout={layout} shape {shape}")
+            q.set_shape(shape)
+            q.run("rowwise", None if count_only else osum, ocnt)
+            torch.cuda.synchronize()
+            pipe = ctx.lib.olap_shape_pipeline(oc.ROWWISE, shape)
+            assert q.pipeline()["fetch_ahead"] == (pipe != 0)
+            if pipe == 2:
+                assert not q.pipeline()["second_step_streamed"]
+            np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"shape {shape} counts")
+            if not count_only:
+                np.testing.assert_array_equal(osum.cpu().numpy(), es, err_msg=f"shape {shape} sums")
+    assert int(ec.sum()) > 0 or rows < 100
+    q.close()

@@ -49,7 +49,9 @@ def test_join_matches_golden(ctx, path):
 
 @pytest.mark.parametrize("vlen,rrows,srows,dup,offset", [
     (1 << 10, 1 << 10, 1_000_003, False, 0),      # LDS-resident vector
-    (32768, 50_000, 2_000_001, True, 1),          # LDS limit, repeated keys, unaligned keys
+    (32768, 50_000, 2_000_001, True, 1),          # 128 KiB in LDS, repeated keys, unaligned keys
+    (40960, 40960, 3_000_001, False, 0),          # the whole LDS (160 KiB), 1024 threads
+    (40961, 40961, 3_000_001, False, 0),          # one entry past it: gathered through L2
     (1 << 22, 1 << 22, 8_000_000, False, 0),      # HBM-resident
     (3_000_017, 2_000_000, 5_000_005, True, 3),   # HBM, repeated, unaligned
 ])
