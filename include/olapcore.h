@@ -202,11 +202,12 @@ olap_status olap_query_destroy(olap_query q);
 int32_t olap_num_shapes(olap_strategy strategy);
 olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *threads,
                             int32_t *rows_per_thread);
-/* Row-wise shapes with a pipeline issue every dependent fetch (the keys of
- * the later probe steps, the measures) one tile ahead: 0 = none (one tile at
- * a time), 1 = pipelined, streaming the second step's keys with the first's
- * when most of their cache lines hold a live row, 2 = pipelined, never
- * streaming them. */
+/* How a row-wise shape fetches: 0 = one tile at a time; 1 = every dependent
+ * fetch (the keys of the later probe steps, the measures) one tile ahead;
+ * 2 = the same, never streaming the second step's keys with the first's
+ * (1 streams them when most of their cache lines hold a live row); 3 / 4 =
+ * survivor queue: the streamed keys two tiles ahead, the rows that pass them
+ * queued per wave and finished 64 / 256 at a time by gathers. */
 int32_t olap_shape_pipeline(olap_strategy strategy, int32_t shape);
 olap_status olap_query_set_shape(olap_query q, int32_t shape);
 /* Aggregation layout of row-wise runs whose group table exceeds LDS:
@@ -243,10 +244,12 @@ olap_status olap_query_launch_info(olap_query q, int32_t *threads, int32_t *rows
 /* bytes of device memory the query holds (packed vectors, scratch) */
 int64_t olap_query_device_bytes(olap_query q);
 /* How the last row-wise run fetched: OLAP_PIPE_AHEAD when its dependent
- * fetches ran one tile ahead, OLAP_PIPE_STREAM2 when the second step's keys
- * streamed with the first's; -1 before any run. */
+ * fetches ran one tile ahead, OLAP_PIPE_QUEUE when it queued survivors,
+ * OLAP_PIPE_STREAM2 when the second step's keys streamed with the first's;
+ * -1 before any run. */
 #define OLAP_PIPE_AHEAD 1
 #define OLAP_PIPE_STREAM2 2
+#define OLAP_PIPE_QUEUE 4
 int32_t olap_query_pipeline(olap_query q);
 
 /* One-shot forms named after the reference drivers; they create, run and

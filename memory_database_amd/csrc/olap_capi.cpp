@@ -343,7 +343,7 @@ olap_status run_rowwise_stage(olap_query_s *q, int64_t *sum, int64_t *count, hip
     q->ran_threads = sh.bs;
     q->ran_rows = sh.unroll * kChunk;
     q->ran_agg = OLAP_AGG_TWOPASS;
-    q->ran_pipe = q->eager1 ? OLAP_PIPE_STREAM2 : 0;
+    q->ran_pipe = 0;
     return OLAP_OK;
 }
 
@@ -391,7 +391,7 @@ olap_status run_rowwise_onepass(olap_query_s *q, int64_t *sum, int64_t *count, h
     q->ran_threads = sh.bs;
     q->ran_rows = sh.unroll * kChunk;
     q->ran_agg = OLAP_AGG_ONEPASS;
-    q->ran_pipe = q->eager1 ? OLAP_PIPE_STREAM2 : 0;
+    q->ran_pipe = 0;
     return OLAP_OK;
 }
 
@@ -1073,11 +1073,7 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
         q->ran_threads = sh.bs;
         q->ran_rows = sh.unroll * kChunk;
         q->ran_agg = L.gt_lds ? OLAP_AGG_LDS : a.hot_bits ? OLAP_AGG_HOT : OLAP_AGG_ATOMICS;
-        q->ran_pipe = 0;
-        if (strategy == OLAP_ROWWISE && vec) {
-            const bool stream2 = nd > 1 && q->eager1 && (sh.pipe == 0 || sh.pipe == 1);
-            q->ran_pipe = (sh.pipe ? OLAP_PIPE_AHEAD : 0) | (stream2 ? OLAP_PIPE_STREAM2 : 0);
-        }
+        q->ran_pipe = strategy == OLAP_ROWWISE ? (int32_t)a.launched : 0;
         return OLAP_OK;
     }
     case OLAP_CWM_SV: {

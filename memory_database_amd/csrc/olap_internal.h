@@ -45,7 +45,9 @@ struct DevProgram {
 struct ScanShape {
     int bs;      // threads per workgroup
     int unroll;  // 128-bit chunks per thread per tile
-    int pipe;    // row-wise: 1 = every dependent fetch issued one tile ahead
+    int pipe;    // row-wise: 0 one tile at a time, 1 / 2 dependent fetches one tile
+                 // ahead (2: second step never streamed), 3 / 4 survivor queue
+                 // drained 1 / 4 rows per lane at a time
 };
 
 // One probe step as the kernels see it. Codes of the step's dimension are
@@ -99,6 +101,10 @@ struct ScanArgs {
     // ids, then u32 fill, chunk.nparts each)
     ChunkArgs chunk;
     uint32_t pc_lds;
+    // row-wise survivor queues: word offset of the per-wave LDS queues
+    uint32_t q_off;
+    uint32_t launched;          // set by the row-wise launcher: OLAP_PIPE_* bits of the
+                                // kernel it chose (not read by the kernels)
 };
 
 }  // namespace olap

@@ -9,6 +9,8 @@
 // mbcnt with one LDS atomic per wave.
 #include "olap_internal.h"
 
+#include <type_traits>
+
 namespace olap {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -850,6 +852,182 @@ __global__ __launch_bounds__(BS) void k_rowwise_pipe(ScanArgs a)
 }
 
 // ------------------------------------------------------------------------
+// row-wise scan with a survivor queue (direct aggregation). The streamed
+// first step(s) are loaded PF tiles ahead and probed as they arrive; the rows
+// that pass them go, with their partial gids, into the wave's LDS queue.
+// Once the queue holds 64 rows, each lane takes one and runs the later steps
+// and the measures for it with 4-byte gathers. When few rows pass the
+// streamed steps, most tiles make no dependent fetch at all, and a wave
+// waits only for keys it issued PF tiles earlier. Rows are numbered in 32
+// bits (the host keeps larger tables off this kernel).
+// ------------------------------------------------------------------------
+
+// queue entries per wave when a drain takes B rows per lane: below 64 B
+// before a chunk's push, which adds at most 256
+constexpr uint32_t queue_cap(int B) { return 64u * (uint32_t)B + 256u; }
+
+uint32_t queue_lds_bytes(int bs, int B) { return (uint32_t)(bs / 64) * 2u * queue_cap(B) * 4u; }
+
+template <int NDIM, bool GLDS, int BS, int U, bool EAGER, int PF, int B>
+__global__ __launch_bounds__(BS) void k_rowwise_queue(ScanArgs a)
+{
+    constexpr uint32_t QCAP = queue_cap(B);
+    constexpr uint32_t BATCH = 64u * B;
+    constexpr uint64_t TILE = (uint64_t)BS * 4u * U;
+    constexpr uint32_t TB = (uint32_t)TILE * 4u;
+    constexpr int FIRST = NDIM > 1 && EAGER ? 2 : 1;
+    typedef __attribute__((address_space(3))) uint32_t lds_u32;
+    extern __shared__ __align__(16) uint32_t lds[];
+    lds_init<NDIM, GLDS>(a, lds);
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    lds_u32 *qrow = (lds_u32 *)lds + a.q_off + (tid >> 6) * (2u * QCAP);
+    lds_u32 *qgid = qrow + QCAP;
+    const uint64_t full = a.rows / TILE;
+    const uint64_t mine = blockIdx.x < full ? (full - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
+    const bool load_m = !a.count_only;
+    const bool has_m2 = a.m2 != nullptr;
+    uint32_t off[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) off[u] = (uint32_t)(u * BS + tid) * 16u;
+
+    // the streamed keys of the next PF tiles: tile i goes to slot i % PF
+    v4i nf0[PF][U], nf1[PF][U];
+    auto issue_first = [&](auto slot, uint64_t i) {
+        constexpr int P = decltype(slot)::value;
+        const bool ok = i < mine;
+        const uint64_t row = ok ? (blockIdx.x + i * gridDim.x) * TILE : 0;
+        const __amdgpu_buffer_rsrc_t r0 = tile_buf(a.st[0].fk, row, ok ? TB : 0u);
+#pragma unroll
+        for (int u = 0; u < U; u++) nf0[P][u] = buf_load4(r0, off[u]);
+        if (FIRST == 2) {
+            const __amdgpu_buffer_rsrc_t r1 = tile_buf(a.st[NDIM > 1 ? 1 : 0].fk, row, ok ? TB : 0u);
+#pragma unroll
+            for (int u = 0; u < U; u++) nf1[P][u] = buf_load4(r1, off[u]);
+        }
+    };
+
+    // the later steps and the measures of the n (<= 64 B) queued rows from
+    // entry base: B rows per lane, their gathers of a step issued together
+    auto drain = [&](uint32_t base, uint32_t n) {
+        uint32_t row[B], g[B];
+        bool alive[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            const uint32_t e = lane + 64u * (uint32_t)j;
+            alive[j] = e < n;
+            row[j] = alive[j] ? qrow[base + e] : 0u;
+            g[j] = alive[j] ? qgid[base + e] : 0u;
+        }
+#pragma unroll
+        for (int s = FIRST; s < NDIM; s++) {
+            int32_t key[B];
+#pragma unroll
+            for (int j = 0; j < B; j++) key[j] = alive[j] ? a.st[s].fk[row[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                if (alive[j]) {
+                    uint32_t code;
+                    alive[j] = probe1_any(a.st[s], lds, key[j], code);
+                    g[j] += code * a.st[s].factor;
+                }
+            }
+        }
+        long long m[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            m[j] = 0;
+            if (alive[j] && load_m) {
+                m[j] = (long long)a.m1[row[j]];
+                if (has_m2) m[j] += (long long)a.m2[row[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; j++)
+            if (alive[j]) aggregate<GLDS>(a, lds, g[j], (unsigned long long)m[j]);
+    };
+
+    uint32_t qn = 0;  // queued rows of this wave (wave-uniform)
+    auto iteration = [&](auto slot, uint64_t it) {
+        constexpr int P = decltype(slot)::value;
+        const bool ok = it < mine;
+        const uint64_t tile_row = ok ? (blockIdx.x + it * gridDim.x) * TILE : 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint32_t alive = ok ? 0xFu : 0u;
+            uint32_t g[4] = {0u, 0u, 0u, 0u};
+            probe4(a.st[0], lds, nf0[P][u], g, alive, true);
+            if (FIRST == 2 && alive) probe4(a.st[NDIM > 1 ? 1 : 0], lds, nf1[P][u], g, alive, false);
+            const uint32_t row0 = (uint32_t)(tile_row + (uint64_t)(u * BS + tid) * 4u);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool want = (alive >> k) & 1u;
+                const unsigned long long mask = __ballot(want);
+                if (mask) {
+                    const uint32_t before = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    if (want) {
+                        qrow[qn + before] = row0 + (uint32_t)k;
+                        qgid[qn + before] = g[k];
+                    }
+                    qn += (uint32_t)__popcll(mask);
+                }
+            }
+            while (qn >= BATCH) {
+                qn -= BATCH;
+                drain(qn, BATCH);
+            }
+        }
+        issue_first(slot, it + PF);
+    };
+
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, (PF > 1 ? 1 : 0)>;
+    using S2 = std::integral_constant<int, (PF > 2 ? 2 : 0)>;
+    using S3 = std::integral_constant<int, (PF > 3 ? 3 : 0)>;
+    issue_first(S0{}, 0);
+    if constexpr (PF > 1) issue_first(S1{}, 1);
+    if constexpr (PF > 2) issue_first(S2{}, 2);
+    if constexpr (PF > 3) issue_first(S3{}, 3);
+    // the iterations past the workgroup's last tile have no live row and load
+    // nothing, so the count rounds up to whole groups of PF
+    const uint64_t iters = (mine + PF - 1) / PF * PF;
+    for (uint64_t it = 0; it < iters; it += PF) {
+        iteration(S0{}, it);
+        if constexpr (PF > 1) iteration(S1{}, it + 1);
+        if constexpr (PF > 2) iteration(S2{}, it + 2);
+        if constexpr (PF > 3) iteration(S3{}, it + 3);
+    }
+    if (qn) drain(0u, qn);
+
+    // rows after the last full tile: every step in turn
+    for (uint64_t r = full * TILE + ((uint64_t)blockIdx.x * BS + tid) * 4u; r < a.rows;
+         r += (uint64_t)gridDim.x * BS * 4u) {
+        uint32_t alive, dummy;
+        uint32_t g[4] = {0u, 0u, 0u, 0u};
+        v4i f = load4<true>(a.st[0].fk, r, a.rows, alive);
+        probe4(a.st[0], lds, f, g, alive, true);
+#pragma unroll
+        for (int s = 1; s < NDIM; s++) {
+            if (!alive) break;
+            f = load4<true>(a.st[s].fk, r, a.rows, dummy);
+            probe4(a.st[s], lds, f, g, alive, false);
+        }
+        if (!alive) continue;
+        v4i m1 = v4i{0, 0, 0, 0}, m2 = v4i{0, 0, 0, 0};
+        if (load_m) {
+            m1 = load4<true>(a.m1, r, a.rows, dummy);
+            if (has_m2) m2 = load4<true>(a.m2, r, a.rows, dummy);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if ((alive >> k) & 1u) aggregate<GLDS>(a, lds, g[k], measure4(a, m1, m2, k));
+    }
+    flush<GLDS>(a, lds);
+}
+
+// ------------------------------------------------------------------------
 // vector-wise static vector (OLAPcore_vwm_sv_thread semantics): every step is
 // probed for every row; all columns of a tile are loaded up front.
 // ------------------------------------------------------------------------
@@ -1252,11 +1430,12 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
 
 // Row-wise launch shapes: (threads per workgroup, 128-bit chunks per thread
 // per tile). The autotuner times them on a live query.
-// The last four run the pipelined kernel (every dependent fetch one tile
-// ahead): pipe 1 streams the second step's keys with the first's when most
-// of their lines are live, pipe 2 never does.
-const ScanShape kRowShapes[] = {{512, 4, 0}, {512, 2, 0}, {1024, 2, 0}, {1024, 4, 0}, {256, 8, 0},
-                                {512, 8, 0}, {1024, 2, 1}, {512, 2, 1},  {1024, 2, 2}, {512, 2, 2}};
+// Shapes 6 and 7 run the pipelined kernel (every dependent fetch one tile
+// ahead), 8-11 the survivor-queue kernel (1024 threads, streamed keys 2
+// tiles ahead, drains of 1 or 4 rows per lane).
+const ScanShape kRowShapes[] = {{512, 4, 0}, {512, 2, 0}, {1024, 2, 0}, {1024, 4, 0},
+                                {256, 8, 0}, {512, 8, 0}, {1024, 2, 1}, {512, 2, 1},
+                                {1024, 2, 3}, {1024, 2, 4}, {1024, 4, 3}, {1024, 4, 4}};
 const int kNumRowShapes = (int)(sizeof(kRowShapes) / sizeof(kRowShapes[0]));
 
 template <int NDIM, bool VEC, bool GLDS>
@@ -1266,8 +1445,32 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
     const ScanShape sh = kRowShapes[shape];
     a.ntiles = (a.rows + (uint64_t)sh.bs * 4u * sh.unroll - 1) / ((uint64_t)sh.bs * 4u * sh.unroll);
     if (a.ntiles == 0) return hipSuccess;
+    if (VEC && sh.pipe >= 3) {
+        // the survivor queues go after the rest of the kernel's LDS; a table
+        // with more rows than 32-bit row numbers, or queues that do not fit,
+        // take the pipelined kernel of the same size
+        const uint32_t qoff = (dyn_lds + 15u) & ~15u;
+        const int B = sh.pipe == 3 ? 1 : 4;
+        const uint32_t dyn = qoff + queue_lds_bytes(1024, B);
+        if (a.rows <= 0xFFFFFFFFull && dyn <= kLdsBytes) {
+            a.q_off = qoff / 4u;
+            const bool eager = NDIM > 1 && a.eager1;
+            a.launched = OLAP_PIPE_QUEUE | (eager ? OLAP_PIPE_STREAM2 : 0);
+#define OLAP_QUEUE(UX, BX)                                                                                        \
+    return eager ? launch_persistent(k_rowwise_queue<NDIM, GLDS, 1024, UX, true, 2, BX>, a, num_cus, dyn, a.ntiles, s, 1024) \
+                 : launch_persistent(k_rowwise_queue<NDIM, GLDS, 1024, UX, false, 2, BX>, a, num_cus, dyn, a.ntiles, s, 1024)
+            if (sh.unroll == 2) {
+                if (B == 1) OLAP_QUEUE(2, 1);
+                OLAP_QUEUE(2, 4);
+            }
+            if (B == 1) OLAP_QUEUE(4, 1);
+            OLAP_QUEUE(4, 4);
+#undef OLAP_QUEUE
+        }
+    }
     if (VEC && sh.pipe) {
-        const bool eager = NDIM > 1 && a.eager1 && sh.pipe == 1;
+        const bool eager = NDIM > 1 && a.eager1 && sh.pipe != 2;
+        a.launched = OLAP_PIPE_AHEAD | (eager ? OLAP_PIPE_STREAM2 : 0);
 #define OLAP_PIPE(BSX, UX)                                                                                  \
     return eager ? launch_persistent(k_rowwise_pipe<NDIM, GLDS, BSX, UX, true>, a, num_cus, dyn_lds, a.ntiles, s, BSX) \
                  : launch_persistent(k_rowwise_pipe<NDIM, GLDS, BSX, UX, false>, a, num_cus, dyn_lds, a.ntiles, s, BSX)
@@ -1275,6 +1478,7 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
         OLAP_PIPE(512, 2);
 #undef OLAP_PIPE
     }
+    a.launched = VEC && NDIM > 1 && a.eager1 ? OLAP_PIPE_STREAM2 : 0;
     if (VEC && NDIM > 1 && a.eager1) {
         // step 1's keys streamed with step 0's
         switch (shape) {
@@ -1300,6 +1504,7 @@ hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bo
                        int shape, hipStream_t s)
 {
     if (shape < 0 || shape >= kNumRowShapes) shape = 0;
+    a.launched = 0;
     if (!vec) {
         // unaligned columns: scalar loads, default shape only
         a.ntiles = (a.rows + kTileRW - 1) / kTileRW;

@@ -662,7 +662,8 @@ class Query:
         p = int(self.ctx.lib.olap_query_pipeline(self.h))
         if p < 0:
             raise OlapError(E_INVALID, "pipeline: the query has not run")
-        return {"fetch_ahead": bool(p & 1), "second_step_streamed": bool(p & 2)}
+        return {"fetch_ahead": bool(p & 1), "survivor_queue": bool(p & 4),
+                "second_step_streamed": bool(p & 2)}
 
     def device_bytes(self) -> int:
         return int(self.ctx.lib.olap_query_device_bytes(self.h))

@@ -574,9 +574,13 @@ def test_row_shapes_every_pipeline_depth(ctx, ncpu, ndims, case):
             q.run("rowwise", None if count_only else osum, ocnt)
             torch.cuda.synchronize()
             pipe = ctx.lib.olap_shape_pipeline(oc.ROWWISE, shape)
-            assert q.pipeline()["fetch_ahead"] == (pipe != 0)
+            ran = q.pipeline()
+            if pipe in (3, 4):  # queues that do not fit the LDS the query leaves take the pipelined form
+                assert ran["survivor_queue"] or (layout == 2 and ran["fetch_ahead"])
+            else:
+                assert ran["fetch_ahead"] == (pipe in (1, 2)) and not ran["survivor_queue"]
             if pipe == 2:
-                assert not q.pipeline()["second_step_streamed"]
+                assert not ran["second_step_streamed"]
             np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"shape {shape} counts")
             if not count_only:
                 np.testing.assert_array_equal(osum.cpu().numpy(), es, err_msg=f"shape {shape} sums")
