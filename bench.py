@@ -696,7 +696,11 @@ def main():
     if not args.no_substeps:
         extra["substeps"] = bench_substeps(ctx, stream)
 
-    kernel = "k_rowwise" if args.strategy == "rowwise" else None
+    # the row-wise kernel the autotuned shape launched names the profile
+    kernel = None
+    if args.strategy == "rowwise":
+        kernel = ("k_rowwise_queue" if launch.get("survivor_queue") else
+                  "k_rowwise_pipe" if launch.get("fetch_ahead") else "k_rowwise")
     traffic, traffic_src, current = (profiled_traffic(kernel, cfg.name) if kernel
                                      else (None, None, None))
 
