@@ -497,8 +497,8 @@ olap_status olap_join_build(olap_ctx ctx, int64_t rows, const int32_t *key, cons
 /* vecjoin probe (:58-64; probe_prefetch :72-82 sums the same): total (host)
  * = sum of vector[key[i]] over the probe relation, in int64 (the reference's
  * int count is its low 32 bits); keys outside [0, vector_len) add nothing
- * and are counted in bad_keys (host, nullable). A vector of up to 32,768
- * entries is probed from LDS. Synchronises. */
+ * and are counted in bad_keys (host, nullable). A vector of up to 40,960
+ * entries (160 KiB) is probed from LDS. Synchronises. */
 olap_status olap_join_probe(olap_ctx ctx, int64_t rows, const int32_t *key, const int32_t *vector,
                             int64_t vector_len, int64_t *total, int64_t *bad_keys, void *stream);
 /* TPC-H Q5's fused snowflake probe, Tri_table_join (code/TPCH_Q5/Q5.cpp:945-978,

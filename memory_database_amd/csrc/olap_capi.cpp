@@ -770,6 +770,7 @@ olap_status olap_ctx_destroy(olap_ctx ctx)
         dev_free(ctx->d_scratch);
         dev_free(ctx->join_win);
         dev_free(ctx->in_vals);
+        delete ctx->vg_stager;
     }
     delete ctx;
     return OLAP_OK;
@@ -1458,8 +1459,34 @@ olap_status olap_vector_group(olap_ctx ctx, int64_t rows, const int32_t *key1, c
     HIPC(ctx, launch_zero_u64((unsigned long long *)group_vector, (size_t)group_num, s));
     HIPC(ctx, launch_zero_u64(dtot, 1, s));
     HIPC(ctx, launch_zero_u32(ctx->d_err, 1, s));
-    HIPC(ctx, launch_vector_group(rows, key1, m1, m2, (unsigned long long *)group_vector, group_num,
-                                  dtot, ctx->d_err, ctx->num_cus, s));
+    // beyond LDS: the staged form (count, scatter records, reduce per
+    // partition in LDS) where its partition state fits LDS, else atomics
+    const int64_t nparts = (group_num + kPartSlots - 1) / kPartSlots;
+    const uint32_t cnt_bytes = align_up((uint32_t)std::min<int64_t>(nparts, kMaxParts) * 4u, 16u);
+    const uint32_t stage_bytes = cnt_bytes + (uint32_t)std::min<int64_t>(nparts, kMaxParts) * 8u;
+    if (rows > 0 && group_num > kVgLdsSlots && group_num <= UINT32_MAX && nparts <= kMaxParts &&
+        stage_bytes + 1024u <= kLdsBytes) {
+        uint32_t grid = 1;
+        HIPC(ctx, vector_group_staged_grid(rows, key1, m1, m2, stage_bytes, ctx->num_cus, &grid));
+        if (!ctx->vg_stager) ctx->vg_stager = new Stager();
+        Stager &S = *ctx->vg_stager;
+        const unsigned long long off0 = 0;
+        const uint32_t G32 = (uint32_t)group_num;
+        HIPC(ctx, S.setup(&off0, &G32, 1, grid, m2 != nullptr, s));
+        unsigned long long *gv = (unsigned long long *)group_vector;
+        HIPC(ctx, run_vector_group_staged(AGG_COUNT, rows, key1, m1, m2, group_num, ctx->d_err,
+                                          S.args(0, cnt_bytes / 4u, gv, nullptr), stage_bytes, grid, s));
+        HIPC(ctx, S.prefix(s));
+        unsigned long long recs = 0;
+        HIPC(ctx, S.total(&recs, s));
+        HIPC(ctx, S.reserve(recs));
+        HIPC(ctx, run_vector_group_staged(AGG_SCATTER, rows, key1, m1, m2, group_num, ctx->d_err,
+                                          S.args(0, cnt_bytes / 4u, gv, nullptr), stage_bytes, grid, s));
+        HIPC(ctx, S.reduce(gv, nullptr, ctx->num_cus, s));
+    } else {
+        HIPC(ctx, launch_vector_group(rows, key1, m1, m2, (unsigned long long *)group_vector, group_num,
+                                      dtot, ctx->d_err, ctx->num_cus, s));
+    }
     HIPC(ctx, launch_sum_u64((const unsigned long long *)group_vector, group_num, dtot,
                              ctx->num_cus, s));
     unsigned long long ht = 0;
@@ -1560,7 +1587,7 @@ olap_status olap_tri_table_join(olap_ctx ctx, const olap_tri_join *j, int64_t *s
                               j->key[1], j->key_bias[1], j->dimvec[1], j->dim_rows[1], j->code_bytes[1],
                               j->price, j->discount, j->rows, j->group_nums,
                               (unsigned long long *)sum, (unsigned long long *)count, dbad,
-                              ctx->num_cus, s));
+                              (uint32_t *)((char *)ctx->d_scratch + 16), ctx->num_cus, s));
     uint32_t hbad = 0;
     HIPC(ctx, hipMemcpyAsync(&hbad, dbad, 4, hipMemcpyDeviceToHost, s));
     HIPC(ctx, hipStreamSynchronize(s));

@@ -135,6 +135,8 @@ struct olap_ctx_s {
     // dimension programs: sorted IN values, grown on demand
     int32_t *in_vals = nullptr;
     size_t in_vals_len = 0;
+    // vector_group beyond LDS: the staged form's scratch (grows only)
+    olap::Stager *vg_stager = nullptr;
 };
 
 struct olap_query_s {
@@ -227,10 +229,20 @@ hipError_t run_cwm_sv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, 
 hipError_t run_cwm_dv_join(ScanArgs &a, int num_cus, uint32_t dyn_lds, bool vec, hipStream_t s);
 hipError_t run_cwm_dv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, hipStream_t s);
 
+// vector_group: the group vector in LDS up to kVgLdsSlots groups, else
+// global atomics (launch_vector_group); or, beyond LDS, the staged form: a
+// COUNT and a SCATTER pass (mode AGG_COUNT / AGG_SCATTER) over one grid, then
+// the Stager's reduce
+constexpr int64_t kVgLdsSlots = 16384;
 hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
                                const int32_t *m2, unsigned long long *gv, int64_t G,
                                unsigned long long *total, uint32_t *bad, int num_cus,
                                hipStream_t s);
+hipError_t vector_group_staged_grid(int64_t rows, const int32_t *key1, const int32_t *m1, const int32_t *m2,
+                                    uint32_t lds_bytes, int num_cus, uint32_t *grid);
+hipError_t run_vector_group_staged(int mode, int64_t rows, const int32_t *key1, const int32_t *m1,
+                                   const int32_t *m2, int64_t G, uint32_t *bad, const StageArgs &st,
+                                   uint32_t lds_bytes, uint32_t grid, hipStream_t s);
 // dimension programs (olap_predicate.hip); err counts rows whose code does
 // not fit an int8 vector
 hipError_t launch_dim_program(const DevProgram &p, int64_t rows, int32_t code_bytes, void *out,
@@ -244,7 +256,7 @@ hipError_t launch_tri_join(const int32_t *skey, int32_t sbias, const void *svec,
                            int32_t sbytes, const int32_t *okey, int32_t obias, const void *ovec,
                            int64_t orows, int32_t obytes, const int32_t *price, const int32_t *disc,
                            int64_t rows, int32_t G, unsigned long long *sum, unsigned long long *cnt,
-                           uint32_t *bad, int num_cus, hipStream_t s);
+                           uint32_t *bad, uint32_t *sampled, int num_cus, hipStream_t s);  // sampled: 2 u32 of scratch
 // dense-vector join sub-step (olap_join.hip); winner: vlen int32 of scratch
 hipError_t launch_join_build(int64_t rows, const int32_t *key, const int32_t *payload,
                              int32_t *winner, int32_t *vec, int64_t vlen, uint32_t *bad,

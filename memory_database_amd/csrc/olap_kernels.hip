@@ -1692,40 +1692,138 @@ hipError_t run_cwm_dv_agg(ScanArgs &a, int num_cus, uint32_t dyn_lds, hipStream_
 // sub-step operators
 // ------------------------------------------------------------------------
 
-constexpr uint32_t kVgLdsSlots = 16384;  // u64 slots of the LDS group vector (128 KB)
+constexpr int kVgBS = 1024;
+constexpr int kVgU = 4;  // 128-bit chunks per lane per loop pass
 
-template <bool GLDS>
-__global__ __launch_bounds__(kBlock) void k_vector_group(int64_t rows, const int32_t *key1,
-                                                         const int32_t *m1, const int32_t *m2,
-                                                         unsigned long long *gv, int64_t G,
-                                                         uint32_t *bad)
+// every row of vector_group: op(k, m1, m2) with k = key - 1 in [0, G); a key
+// outside that range is counted in the return value. VEC: 16-byte aligned
+// columns, 4 keys (and measures) per 128-bit load. NEED_M: the measures are
+// read (the count pass of the staged form reads only the keys). The rows a
+// workgroup visits depend only on the grid, so two passes with one grid
+// visit the same rows per workgroup.
+template <bool VEC, bool NEED_M, typename OP>
+__device__ __forceinline__ uint32_t vg_each(int64_t rows, const int32_t *key1, const int32_t *m1,
+                                            const int32_t *m2, int64_t G, OP op)
 {
-    extern __shared__ __align__(16) uint32_t lds[];
-    unsigned long long *lv = (unsigned long long *)lds;
-    if (GLDS) {
-        for (int64_t g = threadIdx.x; g < G; g += kBlock) lv[g] = 0ull;
-        __syncthreads();
-    }
     uint32_t nbad = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < rows;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int64_t k = (int64_t)key1[i] - 1;
+    auto one = [&](int32_t key, int32_t a, int32_t b) {
+        const int64_t k = (int64_t)key - 1;
         if (k < 0 || k >= G) {
             nbad++;
-            continue;
+            return;
         }
-        long long v = (long long)m1[i];
-        if (m2) v += (long long)m2[i];
-        if (GLDS) atomic_add_u64_lds(&lv[k], (unsigned long long)v);
-        else atomic_add_u64_dev(&gv[k], (unsigned long long)v);
+        op((uint32_t)k, a, b);
+    };
+    int64_t done = 0;
+    if (VEC) {
+        const int64_t chunks = rows / 4;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        const v4i *k4 = (const v4i *)key1;
+        const v4i *a4 = (const v4i *)m1;
+        const v4i *b4 = (const v4i *)m2;
+        for (int64_t c0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c0 < chunks; c0 += stride * kVgU) {
+            v4i kk[kVgU], ma[kVgU], mb[kVgU];
+#pragma unroll
+            for (int u = 0; u < kVgU; u++) {
+                const int64_t c = c0 + u * stride;
+                const bool in = c < chunks;
+                kk[u] = in ? __builtin_nontemporal_load(&k4[c]) : v4i{0, 0, 0, 0};
+                ma[u] = NEED_M && in ? __builtin_nontemporal_load(&a4[c]) : v4i{0, 0, 0, 0};
+                mb[u] = NEED_M && in && m2 ? __builtin_nontemporal_load(&b4[c]) : v4i{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < kVgU; u++) {
+                if (c0 + u * stride >= chunks) break;
+#pragma unroll
+                for (int j = 0; j < 4; j++) one(kk[u][j], ma[u][j], mb[u][j]);
+            }
+        }
+        done = chunks * 4;
     }
+    for (int64_t i = done + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * blockDim.x)
+        one(key1[i], NEED_M ? m1[i] : 0, NEED_M && m2 ? m2[i] : 0);
+    return nbad;
+}
+
+__device__ __forceinline__ void vg_bad(uint32_t nbad, uint32_t *bad)
+{
     for (int o = 32; o > 0; o >>= 1) nbad += __shfl_xor(nbad, o);
     if ((threadIdx.x & 63) == 0 && nbad) atomic_add_u32_dev(bad, nbad);
+}
+
+// group vector in LDS (G <= kVgLdsSlots), flushed with one atomic per slot;
+// or, beyond LDS and where the staged form does not fit, global atomics
+template <bool VEC, bool GLDS>
+__global__ __launch_bounds__(kVgBS) void k_vector_group(int64_t rows, const int32_t *key1,
+                                                        const int32_t *m1, const int32_t *m2,
+                                                        unsigned long long *gv, int64_t G,
+                                                        uint32_t *bad)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    typedef __attribute__((address_space(3))) unsigned long long l_u64;
+    l_u64 *lv = (l_u64 *)lds;
+    if (GLDS) {
+        for (int64_t g = threadIdx.x; g < G; g += kVgBS) lv[g] = 0ull;
+        __syncthreads();
+    }
+    const uint32_t nbad = vg_each<VEC, true>(rows, key1, m1, m2, G, [&](uint32_t k, int32_t a, int32_t b) {
+        const unsigned long long v = (unsigned long long)((long long)a + (long long)b);
+        if (GLDS) __hip_atomic_fetch_add(&lv[k], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else atomic_add_u64_dev(&gv[k], v);
+    });
+    vg_bad(nbad, bad);
     if (GLDS) {
         __syncthreads();
-        for (int64_t g = threadIdx.x; g < G; g += kBlock)
+        for (int64_t g = threadIdx.x; g < G; g += kVgBS)
             if (lv[g]) atomic_add_u64_dev(&gv[g], lv[g]);
     }
+}
+
+// staged form beyond LDS: the COUNT pass counts rows per (workgroup,
+// partition of kPartSlots keys), the SCATTER pass writes one record {key
+// within the partition, m1[, m2]} per row into the workgroup's exact range;
+// the reduce (olap_stage.hip) adds them up per partition in LDS
+template <bool VEC, int MODE>
+__global__ __launch_bounds__(kVgBS) void k_vector_group_staged(int64_t rows, const int32_t *key1,
+                                                               const int32_t *m1, const int32_t *m2,
+                                                               int64_t G, uint32_t *bad, StageArgs st)
+{
+    extern __shared__ __align__(16) uint32_t lds[];
+    stage_begin<MODE>(st, lds);
+    __syncthreads();
+    const uint32_t nbad = vg_each<VEC, MODE == AGG_SCATTER>(
+        rows, key1, m1, m2, G, [&](uint32_t k, int32_t a, int32_t b) {
+            if (MODE == AGG_COUNT) stage_count(st, lds, k >> kPartBits);
+            else stage_scatter(st, lds, k >> kPartBits, k & (kPartSlots - 1u), a, b);
+        });
+    if (MODE == AGG_COUNT) vg_bad(nbad, bad);
+    __syncthreads();
+    stage_end<MODE>(st, lds);
+}
+
+static bool vg_vec(const int32_t *key1, const int32_t *m1, const int32_t *m2)
+{
+    auto al = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    return al(key1) && al(m1) && (!m2 || al(m2));
+}
+
+// grid of a vector_group kernel: workgroups per CU from occupancy (with
+// lds_bytes of dynamic LDS) on every CU, no more than the rows fill
+template <typename K>
+static hipError_t vg_grid(K kernel, uint32_t lds_bytes, int num_cus, int64_t rows, uint32_t *grid)
+{
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kVgBS, lds_bytes);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    *grid = (uint32_t)grid_for(rows, kVgBS * 4 * kVgU, per_cu * num_cus);
+    return hipSuccess;
 }
 
 hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t *m1,
@@ -1735,19 +1833,51 @@ hipError_t launch_vector_group(int64_t rows, const int32_t *key1, const int32_t 
 {
     (void)total;
     if (rows <= 0) return hipSuccess;
-    const int grid = grid_for(rows, kBlock * 16, num_cus * 2);
-    if (G <= (int64_t)kVgLdsSlots) {
-        const uint32_t lds_bytes = (uint32_t)(G * 8);
-        if (lds_bytes > 64 * 1024) {
-            hipError_t e = hipFuncSetAttribute((const void *)k_vector_group<true>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)lds_bytes);
-            if (e != hipSuccess) return e;
-        }
-        k_vector_group<true><<<grid, kBlock, lds_bytes, s>>>(rows, key1, m1, m2, gv, G, bad);
+    const bool vec = vg_vec(key1, m1, m2);
+    const bool glds = G <= (int64_t)kVgLdsSlots;
+    const uint32_t lds = glds ? (uint32_t)(G * 8) : 0u;
+    uint32_t grid = 1;
+    hipError_t e;
+#define OLAP_VG(V, L)                                                                          \
+    e = vg_grid(k_vector_group<V, L>, lds, num_cus, rows, &grid);                              \
+    if (e != hipSuccess) return e;                                                             \
+    k_vector_group<V, L><<<grid, kVgBS, lds, s>>>(rows, key1, m1, m2, gv, G, bad);
+    if (vec && glds) { OLAP_VG(true, true) }
+    else if (vec) { OLAP_VG(true, false) }
+    else if (glds) { OLAP_VG(false, true) }
+    else { OLAP_VG(false, false) }
+#undef OLAP_VG
+    return hipGetLastError();
+}
+
+hipError_t vector_group_staged_grid(int64_t rows, const int32_t *key1, const int32_t *m1, const int32_t *m2,
+                                    uint32_t lds_bytes, int num_cus, uint32_t *grid)
+{
+    uint32_t g0 = 1, g1 = 1;
+    hipError_t e;
+    if (vg_vec(key1, m1, m2)) {
+        e = vg_grid(k_vector_group_staged<true, AGG_COUNT>, lds_bytes, num_cus, rows, &g0);
+        if (e == hipSuccess) e = vg_grid(k_vector_group_staged<true, AGG_SCATTER>, lds_bytes, num_cus, rows, &g1);
     } else {
-        k_vector_group<false><<<grid_for(rows, kBlock * 16, num_cus * 8), kBlock, 0, s>>>(
-            rows, key1, m1, m2, gv, G, bad);
+        e = vg_grid(k_vector_group_staged<false, AGG_COUNT>, lds_bytes, num_cus, rows, &g0);
+        if (e == hipSuccess) e = vg_grid(k_vector_group_staged<false, AGG_SCATTER>, lds_bytes, num_cus, rows, &g1);
+    }
+    *grid = std::min(g0, g1);
+    return e;
+}
+
+hipError_t run_vector_group_staged(int mode, int64_t rows, const int32_t *key1, const int32_t *m1,
+                                   const int32_t *m2, int64_t G, uint32_t *bad, const StageArgs &st,
+                                   uint32_t lds_bytes, uint32_t grid, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    const bool vec = vg_vec(key1, m1, m2);
+    if (mode == AGG_COUNT) {
+        if (vec) k_vector_group_staged<true, AGG_COUNT><<<grid, kVgBS, lds_bytes, s>>>(rows, key1, m1, m2, G, bad, st);
+        else k_vector_group_staged<false, AGG_COUNT><<<grid, kVgBS, lds_bytes, s>>>(rows, key1, m1, m2, G, bad, st);
+    } else {
+        if (vec) k_vector_group_staged<true, AGG_SCATTER><<<grid, kVgBS, lds_bytes, s>>>(rows, key1, m1, m2, G, bad, st);
+        else k_vector_group_staged<false, AGG_SCATTER><<<grid, kVgBS, lds_bytes, s>>>(rows, key1, m1, m2, G, bad, st);
     }
     return hipGetLastError();
 }

@@ -81,7 +81,7 @@ struct StageArgs {
     // columns that have since changed): never lost, added directly
     const unsigned long long *part_out;  // first output slot of each partition
     unsigned long long *gsum;
-    unsigned long long *gcnt;
+    unsigned long long *gcnt;         // null: sums only
 };
 
 #if defined(__HIPCC__)
@@ -104,7 +104,7 @@ __device__ __forceinline__ void stage_overflow(const StageArgs &s, uint32_t p, u
     typedef __attribute__((address_space(1))) unsigned long long s_g_u64;
     const unsigned long long o = s.part_out[p] + gid_low;
     __hip_atomic_fetch_add((s_g_u64 *)&s.gsum[o], (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add((s_g_u64 *)&s.gcnt[o], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s.gcnt) __hip_atomic_fetch_add((s_g_u64 *)&s.gcnt[o], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(s.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -192,6 +192,7 @@ struct Stager {
     hipError_t reserve(unsigned long long records);
     // first partition of query q
     uint32_t part_base(int q) const { return qbase_.empty() ? 0 : qbase_[(size_t)q]; }
+    // (gcnt may be null: sums only)
     hipError_t reduce(unsigned long long *gsum, unsigned long long *gcnt, int num_cus,
                       hipStream_t s);
     StageArgs args(uint32_t cur_lds, uint32_t start_lds, unsigned long long *gsum, unsigned long long *gcnt) const;

@@ -140,16 +140,30 @@ __device__ __forceinline__ void reduce_add(unsigned long long *lsum, uint32_t *l
     }
 }
 
-// the non-empty slots of the LDS table into the outputs
+// the non-empty slots of the LDS table into the outputs. sole: this work
+// item is the only one of its partition, and the scan that wrote the records
+// has finished, so no one else writes these output slots now: a plain
+// read-modify-write replaces the atomics
 __device__ __forceinline__ void reduce_flush(const unsigned long long *lsum, const uint32_t *lcnt, uint32_t slots,
                                              unsigned long long out, unsigned long long *gsum,
-                                             unsigned long long *gcnt)
+                                             unsigned long long *gcnt, bool sole = false)
 {
+    if (sole) {
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            const uint32_t c = lcnt[s];
+            if (c) {
+                if (gcnt) gcnt[out + s] += (unsigned long long)c;
+                gsum[out + s] += lsum[s];
+            }
+        }
+        return;
+    }
     for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
         const uint32_t c = lcnt[s];
         if (c) {
-            __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (gcnt)
+                __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], (unsigned long long)c, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], lsum[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -209,7 +223,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_stage_reduce(
             }
         }
         __syncthreads();
-        reduce_flush(lsum, lcnt, slots, part_out[p], gsum, gcnt);
+        reduce_flush(lsum, lcnt, slots, part_out[p], gsum, gcnt, item_begin[p + 1] - item_begin[p] == 1);
         __syncthreads();
     }
 }

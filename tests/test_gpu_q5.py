@@ -69,3 +69,44 @@ def test_q5_at_scale_factor_100(ctx):
     _say(f"Q5 SF=100: {int(wc.sum())} matching lineitems of {cfg.lineitems}")
     assert torch.equal(out_c, wc) and torch.equal(out_s, ws)
     assert int(wc.sum()) > 0
+
+
+@pytest.mark.parametrize("sparse", ["supplier", "orders"])
+@pytest.mark.parametrize("code_bytes", [1, 4])
+def test_fused_probe_either_vector_first(ctx, sparse, code_bytes):
+    """The probe reads the vector with the smaller non-NULL fraction first
+    and the other only where the first found a code: with either vector the
+    sparse one, int8 or int32 codes, keys outside both vectors and a ragged
+    row count, the result equals the oracle."""
+    import torch
+
+    rng = np.random.default_rng(31 + code_bytes + len(sparse))
+    rows, S, O, G = 3_000_003, 50_000, 700_000, 25
+
+    def vec(n, frac):
+        v = rng.integers(0, G, n, dtype=np.int32)
+        v[rng.random(n) >= frac] = -1
+        return v
+
+    sv = vec(S, 0.02 if sparse == "supplier" else 0.9)
+    ov = vec(O, 0.02 if sparse == "orders" else 0.9)
+    skey = rng.integers(-2, S + 3, rows, dtype=np.int32)   # 1-based, some outside
+    okey = np.sort(rng.integers(-2, O + 2, rows, dtype=np.int32))
+    price = rng.integers(90_000, 10_500_000, rows, dtype=np.int32)
+    disc = rng.integers(0, 11, rows, dtype=np.int32)
+    es, ec, bad = po.tri_table_join(skey, sv, okey, ov, price, disc, G)
+    assert bad == 0 and ec.sum() > 0
+
+    def dev_vec(v):
+        if code_bytes == 4:
+            return torch.from_numpy(v).cuda()
+        return torch.from_numpy(np.where(v == -1, 127, v).astype(np.int8)).cuda()
+
+    d = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    out_s = torch.empty(G, dtype=torch.int64, device="cuda")
+    out_c = torch.empty(G, dtype=torch.int64, device="cuda")
+    _say(f"fused probe: sparse {sparse}, {code_bytes}-byte codes, {rows} rows")
+    ctx.tri_table_join(d(skey), dev_vec(sv), d(okey), dev_vec(ov), d(price), d(disc), G, out_s, out_c)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out_c.cpu().numpy(), ec)
+    np.testing.assert_array_equal(out_s.cpu().numpy(), es)

@@ -116,6 +116,35 @@ def test_vector_group_beyond_lds_matches_oracle(ctx, log2g):
     np.testing.assert_array_equal(out.cpu().numpy(), gv)
 
 
+@pytest.mark.parametrize("log2g,m2,offset", [(17, False, 0), (21, False, 1), (21, True, 3), (27, False, 0)])
+def test_vector_group_forms(ctx, log2g, m2, offset):
+    """The staged form with one measure (8-byte records) and with two, on
+    aligned and unaligned columns (scalar loads), and 2^27 groups, whose
+    partition state does not fit LDS: global atomics."""
+    import torch
+
+    G = 1 << log2g
+    rows = 3_000_007
+    rng = np.random.default_rng(log2g + offset)
+    key = rng.integers(0, G + 2, rows, dtype=np.int32)   # 0 and G + 1 are outside [1, G]
+    a = rng.integers(-(2 ** 31), 2 ** 31 - 1, rows, dtype=np.int32)
+    b = rng.integers(0, 1000, rows, dtype=np.int32) if m2 else None
+    tot, gv, bad = po.vector_group(key, a, b, G)
+
+    def up(x):
+        if x is None:
+            return None
+        buf = torch.empty(x.size + offset, dtype=torch.int32, device="cuda")
+        buf[offset:] = torch.from_numpy(x).cuda()
+        return buf[offset:]
+
+    _say(f"vector_group G=2^{log2g} m2={m2} offset={offset}")
+    out = torch.empty(G, dtype=torch.int64, device="cuda")
+    gt, gbad = ctx.vector_group(up(key), up(a), up(b), out)
+    assert (gt, gbad) == (tot, bad)
+    np.testing.assert_array_equal(out.cpu().numpy(), gv)
+
+
 @pytest.mark.parametrize("log2g", [20, 26])
 def test_vector_group_at_reference_size(ctx, log2g):
     """600M rows (the reference's DATA_NUM_BASE x SF=100) against torch's
