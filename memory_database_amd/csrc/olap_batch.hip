@@ -12,9 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -42,7 +44,8 @@ struct BQDesc {
     uint64_t M[4];      // words 16..23: fast-modulo multiplier UINT64_MAX / g + 1 (0 for g 0)
     uint32_t pbase;     // word 24: first staged-aggregation partition of the query
     uint32_t pbase_c;   // the same with the compact one-pass partitions (2^12 slots)
-    uint32_t pad1[2];
+    uint32_t seg_base;  // segmented kernel: first position of the query's event segment
+    uint32_t seg_cap;   // and its capacity (a multiple of 64, at least 64)
 };
 static_assert(sizeof(BQDesc) == 112, "descriptor layout");
 constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;
@@ -82,6 +85,7 @@ struct BatchArgs {
     // spare chunk ids, then u32 fill, nparts each)
     ChunkArgs chunk;
     uint32_t pc_lds;
+    uint64_t qvalid;        // bit q: q < nq
 };
 
 struct MaskPred {
@@ -385,7 +389,136 @@ __device__ __forceinline__ unsigned long long rotr64(unsigned long long w, uint3
     return (w >> r) | (w << ((64u - r) & 63u));
 }
 
+// w rotated left by r (0..63)
+__device__ __forceinline__ unsigned long long rotl64(unsigned long long w, uint32_t r)
+{
+    return (w << r) | (w >> ((64u - r) & 63u));
+}
+
 typedef int v2i_a4 __attribute__((ext_vector_type(2), aligned(4)));  // a pair at any 4-byte offset
+
+// One event of a wave-uniform query q (descriptor d): the gid from the
+// row's staged attributes and the query's group-by terms, then the
+// aggregation the descriptor names -- its LDS table, global atomics, or a
+// staged record (the COUNT/SCATTER passes or a one-pass chunk). `valid`
+// false is a padding lane: it takes part in the wave's ballots only.
+template <int MODE>
+__device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d, bool valid, uint32_t row,
+                                            uint32_t *lds, const l_u32 *rattr, const l_u32 *rm, uint32_t R,
+                                            stage_l_u32 *pc_cur, stage_l_u32 *pc_spare, stage_l_u32 *pc_fill,
+                                            stage_l_u32 *scur, const stage_l_u64 *sstart, uint32_t CH,
+                                            uint32_t lane)
+{
+    // the row's attributes of the four tables and its measures (at fixed
+    // LDS offsets, issued with the descriptor's scalar loads), then the
+    // group-by terms of every table (a table the query does not group by
+    // adds 0, whatever the staged word holds)
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) const v4u l_cu4;
+    const v4u a4 = ((l_cu4 *)rattr)[row];
+    const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
+    const uint32_t m1v = rm[row];
+    const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
+    const uint32_t G = d.G;
+    const uint32_t lds_tab = d.lds_tab;
+    uint32_t gid = 0u;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
+    const bool ok = valid && gid < G;
+    if (valid && gid >= G)
+        __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long mv = (long long)(int32_t)m1v + (long long)(int32_t)m2v;
+    if (lds_tab != kNoLds) {
+        if (MODE != AGG_SCATTER && ok) b_atomic_lds(lds + lds_tab, G, gid, (unsigned long long)mv);
+    } else if (MODE == AGG_DIRECT) {
+        if (ok) {
+            b_atomic_dev(&a.gsum[d.out_off + gid], (unsigned long long)mv);
+            b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+        }
+    } else {
+        const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
+        const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
+        const uint32_t pbase = compact ? d.pbase_c : d.pbase;
+        const uint32_t part = pbase + (gid >> pbits);
+        const uint32_t glow = gid & ((1u << pbits) - 1u);
+        const long long moff = mv - (long long)a.chunk.mbase;
+        // a measure outside the range 4-byte records were packed for (the
+        // columns changed since the range was taken) is added directly and
+        // takes no record slot
+        bool rec = ok;
+        if (compact && ok && (moff < 0 || moff >= (1ll << kCompactMeasureBits))) {
+            const unsigned long long o = a.chunk.part_out[part] + glow;
+            b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
+            b_atomic_dev(&a.gcnt[o], 1ull);
+            __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            rec = false;
+        }
+        // counters: the COUNT pass's per-partition counts, the SCATTER
+        // pass's cursors, or the one-pass chunk fills
+        stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_fill : scur;
+        // the partition's chunk ids (fixed during the events step), read
+        // before the fill atomic
+        uint32_t id_cur = 0u, id_spare = 0u;
+        if (MODE == AGG_CHUNK && rec) {
+            id_cur = pc_cur[part];
+            id_spare = pc_spare[part];
+        }
+        uint32_t pos = 0u;
+        if (G <= (1u << pbits)) {
+            // one partition: one counter / cursor atomic for the wave
+            const unsigned long long act = __ballot(rec);
+            if (act) {
+                const int leader = __ffsll(act) - 1;
+                // the leader's rank is 0: adding it keeps the address per
+                // lane, so the compiler does not rewrite this single-lane
+                // atomic as a wave reduction
+                const uint32_t rank = lane_rank(act);
+                uint32_t at = 0u;
+                if ((int)lane == leader)
+                    at = __hip_atomic_fetch_add(ctr + pbase + rank, (uint32_t)__popcll(act),
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                pos = (uint32_t)__builtin_amdgcn_readlane((int)at, leader) + rank;
+            }
+        } else if (rec) {
+            pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (MODE == AGG_CHUNK && rec) {
+            // a round adds at most R = CH records to a partition whose fill
+            // is below CH: pos < 2 CH
+            const uint32_t id = pos < CH ? id_cur : id_spare;
+            const uint32_t off = pos < CH ? pos : pos - CH;
+            const unsigned long long base = (unsigned long long)id << a.chunk.ch_bits;
+            if (id == kNoChunk) {
+                // no chunk (the pool is used up): added directly
+                const unsigned long long o = a.chunk.part_out[part] + glow;
+                b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
+                b_atomic_dev(&a.gcnt[o], 1ull);
+                __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (compact) {
+                ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | (uint32_t)moff;
+            } else if (a.m2) {
+                ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, m1v, m2v, 0u);
+            } else {
+                // one measure: a chunk holds CH u32 measures, then CH u16
+                // slots (6 bytes a record)
+                char *cp = (char *)a.chunk.pool + 6ull * base;
+                ((uint32_t *)cp)[off] = m1v;
+                ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
+            }
+        }
+        if (MODE == AGG_SCATTER && ok) {
+            const unsigned long long idx = sstart[part] + pos;
+            if (idx >= a.stage.cap) {
+                stage_overflow(a.stage, part, gid & (kPartSlots - 1u), mv);
+            } else if (a.stage.rec4) {
+                a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), m1v, m2v, 0u);
+            } else {
+                a.stage.rec2[idx] = make_uint2(gid & (kPartSlots - 1u), m1v);
+            }
+        }
+    }
+}
 
 template <int BS, int MODE>
 __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
@@ -658,117 +791,8 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
                 const bool valid = (e & 0xffffu) != 0xffffu;
                 const uint32_t row = valid ? (e & 0xffffu) : 0u;
                 const c_desc &d = ((const c_desc *)a.desc)[q];
-                // the row's attributes of the four tables and its measures
-                // (at fixed LDS offsets, issued with the descriptor's scalar
-                // loads), then the group-by terms of every table (a table
-                // the query does not group by adds 0, whatever the staged
-                // word holds)
-                typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-                typedef __attribute__((address_space(3))) const v4u l_cu4;
-                const v4u a4 = ((l_cu4 *)rattr)[row];
-                const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
-                const uint32_t m1v = rm[row];
-                const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
-                const uint32_t G = d.G;
-                const uint32_t lds_tab = d.lds_tab;
-                uint32_t gid = 0u;
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
-                const bool ok = valid && gid < G;
-                if (valid && gid >= G)
-                    __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const long long mv = (long long)(int32_t)m1v + (long long)(int32_t)m2v;
-                if (lds_tab != kNoLds) {
-                    if (MODE != AGG_SCATTER && ok) b_atomic_lds(lds + lds_tab, G, gid, (unsigned long long)mv);
-                } else if (MODE == AGG_DIRECT) {
-                    if (ok) {
-                        b_atomic_dev(&a.gsum[d.out_off + gid], (unsigned long long)mv);
-                        b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
-                    }
-                } else {
-                    const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
-                    const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
-                    const uint32_t pbase = compact ? d.pbase_c : d.pbase;
-                    const uint32_t part = pbase + (gid >> pbits);
-                    const uint32_t glow = gid & ((1u << pbits) - 1u);
-                    const long long moff = mv - (long long)a.chunk.mbase;
-                    // a measure outside the range 4-byte records were packed
-                    // for (the columns changed since the range was taken) is
-                    // added directly and takes no record slot
-                    bool rec = ok;
-                    if (compact && ok && (moff < 0 || moff >= (1ll << kCompactMeasureBits))) {
-                        const unsigned long long o = a.chunk.part_out[part] + glow;
-                        b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
-                        b_atomic_dev(&a.gcnt[o], 1ull);
-                        __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        rec = false;
-                    }
-                    // counters: the COUNT pass's per-partition counts, the
-                    // SCATTER pass's cursors, or the one-pass chunk fills
-                    stage_l_u32 *ctr = MODE == AGG_CHUNK ? pc_fill : scur;
-                    // the partition's chunk ids (fixed during the events
-                    // step), read before the fill atomic
-                    uint32_t id_cur = 0u, id_spare = 0u;
-                    if (MODE == AGG_CHUNK && rec) {
-                        id_cur = pc_cur[part];
-                        id_spare = pc_spare[part];
-                    }
-                    uint32_t pos = 0u;
-                    if (G <= (1u << pbits)) {
-                        // one partition: one counter / cursor atomic for the wave
-                        const unsigned long long act = __ballot(rec);
-                        if (act) {
-                            const int leader = __ffsll(act) - 1;
-                            // the leader's rank is 0: adding it keeps the
-                            // address per lane, so the compiler does not
-                            // rewrite this single-lane atomic as a wave
-                            // reduction
-                            const uint32_t rank = lane_rank(act);
-                            uint32_t at = 0u;
-                            if ((int)lane == leader)
-                                at = __hip_atomic_fetch_add(ctr + pbase + rank, (uint32_t)__popcll(act),
-                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            pos = (uint32_t)__builtin_amdgcn_readlane((int)at, leader) + rank;
-                        }
-                    } else if (rec) {
-                        pos = __hip_atomic_fetch_add(ctr + part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    if (MODE == AGG_CHUNK && rec) {
-                        // a round adds at most R = CH records to a partition
-                        // whose fill is below CH: pos < 2 CH
-                        const uint32_t id = pos < CH ? id_cur : id_spare;
-                        const uint32_t off = pos < CH ? pos : pos - CH;
-                        const unsigned long long base = (unsigned long long)id << a.chunk.ch_bits;
-                        if (id == kNoChunk) {
-                            // no chunk (the pool is used up): added directly
-                            const unsigned long long o = a.chunk.part_out[part] + glow;
-                            b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
-                            b_atomic_dev(&a.gcnt[o], 1ull);
-                            __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        } else if (compact) {
-                            ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | (uint32_t)moff;
-                        } else if (a.m2) {
-                            ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, m1v, m2v, 0u);
-                        } else {
-                            // one measure: a chunk holds CH u32 measures,
-                            // then CH u16 slots (6 bytes a record)
-                            char *cp = (char *)a.chunk.pool + 6ull * base;
-                            ((uint32_t *)cp)[off] = m1v;
-                            ((uint16_t *)(cp + 4u * CH))[off] = (uint16_t)glow;
-                        }
-                    }
-                    if (MODE == AGG_SCATTER && ok) {
-                        const unsigned long long idx = sstart[part] + pos;
-                        if (idx >= a.stage.cap) {
-                            stage_overflow(a.stage, part, gid & (kPartSlots - 1u), mv);
-                        } else if (a.stage.rec4) {
-                            a.stage.rec4[idx] = make_uint4(gid & (kPartSlots - 1u), m1v, m2v, 0u);
-                        } else {
-                            a.stage.rec2[idx] = make_uint2(gid & (kPartSlots - 1u), m1v);
-                        }
-                    }
-                }
+                batch_event<MODE>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, scur,
+                                  sstart, CH, lane);
             }
             if (late_gathers) gather_c();
             late_gathers = false;
@@ -812,6 +836,304 @@ __global__ __launch_bounds__(BS) void k_batch_ev(BatchArgs a)
     }
 }
 
+// ------------------------------------------------------- segmented events
+// Event-grouped scan without the count step. Every query owns a fixed
+// segment of the tile's event list (16-bit row numbers), sized on the host
+// from its expected matches per tile with room to spare, so a tile takes
+// two barriers instead of four:
+//  1. rows: as in k_batch_ev;
+//  2. list: one LDS atomic per (row, query) match on the query's segment
+//     word (its end in the high half, its cursor in the low half) places
+//     the row in the segment; a match past the segment's end (more matches
+//     in the tile than the segment holds) is aggregated at once by its lane
+//     with global atomics (batch_event_direct);
+//  3. events: every wave numbers the chunks of 64 placed events of all
+//     queries itself (a prefix over the 64 segment fills) and takes every
+//     NW-th chunk; the events of a chunk are one query's, as in k_batch_ev.
+// The segment words alternate between two copies by tile: the copy of the
+// next tile is reset during this tile's events step. One-pass staging only
+// (AGG_CHUNK); the chunk retirement runs at the start of each list step,
+// where no step touches the chunk state.
+
+// a match of query q (per lane) aggregated directly into the outputs: the
+// descriptor is read per lane, the row's attributes and measures from LDS
+__device__ __forceinline__ void batch_event_direct(const BatchArgs &a, uint32_t q, uint32_t row, const l_u32 *rattr,
+                                                const l_u32 *rm, uint32_t R)
+{
+    const BQDesc &d = a.desc[q];
+    uint32_t gid = 0u;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)rattr[4u * row + (uint32_t)t], (uint64_t)d.g[t]) * d.fac[t];
+    if (gid >= d.G) {
+        __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const long long mv = (long long)(int32_t)rm[row] + (a.m2 ? (long long)(int32_t)rm[R + row] : 0ll);
+    b_atomic_dev(&a.gsum[d.out_off + gid], (unsigned long long)mv);
+    b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_seg(BatchArgs a)
+{
+    constexpr int MODE = AGG_CHUNK;
+    constexpr uint32_t R = 2u * BS;
+    extern __shared__ __align__(16) uint32_t lds[];
+    for (int t = 0; t < 4; t++) {
+        if (a.mask_lds[t] != kNoLds) {
+            const uint32_t *src = (const uint32_t *)a.gmask[t];
+            for (uint32_t i = threadIdx.x; i < 2u * a.dim_rows[t]; i += BS) lds[a.mask_lds[t] + i] = src[i];
+        }
+        if (a.attr_lds[t] != kNoLds)
+            for (uint32_t i = threadIdx.x; i < a.dim_rows[t]; i += BS) lds[a.attr_lds[t] + i] = a.grec[t][i].z;
+    }
+    for (uint32_t i = a.tab_begin + threadIdx.x; i < a.tab_end; i += BS) lds[i] = 0u;
+    const uint32_t np = a.chunk.nparts;
+    const uint32_t CH = 1u << a.chunk.ch_bits;
+    stage_l_u32 *pc_cur = (stage_l_u32 *)lds + a.pc_lds;
+    stage_l_u32 *pc_spare = pc_cur + np;
+    stage_l_u32 *pc_fill = pc_spare + np;
+    for (uint32_t p = threadIdx.x; p < np; p += BS) {
+        pc_cur[p] = chunk_claim(a.chunk, p);
+        pc_spare[p] = chunk_claim(a.chunk, p);
+        pc_fill[p] = 0u;
+    }
+    l_u32 *L = (l_u32 *)lds;
+    l_u32 *qseg = L + a.cnt_lds;       // [2][64] segment words: end << 16 | cursor
+    l_u32 *sinit = qseg + 128u;        // [64] their value at the start of a tile
+    l_u32 *rattr = L + a.row_lds;      // [R] the row's four attributes (16 bytes a row)
+    l_u32 *rm = L + a.m_lds;           // [R] m1, then [R] m2
+    typedef __attribute__((address_space(3))) uint16_t l_u16;
+    l_u16 *ev = (l_u16 *)(L + a.ev_lds);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    if (tid < 64u) {
+        uint32_t w = 0u;
+        if (tid < a.nq) {
+            const uint32_t b = a.desc[tid].seg_base, c = a.desc[tid].seg_cap;
+            w = ((b + c) << 16) | b;
+        }
+        sinit[tid] = w;
+        qseg[tid] = w;
+        qseg[64u + tid] = w;
+    }
+    __syncthreads();
+    const uint64_t rows = a.rows;
+    // the next tile's keys and measures, and its gathers of the records of
+    // dimension rows outside LDS, are issued one tile ahead (as in
+    // k_batch_ev): the gathers of each row's first member after the list
+    // step, of its second after the events step
+    const uint64_t rlast = rows - 2u;
+    const int32_t *colp[6];
+#pragma unroll
+    for (int t = 0; t < 4; t++) colp[t] = (a.used >> t) & 1u ? a.fk[t] : a.m1;
+    colp[4] = a.m1;
+    colp[5] = a.m2 ? a.m2 : a.m1;
+    int2 nk[4], nm1, nm2;
+    uint4 ng[4][2];
+    auto load_tile = [&](uint64_t tl) {
+        const uint64_t r = tl * R + 2u * tid;
+        const uint64_t rl = min(r, rlast);
+        const bool shifted = r != rl;
+        auto ld2 = [&](const int32_t *p) {
+            const v2i_a4 v = __builtin_nontemporal_load((const v2i_a4 *)(p + rl));
+            return make_int2(shifted ? v.y : v.x, v.y);
+        };
+#pragma unroll
+        for (int t = 0; t < 4; t++) nk[t] = ld2(colp[t]);
+        nm1 = ld2(colp[4]);
+        nm2 = ld2(colp[5]);
+    };
+    auto gather = [&](int t, int j) {
+        const bool hbm = ((a.used >> t) & 1u) && a.mask_lds[t] == kNoLds;
+        const uint4 *g = hbm ? a.grec[t] : a.grec_any;
+        const uint32_t kmax = hbm ? max(a.dim_rows[t], 1u) - 1u : 0u;
+        ng[t][j] = g[min((uint32_t)(j ? nk[t].y : nk[t].x), kmax)];
+    };
+    auto gather_first = [&]() { gather(0, 0); gather(1, 0); gather(2, 0); gather(3, 0); };
+    auto gather_second = [&]() { gather(0, 1); gather(1, 1); gather(2, 1); gather(3, 1); };
+    if (blockIdx.x < a.ntiles) {
+        load_tile(blockIdx.x);
+        gather_first();
+        gather_second();
+    }
+    auto retire_chunks = [&]() {
+        for (uint32_t p = tid; p < np; p += BS) {
+            const uint32_t f = pc_fill[p];
+            if (f >= CH) {
+                const uint32_t cur = pc_cur[p];
+                if (cur != kNoChunk) a.chunk.chunk_fill[cur] = CH;
+                pc_cur[p] = pc_spare[p];
+                pc_fill[p] = f - CH;
+                pc_spare[p] = chunk_claim(a.chunk, p);
+            }
+        }
+    };
+    uint32_t par = 0u;  // the copy of the segment words this tile uses
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const uint64_t r0 = tile * R + 2u * tid;
+        const uint64_t next = tile + gridDim.x < a.ntiles ? tile + gridDim.x : tile;
+        int2 ck[4];
+        uint4 cg[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            ck[t] = nk[t];
+            cg[t][0] = ng[t][0];
+            cg[t][1] = ng[t][1];
+        }
+        const int2 cm1 = nm1, cm2 = nm2;
+        // 1. rows
+        unsigned long long al0 = r0 < rows ? a.qvalid : 0ull;
+        unsigned long long al1 = r0 + 1 < rows ? a.qvalid : 0ull;
+        uint32_t atr[2][4] = {{0u, 0u, 0u, 0u}, {0u, 0u, 0u, 0u}};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            uint32_t at0 = 0u, at1 = 0u;
+            if ((a.used >> t) & 1u) {
+                const int2 k = ck[t];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint32_t key = (uint32_t)(j ? k.y : k.x);
+                    unsigned long long &al = j ? al1 : al0;
+                    uint32_t &at = j ? at1 : at0;
+                    if (al) {
+                        if (key < a.dim_rows[t]) {
+                            if (a.mask_lds[t] != kNoLds) {
+                                const l_u64 *w = (const l_u64 *)(L + a.mask_lds[t]);
+                                al &= w[key];
+                                at = L[a.attr_lds[t] + key];
+                            } else {
+                                const uint4 rec = cg[t][j];
+                                al &= (unsigned long long)rec.x | ((unsigned long long)rec.y << 32);
+                                at = rec.z;
+                            }
+                        } else {
+                            al &= a.nouse[t];
+                        }
+                    }
+                }
+                atr[0][t] = at0;
+                atr[1][t] = at1;
+            }
+        }
+        {
+            typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+            typedef __attribute__((address_space(3))) v4u l_u4;
+            l_u4 *r4 = (l_u4 *)rattr;
+            r4[2u * tid] = v4u{atr[0][0], atr[0][1], atr[0][2], atr[0][3]};
+            r4[2u * tid + 1u] = v4u{atr[1][0], atr[1][1], atr[1][2], atr[1][3]};
+        }
+        if (al0 | al1) {
+            rm[2u * tid] = (uint32_t)cm1.x;
+            rm[2u * tid + 1u] = (uint32_t)cm1.y;
+            if (a.m2) {
+                rm[R + 2u * tid] = (uint32_t)cm2.x;
+                rm[R + 2u * tid + 1u] = (uint32_t)cm2.y;
+            }
+        }
+        load_tile(next);
+        retire_chunks();  // after the previous tile's events
+        // 2. list: each lane walks its masks from its own lane number on (bit
+        // b of the rotated mask is query (b + lane) mod 64), so the lanes of
+        // one atomic instruction mostly hit different words; a match past
+        // its segment's end writes the spare slot after the list and is
+        // aggregated directly
+        l_u32 *seg = qseg + 64u * par;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint16_t row = (uint16_t)(2u * tid + (uint32_t)j);
+            for (unsigned long long w = rotr64(j ? al1 : al0, lane); w; w &= w - 1ull) {
+                const uint32_t q = ((uint32_t)__builtin_ctzll(w) + lane) & 63u;
+                const uint32_t v = __hip_atomic_fetch_add(seg + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t pos = v & 0xffffu;
+                const bool fits = pos < (v >> 16);
+                ev[fits ? pos : a.ev_cap] = row;
+                if (!fits) batch_event_direct(a, q, row, rattr, rm, R);
+            }
+        }
+        gather_first();
+        lds_barrier();
+        // 3. events; wave 0 resets the other copy for the next tile. Every
+        // wave numbers the chunks of 64 events of all queries itself (lane
+        // q: query q's events placed in this tile, its chunks and the
+        // exclusive sum of the chunks before it) and takes every NW-th
+        // chunk, so the waves' shares differ by at most one chunk
+        if (tid < 64u) qseg[64u * (par ^ 1u) + tid] = sinit[tid];
+        const uint32_t sbase = sinit[lane] & 0xffffu;
+        uint32_t nq_l, cstart, ctot;
+        {
+            const uint32_t v = seg[lane];
+            nq_l = min(v & 0xffffu, v >> 16) - sbase;
+            const uint32_t c = (nq_l + 63u) >> 6;
+            uint32_t incl = c;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)incl, d);
+                if (lane >= (uint32_t)d) incl += u;
+            }
+            cstart = incl - c;
+            ctot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            nq_l = c ? nq_l : 0u;
+        }
+        // chunk i belongs to the last query whose chunks start at or before i
+        // (queries without events take no chunk)
+        auto owner = [&](uint32_t i) {
+            const unsigned long long m = __ballot(cstart <= i && nq_l != 0u);
+            return 63u - (uint32_t)__builtin_clzll(m);
+        };
+        constexpr uint32_t NW = BS / 64u;
+        const uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+        // each chunk's event words are read one chunk ahead (an index past
+        // the last chunk reads the last one again)
+        uint32_t i = w0;
+        uint32_t qn = 0u, e_next = 0u;
+        if (i < ctot) {
+            qn = owner(i);
+            const uint32_t k64 = (i - (uint32_t)__builtin_amdgcn_readlane((int)cstart, (int)qn)) << 6;
+            e_next = ev[(uint32_t)__builtin_amdgcn_readlane((int)sbase, (int)qn) + k64 + lane];
+        }
+        for (; i < ctot; i += NW) {
+            const uint32_t q = qn;
+            const uint32_t k64 = (i - (uint32_t)__builtin_amdgcn_readlane((int)cstart, (int)q)) << 6;
+            const uint32_t e = e_next;
+            const uint32_t inext = min(i + NW, ctot - 1u);
+            qn = owner(inext);
+            e_next = ev[(uint32_t)__builtin_amdgcn_readlane((int)sbase, (int)qn) +
+                        (((inext - (uint32_t)__builtin_amdgcn_readlane((int)cstart, (int)qn)) << 6)) + lane];
+            const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)nq_l, (int)q) - k64;
+            const bool valid = lane < n;
+            const uint32_t row = valid ? e : 0u;
+            const c_desc &d = ((const c_desc *)a.desc)[q];
+            batch_event<MODE>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, nullptr, nullptr,
+                              CH, lane);
+        }
+        gather_second();
+        lds_barrier();
+        par ^= 1u;
+    }
+    retire_chunks();  // after the last tile's events
+    lds_barrier();
+    for (uint32_t p = tid; p < np; p += BS) {
+        if (pc_cur[p] != kNoChunk) a.chunk.chunk_fill[pc_cur[p]] = pc_fill[p];
+        if (pc_spare[p] != kNoChunk) a.chunk.chunk_fill[pc_spare[p]] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < a.nq; q++) {
+        const BQDesc &d = a.desc[q];
+        if (d.lds_tab == kNoLds) continue;
+        const l_u64 *ls = (const l_u64 *)((l_u32 *)lds + d.lds_tab);
+        const l_u32 *lc = (l_u32 *)lds + d.lds_tab + 2u * d.G;
+        for (uint32_t g = tid; g < d.G; g += BS) {
+            const uint32_t c = lc[g];
+            if (c) {
+                b_atomic_dev(&a.gcnt[d.out_off + g], (unsigned long long)c);
+                b_atomic_dev(&a.gsum[d.out_off + g], ls[g]);
+            }
+        }
+    }
+}
+
 // (threads, 128-bit chunks per thread per tile, LDS KiB for query tables,
 // staged aggregation of the queries without an LDS table, event-grouped
 // kernel with two rows per thread, copies of small tables in LDS)
@@ -823,6 +1145,7 @@ struct BatchCfg {
     int grouped;
     int small;
     int evk;   // grouped: event-list capacity in events per tile row (at most)
+    int seg;   // grouped, one-pass: fixed per-query event segments (k_batch_seg)
 };
 const BatchCfg kBatchCfgs[] = {
     {512, 1, 48, 0, 0, 1, 0},   {512, 1, 96, 0, 0, 1, 0},   {256, 1, 48, 0, 0, 1, 0},
@@ -835,12 +1158,20 @@ const BatchCfg kBatchCfgs[] = {
     {1024, 1, 8, 2, 1, 0, 2},   {512, 1, 8, 2, 1, 0, 4},    {512, 1, 0, 2, 1, 0, 4},
     {512, 1, 0, 2, 1, 0, 3},    {512, 1, 8, 2, 1, 0, 6},    {512, 1, 8, 2, 1, 0, 8},
     {1024, 1, 8, 2, 1, 0, 4},   {512, 1, 8, 1, 1, 0, 8},    {512, 1, 16, 2, 1, 0, 4},
-    {1024, 1, 0, 2, 1, 1, 6},   {1024, 1, 0, 2, 1, 1, 8},   {1024, 1, 4, 2, 1, 1, 6}};
+    {1024, 1, 0, 2, 1, 1, 6},   {1024, 1, 0, 2, 1, 1, 8},   {1024, 1, 4, 2, 1, 1, 6},
+    {1024, 1, 0, 2, 1, 1, 0, 1}, {1024, 1, 4, 2, 1, 1, 0, 1}, {512, 1, 0, 2, 1, 0, 0, 1}};
 const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
+// segmented kernel: at most kSegEvents events in a tile's list (16-bit
+// positions that leave the segment word's cursor room to count past its end)
+constexpr uint32_t kSegEvents = 32768;
 
 template <int MODE>
 static const void *batch_kernel(const BatchCfg &c)
 {
+    if (c.seg) {
+        if (MODE != AGG_CHUNK) return nullptr;
+        return c.bs == 512 ? (const void *)k_batch_seg<512> : (const void *)k_batch_seg<1024>;
+    }
     if (c.grouped) return c.bs == 512 ? (const void *)k_batch_ev<512, MODE> : (const void *)k_batch_ev<1024, MODE>;
     if (MODE == AGG_CHUNK) return nullptr;  // one-pass staging exists in the grouped kernel only
     if (c.bs == 256) return c.unroll == 1 ? (const void *)k_batch<256, 1, MODE> : (const void *)k_batch<256, 2, MODE>;
@@ -1022,6 +1353,55 @@ olap_status bhip(olap_ctx ctx, hipError_t e, const char *what)
 
 uint32_t up16(uint32_t bytes) { return (bytes + 15u) & ~15u; }
 
+// Segments of the segmented kernel's event list: query q expects
+// E = R * sel[q] matches a tile; its segment holds E + 4 sqrt(E) rounded up
+// to whole chunks of 64 (at least one), scaled down to the list when the sum
+// exceeds it; the chunks left over go to the queries in turn, most matches
+// first, so skewed data leaves fewer residual passes.
+void plan_segments(olap_batch b, const BatchCfg &cfg, uint32_t cap_events, std::vector<BQDesc> &desc)
+{
+    const int nq = b->nq;
+    const double R = (double)batch_tile_rows(cfg);
+    const uint32_t chunks = cap_events / 64u;   // >= nq (ev_min)
+    std::vector<uint32_t> k((size_t)nq);
+    uint64_t total = 0;
+    for (int q = 0; q < nq; q++) {
+        const double E = std::min(R, std::max(0.0, b->sel[(size_t)q]) * R);
+        k[(size_t)q] = std::max<uint32_t>(1u, (uint32_t)std::ceil((E + 4.0 * std::sqrt(E)) / 64.0));
+        total += k[(size_t)q];
+    }
+    if (total > chunks) {
+        uint64_t t2 = 0;
+        for (int q = 0; q < nq; q++) {
+            k[(size_t)q] = std::max<uint32_t>(1u, (uint32_t)((uint64_t)k[(size_t)q] * (chunks - (uint32_t)nq) / total));
+            t2 += k[(size_t)q];
+        }
+        total = t2;  // sum of max(1, floor(k (chunks - nq) / total)) <= chunks
+    }
+    std::vector<int> order((size_t)nq);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return b->sel[(size_t)x] > b->sel[(size_t)y]; });
+    // a segment never needs more than a tile's rows
+    const uint32_t kmax = (uint32_t)(R / 64.0);
+    for (bool grew = true; total < chunks && grew;) {
+        grew = false;
+        for (int q : order) {
+            if (total >= chunks) break;
+            if (k[(size_t)q] < kmax) {
+                k[(size_t)q]++;
+                total++;
+                grew = true;
+            }
+        }
+    }
+    uint32_t base = 0;
+    for (int q = 0; q < nq; q++) {
+        desc[(size_t)q].seg_base = base;
+        desc[(size_t)q].seg_cap = 64u * k[(size_t)q];
+        base += 64u * k[(size_t)q];
+    }
+}
+
 // LDS layout of one configuration: descriptors, then masks/attributes of the
 // smallest tables, then per-query group tables by events per byte.
 olap_status plan_config(olap_batch b, int c)
@@ -1043,11 +1423,14 @@ olap_status plan_config(olap_batch b, int c)
     // the event-grouped kernel stages per-tile rows and at least one event
     // per row of the tile
     const uint32_t R = (uint32_t)batch_tile_rows(cfg);
-    const uint32_t ev_bytes = 4u;
-    const uint32_t ev_min = cfg.grouped ? ev_bytes * (R + 64u) : 0u;
+    // (the segmented kernel: 16-bit events, at least 64 per query)
+    const uint32_t ev_bytes = cfg.seg ? 2u : 4u;
+    const uint32_t ev_min = cfg.seg ? ev_bytes * 64u * (uint32_t)b->nq : cfg.grouped ? ev_bytes * (R + 64u) : 0u;
     if (cfg.grouped) {
         a.cnt_lds = cur / 4u;
-        cur += up16(4u * (64u + 64u + 64u + 65u));
+        // counters, cursors, starts and the total; or the two copies of the
+        // segment words and their initial values
+        cur += cfg.seg ? up16(4u * (128u + 64u)) : up16(4u * (64u + 64u + 64u + 65u));
         a.row_lds = cur / 4u;
         cur += 16u * R;
         a.m_lds = cur / 4u;
@@ -1133,13 +1516,19 @@ olap_status plan_config(olap_batch b, int c)
     b->stage_lds[c] = cur / 4u;
     cur += stage_bytes;
     if (cfg.staged == 2) a.pc_lds = b->stage_lds[c];
-    if (cfg.grouped) {
+    if (cfg.seg) {
+        // the rest of LDS holds the event segments
+        a.ev_cap = std::min<uint32_t>(kSegEvents, (kLdsBytes - 1024u - 16u - cur) / ev_bytes) & ~63u;
+        a.ev_lds = cur / 4u;
+        cur += up16(ev_bytes * (a.ev_cap + 1u));  // (+ the spare slot of matches past a segment)
+    } else if (cfg.grouped) {
         // the rest of LDS holds the event list (at most evk events per row)
         a.ev_cap = std::min<uint32_t>(std::max<uint32_t>((uint32_t)cfg.evk * R, R + 64u),
                                       (kLdsBytes - 1024u - cur) / ev_bytes) & ~63u;
         a.ev_lds = cur / 4u;
         cur += 4u * a.ev_cap;
     }
+    if (cfg.seg) plan_segments(b, cfg, a.ev_cap, desc);
     uint32_t pbase = 0, pbase_c = 0;
     for (int q = 0; q < b->nq; q++) {
         const olap_batch_query &Q = b->queries[q];
@@ -1179,6 +1568,7 @@ olap_status plan_config(olap_batch b, int c)
     a.m2 = b->star.m2;
     a.rows = (uint64_t)b->star.rows;
     a.nq = (uint32_t)b->nq;
+    a.qvalid = b->nq >= 64 ? ~0ull : (1ull << b->nq) - 1ull;
     a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);

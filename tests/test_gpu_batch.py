@@ -342,3 +342,66 @@ def test_batch_graph_replay_after_columns_grow_matches(ctx, aggregation, new_mea
     assert torch.equal(fsum, es) and torch.equal(fcnt, ec), "fresh batch"
     b2.close()
     b.close()
+
+
+def test_batch_segments_overflow_into_residual_passes(ctx, ncpu):
+    """Fact keys concentrated on the dimension rows that every query passes.
+    The segmented kernel sizes each query's event segment from the pass
+    fractions of the dimension rows (2% per table here), but most fact rows
+    match most queries, so the segments overflow in every tile and the
+    residual passes place the rest. Every configuration equals the oracle."""
+    import torch
+
+    rows = 200_003
+    rng = np.random.default_rng(5)
+    dims = [2556, 30000, 200000, 2000]
+    filters = [rng.integers(0, 100, d, dtype=np.int32) for d in dims]
+    attrs = [rng.integers(0, 1 << 20, d, dtype=np.int32) for d in dims]
+    fks = []
+    for t, d in enumerate(dims):
+        hot = np.flatnonzero(filters[t] < 2)
+        pick_hot = rng.random(rows) < 0.8
+        fks.append(np.where(pick_hot, rng.choice(hot, rows), rng.integers(0, d, rows)).astype(np.int32))
+    m1 = rng.integers(1000, 1_000_001, rows, dtype=np.int32)
+    qs = []
+    for q in range(64):
+        k = int(rng.integers(1, 5))
+        tables = [int(x) for x in rng.permutation(4)[:k]]
+        gs = [int(rng.integers(1, 40)) for _ in tables]
+        steps = [(t, 0, 2, g) for t, g in zip(tables, gs)]
+        factor = [int(np.prod(gs[i + 1:], dtype=np.int64)) for i in range(k)]
+        qs.append((steps, factor, int(np.prod(gs, dtype=np.int64))))
+    dev = lambda a: torch.from_numpy(a).cuda()
+    star = oc.Context.star([dev(f) for f in filters], [dev(a) for a in attrs], [dev(f) for f in fks], dev(m1),
+                           None, rows=rows)
+    b = ctx.batch(star, [oc.BatchQuery.make(*q) for q in qs])
+    expect = []
+    for steps, factor, G in qs:
+        dvs = [None] * 4
+        for (t, lo, hi, g) in steps:
+            dvs[t] = po.build_dimvec(filters[t], attrs[t], lo, hi, 1 if g > 0 else 0, 4, group_mod=g)
+        p = po.plan([st[0] for st in steps], factor, G)
+        es, ec, bad = po.run("rowwise", p, po.Star(dvs, fks, m1), nthreads=ncpu)
+        assert bad == 0
+        expect.append((es, ec))
+    _say(f"  {sum(int(ec.sum()) for _, ec in expect) / rows:.1f} matches per fact row "
+         f"(expected from the pass fractions: {b.expected_events():.4f})")
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    ran = 0
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        try:
+            b.set_config(k)
+        except oc.OlapError as e:
+            assert e.status == oc.E_CAPACITY, e
+            continue
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        s, c = osum.cpu().numpy(), ocnt.cpu().numpy()
+        for i, (es, ec) in enumerate(expect):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            np.testing.assert_array_equal(c[lo:hi], ec, err_msg=f"config {k} query {i} counts")
+            np.testing.assert_array_equal(s[lo:hi], es, err_msg=f"config {k} query {i} sums")
+        ran += 1
+    assert ran > 0
+    b.close()
