@@ -29,10 +29,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kNoLds = 0xffffffffu;
 
-// one query as the batched kernels read it (112 bytes). The group-by terms
+// one query as the batched kernels read it (128 bytes). The group-by terms
 // are kept by table (the kernels' numbering): gid = sum over the four tables
-// of (attr mod g[t]) * fac[t], a table the query does not group by having
-// g 0, multiplier 0 and factor 0 (it adds 0 whatever its attribute)
+// of (attr mod g[t]) * fac[t], a table the query does not group by, or
+// groups by modulo 1, having factor 0 (it adds 0 whatever its attribute)
 struct BQDesc {
     uint32_t nsteps;    // probe steps (the kernels compute all four table terms)
     uint32_t lds_tab;   // word offset of its LDS group table, kNoLds when in HBM
@@ -40,14 +40,16 @@ struct BQDesc {
     uint32_t out_off;   // first slot in the concatenated outputs
     uint32_t fac[4];    // words 4..7
     uint32_t g[4];      // words 8..11: group_mod (0 or 1: code 0)
-    uint32_t pad0[4];
+    uint32_t m32[4];    // words 12..15: 32-bit modulo multiplier (attributes below 2^31)
     uint64_t M[4];      // words 16..23: fast-modulo multiplier UINT64_MAX / g + 1 (0 for g 0)
     uint32_t pbase;     // word 24: first staged-aggregation partition of the query
-    uint32_t pbase_c;   // the same with the compact one-pass partitions (2^12 slots)
+    uint32_t pbase1;    // the same for the one-pass records of the batch's format
     uint32_t seg_base;  // segmented kernel: first position of the query's event segment
     uint32_t seg_cap;   // and its capacity (a multiple of 64, at least 64)
+    uint32_t sh4;       // word 28: the shifts of the 32-bit modulo, 8 bits a table
+    uint32_t pad[3];
 };
-static_assert(sizeof(BQDesc) == 112, "descriptor layout");
+static_assert(sizeof(BQDesc) == 128, "descriptor layout");
 constexpr uint32_t kDescWords = sizeof(BQDesc) / 4;
 
 struct BatchArgs {
@@ -86,6 +88,9 @@ struct BatchArgs {
     ChunkArgs chunk;
     uint32_t pc_lds;
     uint64_t qvalid;        // bit q: q < nq
+    // every attribute below 2^31 and every G below 2^24: the group-by terms
+    // take the 32-bit multiply-high modulo (fast_code)
+    uint32_t fastmod;
 };
 
 struct MaskPred {
@@ -121,9 +126,11 @@ __device__ __forceinline__ void b_atomic_lds(uint32_t *tab, uint32_t G, uint32_t
 
 // ---------------------------------------------------------------- masks
 
+// (neg: set when an attribute is negative, i.e. 2^31 or more as the kernels
+// read it)
 __global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32_t D,
                               const MaskPred *preds, int np, unsigned long long base,
-                              unsigned long long *mask, uint4 *rec, uint32_t *pass)
+                              unsigned long long *mask, uint4 *rec, uint32_t *pass, uint32_t *neg)
 {
     for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < D; r0 += gridDim.x * blockDim.x) {
         const uint32_t r = r0 + threadIdx.x;
@@ -139,8 +146,10 @@ __global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32
             if ((threadIdx.x & 63) == 0 && b) atomicAdd(&pass[i], (uint32_t)__popcll(b));
         }
         if (in) {
+            const int32_t at = attr ? attr[r] : 0;
             mask[r] = m;
-            rec[r] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), attr ? (uint32_t)attr[r] : 0u, 0u);
+            rec[r] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)at, 0u);
+            if (at < 0) atomicOr(neg, 1u);
         }
     }
 }
@@ -397,6 +406,19 @@ __device__ __forceinline__ unsigned long long rotl64(unsigned long long w, uint3
 
 typedef int v2i_a4 __attribute__((ext_vector_type(2), aligned(4)));  // a pair at any 4-byte offset
 
+// (attr mod g) * fac for attr < 2^31, g < 2^24 and fac < 2^24 (m and s from
+// fastmod_magic; g 0 or 1 has m 0 and fac 0): the quotient is
+// mulhi(attr, m) >> s. For g not a power of two, s = floor(log2 g) and
+// m = ceil(2^(32+s) / g) = (2^(32+s) + e) / g with 0 < e < g; for
+// attr = q g + r, attr m / 2^(32+s) = q + (r + attr e / 2^(32+s)) / g, and
+// attr e / 2^(32+s) < attr g / 2^(32+s) < attr / 2^31 < 1, so the floor is
+// q. For g = 2^s, m = 2^31 and the shift is s - 1.
+__device__ __forceinline__ uint32_t fast_term(uint32_t attr, uint32_t m, uint32_t s, uint32_t g, uint32_t fac)
+{
+    const uint32_t q = __umulhi(attr, m) >> s;
+    return __umul24(attr - q * g, fac);
+}
+
 // One event of a wave-uniform query q (descriptor d): the gid from the
 // row's staged attributes and the query's group-by terms, then the
 // aggregation the descriptor names -- its LDS table, global atomics, or a
@@ -422,9 +444,14 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
     const uint32_t G = d.G;
     const uint32_t lds_tab = d.lds_tab;
     uint32_t gid = 0u;
+    if (a.fastmod) {
 #pragma unroll
-    for (int t = 0; t < 4; t++)
-        gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
+        for (int t = 0; t < 4; t++) gid += fast_term(av[t], d.m32[t], (d.sh4 >> (8 * t)) & 31u, d.g[t], d.fac[t]);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
+    }
     const bool ok = valid && gid < G;
     if (valid && gid >= G)
         __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -439,15 +466,16 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
     } else {
         const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
         const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
-        const uint32_t pbase = compact ? d.pbase_c : d.pbase;
+        const uint32_t pbase = MODE == AGG_CHUNK ? d.pbase1 : d.pbase;
         const uint32_t part = pbase + (gid >> pbits);
         const uint32_t glow = gid & ((1u << pbits) - 1u);
-        const long long moff = mv - (long long)a.chunk.mbase;
+        // (4-byte records hold one measure: m1 - mbase in 32 bits)
+        const uint32_t moff = m1v - (uint32_t)a.chunk.mbase;
         // a measure outside the range 4-byte records were packed for (the
         // columns changed since the range was taken) is added directly and
         // takes no record slot
         bool rec = ok;
-        if (compact && ok && (moff < 0 || moff >= (1ll << kCompactMeasureBits))) {
+        if (compact && ok && ((int32_t)m1v < a.chunk.mbase || moff >= (1u << kCompactMeasureBits))) {
             const unsigned long long o = a.chunk.part_out[part] + glow;
             b_atomic_dev(&a.gsum[o], (unsigned long long)mv);
             b_atomic_dev(&a.gcnt[o], 1ull);
@@ -496,7 +524,7 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
                 b_atomic_dev(&a.gcnt[o], 1ull);
                 __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (compact) {
-                ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | (uint32_t)moff;
+                ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | moff;
             } else if (a.m2) {
                 ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, m1v, m2v, 0u);
             } else {
@@ -1164,6 +1192,8 @@ const int kNumBatchCfgs = (int)(sizeof(kBatchCfgs) / sizeof(kBatchCfgs[0]));
 // segmented kernel: at most kSegEvents events in a tile's list (16-bit
 // positions that leave the segment word's cursor room to count past its end)
 constexpr uint32_t kSegEvents = 32768;
+// the mask kernels' pass counts (one per query and table) and a flag
+constexpr uint32_t kPassWords = 4u * OLAP_BATCH_MAX + 1u;
 
 template <int MODE>
 static const void *batch_kernel(const BatchCfg &c)
@@ -1241,7 +1271,7 @@ struct BatchWorkspace {
     size_t dim_cap[4] = {0, 0, 0, 0};
     BQDesc *desc[kNumBatchCfgs] = {};   // OLAP_BATCH_MAX descriptors per configuration
     MaskPred *preds = nullptr;          // up to 4 x OLAP_BATCH_MAX predicates (one per query and table)
-    uint32_t *pass = nullptr;           // their pass counts
+    uint32_t *pass = nullptr;           // their pass counts, then the negative-attribute flag
     int32_t *minmax = nullptr;          // [2] device scratch of measure_range
     int8_t *progvec = nullptr;          // pass vectors of the steps' general predicates
     size_t progvec_cap = 0;
@@ -1279,7 +1309,7 @@ struct BatchWorkspace {
         for (int c = 0; c < kNumBatchCfgs && e == hipSuccess; c++)
             if (!desc[c]) e = hipMalloc(&desc[c], sizeof(BQDesc) * OLAP_BATCH_MAX);
         if (e == hipSuccess && !preds) e = hipMalloc(&preds, sizeof(MaskPred) * 4 * OLAP_BATCH_MAX);
-        if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * 4 * OLAP_BATCH_MAX);
+        if (e == hipSuccess && !pass) e = hipMalloc(&pass, 4 * kPassWords);
         if (e == hipSuccess && !minmax) e = hipMalloc(&minmax, 8);
         return e;
     }
@@ -1332,6 +1362,7 @@ struct olap_batch_s {
     uint32_t nparts = 0;
     uint32_t nparts_c = 0;   // partitions of 2^12 slots (compact one-pass records)
     bool compact = false;    // one measure whose range spans fewer than 2^20 values
+    bool fastmod = false;    // every probed table's attributes in [0, 2^31), every G below 2^24
     int32_t mbase = 0;       // its minimum
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1352,6 +1383,22 @@ olap_status bhip(olap_ctx ctx, hipError_t e, const char *what)
 }
 
 uint32_t up16(uint32_t bytes) { return (bytes + 15u) & ~15u; }
+
+// the multiplier and shift of fast_term for modulo g (0 and 0 for g 0 or 1)
+void fastmod_magic(uint32_t g, uint32_t *m, uint32_t *sh)
+{
+    *m = 0u;
+    *sh = 0u;
+    if (g < 2u) return;
+    const uint32_t s = 31u - (uint32_t)__builtin_clz(g);
+    if ((g & (g - 1u)) == 0u) {
+        *m = 1u << 31;
+        *sh = s - 1u;
+    } else {
+        *m = (uint32_t)(((1ull << (32u + s)) + g - 1u) / g);
+        *sh = s;
+    }
+}
 
 // Segments of the segmented kernel's event list: query q expects
 // E = R * sel[q] matches a tile; its segment holds E + 4 sqrt(E) rounded up
@@ -1537,14 +1584,18 @@ olap_status plan_config(olap_batch b, int c)
         d.G = (uint32_t)Q.group_nums;
         d.out_off = (uint32_t)b->offsets[q];
         d.pbase = pbase;
+        d.pbase1 = b->compact ? pbase_c : pbase;
         pbase += (d.G + kPartSlots - 1) / kPartSlots;
-        d.pbase_c = pbase_c;
         pbase_c += (d.G + (1u << kCompactPartBits) - 1) >> kCompactPartBits;
         for (int s = 0; s < Q.ndims; s++) {
             const int t = pm[Q.table[s]];
-            d.fac[t] = (uint32_t)Q.factor[s];
             d.g[t] = (uint32_t)Q.group_mod[s];
+            d.fac[t] = d.g[t] > 1u ? (uint32_t)Q.factor[s] : 0u;  // modulo 0 or 1: code 0
             d.M[t] = d.g[t] ? (UINT64_MAX / d.g[t] + 1ull) : 0ull;
+            uint32_t m = 0u, sh = 0u;
+            fastmod_magic(d.g[t], &m, &sh);
+            d.m32[t] = m;
+            d.sh4 |= sh << (8 * t);
         }
     }
     hipError_t e = hipMemcpyAsync(b->ws->desc[c], desc.data(), sizeof(BQDesc) * b->nq, hipMemcpyHostToDevice,
@@ -1569,6 +1620,7 @@ olap_status plan_config(olap_batch b, int c)
     a.rows = (uint64_t)b->star.rows;
     a.nq = (uint32_t)b->nq;
     a.qvalid = b->nq >= 64 ? ~0ull : (1ull << b->nq) - 1ull;
+    a.fastmod = b->fastmod ? 1u : 0u;
     a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);
@@ -1761,18 +1813,18 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
     pbeg[4] = (uint32_t)preds.size();
     if (e == hipSuccess)
         e = hipMemcpyAsync(ws->preds, preds.data(), sizeof(MaskPred) * preds.size(), hipMemcpyHostToDevice, strm);
-    if (e == hipSuccess) e = launch_zero_u32(ws->pass, preds.size(), strm);
+    if (e == hipSuccess) e = launch_zero_u32(ws->pass, kPassWords, strm);
     for (int t = 0; t < 4 && e == hipSuccess; t++) {
         const uint32_t D = (uint32_t)star->dim_rows[t];
         if (!((b->used >> t) & 1u) || !D) continue;
         const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
         k_batch_masks<<<grid, 256, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds + pbeg[t],
                                               (int)(pbeg[t + 1] - pbeg[t]), b->nouse[t], ws->mask[t], ws->rec[t],
-                                              ws->pass + pbeg[t]);
+                                              ws->pass + pbeg[t], ws->pass + kPassWords - 1);
         e = hipGetLastError();
     }
-    std::vector<uint32_t> pass(preds.size(), 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(pass.data(), ws->pass, 4 * preds.size(), hipMemcpyDeviceToHost, strm);
+    std::vector<uint32_t> pass(kPassWords, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(pass.data(), ws->pass, 4 * kPassWords, hipMemcpyDeviceToHost, strm);
     if (e == hipSuccess) e = hipStreamSynchronize(strm);
     for (int t = 0; t < 4 && e == hipSuccess; t++) {
         const uint32_t D = (uint32_t)star->dim_rows[t];
@@ -1799,6 +1851,10 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
         b->compact = (int64_t)ws->range[1] - (int64_t)ws->range[0] < (1ll << kCompactMeasureBits);
         b->mbase = ws->range[0];
     }
+    // the 32-bit modulo of the group-by terms: every attribute of the probed
+    // tables non-negative and every G below 2^24
+    b->fastmod = e == hipSuccess && pass[kPassWords - 1] == 0u;
+    for (int q = 0; q < nq; q++) b->fastmod = b->fastmod && queries[q].group_nums < (1 << 24);
     b->events = 0;
     for (double v : b->sel) b->events += v;
     for (int q = 0; q < nq; q++) {

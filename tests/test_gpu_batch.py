@@ -405,3 +405,67 @@ def test_batch_segments_overflow_into_residual_passes(ctx, ncpu):
         ran += 1
     assert ran > 0
     b.close()
+
+
+@pytest.mark.parametrize("attrs", ["below_2_31", "negative"])
+def test_batch_group_terms_wide_attributes_and_moduli(ctx, ncpu, attrs):
+    """Group-by terms at the edges of the modulo arithmetic: attributes over
+    the whole non-negative int32 range (the 32-bit multiply-high modulo) or
+    with negative values (read as 2^31 and above: the 64-bit one), and
+    moduli at and around powers of two up to 2^24 - 1. Every configuration
+    equals the oracle."""
+    import torch
+
+    rows = 300_007
+    rng = np.random.default_rng(17)
+    dims = [2556, 30000, 200000, 2000]
+    filters = [rng.integers(0, 100, d, dtype=np.int32) for d in dims]
+    lo_attr = -2 ** 31 if attrs == "negative" else 0
+    attr_cols = [rng.integers(lo_attr, 2 ** 31, d, dtype=np.int64).astype(np.int32) for d in dims]
+    for a in attr_cols:   # the extremes of the range
+        a[:4] = [0, 1, 2 ** 31 - 1, 2 ** 31 - 2] if attrs != "negative" else [-1, -2 ** 31, 2 ** 31 - 1, 0]
+    fks = [rng.integers(0, d, rows, dtype=np.int32) for d in dims]
+    m1 = rng.integers(1000, 1_000_001, rows, dtype=np.int32)
+    big = [2, 3, 4, 7, 8, 255, 256, 257, 65535, 65536, 65537, 99991, (1 << 20) + 1, (1 << 23) - 1,
+           (1 << 24) - 1]
+    qs = []
+    for i, g in enumerate(big):   # one table grouped by a large modulo
+        t = i % 4
+        qs.append(([(t, 0, 60, g)], [1], g))
+    for i in range(14):           # two tables
+        t1, t2 = [int(x) for x in rng.permutation(4)[:2]]
+        g1 = int(rng.choice([2, 3, 5, 16, 17, 255, 256, 257, 4095, 4096]))
+        g2 = int(rng.integers(1, 4097))
+        qs.append(([(t1, 10, 70, g1), (t2, 0, 50, g2)], [g2, 1], g1 * g2))
+    dev = lambda a: torch.from_numpy(a).cuda()
+    star = oc.Context.star([dev(f) for f in filters], [dev(a) for a in attr_cols], [dev(f) for f in fks], dev(m1),
+                           None, rows=rows)
+    b = ctx.batch(star, [oc.BatchQuery.make(*q) for q in qs])
+    expect = []
+    for steps, factor, G in qs:
+        dvs = [None] * 4
+        for (t, lo, hi, g) in steps:
+            dvs[t] = po.build_dimvec(filters[t], attr_cols[t], lo, hi, 1 if g > 0 else 0, 4, group_mod=g)
+        p = po.plan([st[0] for st in steps], factor, G)
+        es, ec, bad = po.run("rowwise", p, po.Star(dvs, fks, m1), nthreads=ncpu)
+        assert bad == 0
+        expect.append((es, ec))
+    osum = torch.empty(b.total_slots, dtype=torch.int64, device="cuda")
+    ocnt = torch.empty_like(osum)
+    ran = 0
+    for k in range(ctx.lib.olap_batch_num_configs()):
+        try:
+            b.set_config(k)
+        except oc.OlapError as e:
+            _say(f"  config {k} skipped: {e}")
+            continue
+        b.run(osum, ocnt)
+        torch.cuda.synchronize()
+        s, c = osum.cpu().numpy(), ocnt.cpu().numpy()
+        for i, (es, ec) in enumerate(expect):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            np.testing.assert_array_equal(c[lo:hi], ec, err_msg=f"config {k} query {i} counts")
+            np.testing.assert_array_equal(s[lo:hi], es, err_msg=f"config {k} query {i} sums")
+        ran += 1
+    assert ran > 0
+    b.close()
