@@ -419,56 +419,34 @@ __device__ __forceinline__ uint32_t fast_term(uint32_t attr, uint32_t m, uint32_
     return __umul24(attr - q * g, fac);
 }
 
-// One event of a wave-uniform query q (descriptor d): the gid from the
-// row's staged attributes and the query's group-by terms, then the
-// aggregation the descriptor names -- its LDS table, global atomics, or a
-// staged record (the COUNT/SCATTER passes or a one-pass chunk). `valid`
-// false is a padding lane: it takes part in the wave's ballots only.
+// The aggregation of one event whose gid is known (wave-uniform G and
+// lds_tab; pbase the query's first partition for MODE): its LDS table,
+// global atomics, or a staged record (the COUNT/SCATTER passes or a one-pass
+// chunk). `valid` false is a padding lane: it takes part in the wave's
+// ballots only.
 template <int MODE>
-__device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d, bool valid, uint32_t row,
-                                            uint32_t *lds, const l_u32 *rattr, const l_u32 *rm, uint32_t R,
-                                            stage_l_u32 *pc_cur, stage_l_u32 *pc_spare, stage_l_u32 *pc_fill,
-                                            stage_l_u32 *scur, const stage_l_u64 *sstart, uint32_t CH,
-                                            uint32_t lane)
+__device__ __forceinline__ void event_aggregate(const BatchArgs &a, bool valid, uint32_t gid, uint32_t G,
+                                                uint32_t lds_tab, uint32_t pbase, uint32_t out_off, uint32_t m1v,
+                                                uint32_t m2v, uint32_t *lds, stage_l_u32 *pc_cur,
+                                                stage_l_u32 *pc_spare, stage_l_u32 *pc_fill, stage_l_u32 *scur,
+                                                const stage_l_u64 *sstart, uint32_t CH, uint32_t lane)
 {
-    // the row's attributes of the four tables and its measures (at fixed
-    // LDS offsets, issued with the descriptor's scalar loads), then the
-    // group-by terms of every table (a table the query does not group by
-    // adds 0, whatever the staged word holds)
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    typedef __attribute__((address_space(3))) const v4u l_cu4;
-    const v4u a4 = ((l_cu4 *)rattr)[row];
-    const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
-    const uint32_t m1v = rm[row];
-    const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
-    const uint32_t G = d.G;
-    const uint32_t lds_tab = d.lds_tab;
-    uint32_t gid = 0u;
-    if (a.fastmod) {
-#pragma unroll
-        for (int t = 0; t < 4; t++) gid += fast_term(av[t], d.m32[t], (d.sh4 >> (8 * t)) & 31u, d.g[t], d.fac[t]);
-    } else {
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-            gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
-    }
     const bool ok = valid && gid < G;
     if (valid && gid >= G)
         __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const long long mv = (long long)(int32_t)m1v + (long long)(int32_t)m2v;
+    const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
+    const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
+    const uint32_t part = pbase + (gid >> pbits);
+    const uint32_t glow = gid & ((1u << pbits) - 1u);
     if (lds_tab != kNoLds) {
         if (MODE != AGG_SCATTER && ok) b_atomic_lds(lds + lds_tab, G, gid, (unsigned long long)mv);
     } else if (MODE == AGG_DIRECT) {
         if (ok) {
-            b_atomic_dev(&a.gsum[d.out_off + gid], (unsigned long long)mv);
-            b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
+            b_atomic_dev(&a.gsum[out_off + gid], (unsigned long long)mv);
+            b_atomic_dev(&a.gcnt[out_off + gid], 1ull);
         }
     } else {
-        const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
-        const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
-        const uint32_t pbase = MODE == AGG_CHUNK ? d.pbase1 : d.pbase;
-        const uint32_t part = pbase + (gid >> pbits);
-        const uint32_t glow = gid & ((1u << pbits) - 1u);
         // (4-byte records hold one measure: m1 - mbase in 32 bits)
         const uint32_t moff = m1v - (uint32_t)a.chunk.mbase;
         // a measure outside the range 4-byte records were packed for (the
@@ -546,6 +524,44 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
             }
         }
     }
+}
+
+// One event of a wave-uniform query q (descriptor d): the gid from the
+// row's staged attributes and the query's group-by terms, then its
+// aggregation. FAST: 1 the 32-bit modulo, 0 the 64-bit one, -1 as a.fastmod
+// says. The descriptor words every path needs are read before any branch,
+// so that their scalar loads are issued, and waited for, together.
+template <int MODE, int FAST = -1>
+__device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d, bool valid, uint32_t row,
+                                            uint32_t *lds, const l_u32 *rattr, const l_u32 *rm, uint32_t R,
+                                            stage_l_u32 *pc_cur, stage_l_u32 *pc_spare, stage_l_u32 *pc_fill,
+                                            stage_l_u32 *scur, const stage_l_u64 *sstart, uint32_t CH,
+                                            uint32_t lane)
+{
+    // the row's attributes of the four tables and its measures (at fixed
+    // LDS offsets, issued with the descriptor's scalar loads), then the
+    // group-by terms of every table (a table the query does not group by
+    // adds 0, whatever the staged word holds)
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) const v4u l_cu4;
+    const v4u a4 = ((l_cu4 *)rattr)[row];
+    const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
+    const uint32_t m1v = rm[row];
+    const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
+    const uint32_t G = d.G;
+    const uint32_t lds_tab = d.lds_tab;
+    const uint32_t pbase = MODE == AGG_CHUNK ? d.pbase1 : d.pbase;
+    uint32_t gid = 0u;
+    if (FAST < 0 ? a.fastmod != 0u : FAST == 1) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) gid += fast_term(av[t], d.m32[t], (d.sh4 >> (8 * t)) & 31u, d.g[t], d.fac[t]);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
+    }
+    event_aggregate<MODE>(a, valid, gid, G, lds_tab, pbase, d.out_off, m1v, m2v, lds, pc_cur, pc_spare, pc_fill,
+                          scur, sstart, CH, lane);
 }
 
 template <int BS, int MODE>
@@ -902,7 +918,7 @@ __device__ __forceinline__ void batch_event_direct(const BatchArgs &a, uint32_t 
     b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
 }
 
-template <int BS>
+template <int BS, int FAST>
 __global__ __launch_bounds__(BS) void k_batch_seg(BatchArgs a)
 {
     constexpr int MODE = AGG_CHUNK;
@@ -1133,8 +1149,8 @@ __global__ __launch_bounds__(BS) void k_batch_seg(BatchArgs a)
             const bool valid = lane < n;
             const uint32_t row = valid ? e : 0u;
             const c_desc &d = ((const c_desc *)a.desc)[q];
-            batch_event<MODE>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, nullptr, nullptr,
-                              CH, lane);
+            batch_event<MODE, FAST>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, nullptr,
+                                    nullptr, CH, lane);
         }
         gather_second();
         lds_barrier();
@@ -1196,11 +1212,12 @@ constexpr uint32_t kSegEvents = 32768;
 constexpr uint32_t kPassWords = 4u * OLAP_BATCH_MAX + 1u;
 
 template <int MODE>
-static const void *batch_kernel(const BatchCfg &c)
+static const void *batch_kernel(const BatchCfg &c, bool fast)
 {
     if (c.seg) {
         if (MODE != AGG_CHUNK) return nullptr;
-        return c.bs == 512 ? (const void *)k_batch_seg<512> : (const void *)k_batch_seg<1024>;
+        if (fast) return c.bs == 512 ? (const void *)k_batch_seg<512, 1> : (const void *)k_batch_seg<1024, 1>;
+        return c.bs == 512 ? (const void *)k_batch_seg<512, 0> : (const void *)k_batch_seg<1024, 0>;
     }
     if (c.grouped) return c.bs == 512 ? (const void *)k_batch_ev<512, MODE> : (const void *)k_batch_ev<1024, MODE>;
     if (MODE == AGG_CHUNK) return nullptr;  // one-pass staging exists in the grouped kernel only
@@ -1209,12 +1226,13 @@ static const void *batch_kernel(const BatchCfg &c)
     return (const void *)k_batch<1024, 1, MODE>;
 }
 
-static const void *batch_kernel_mode(int mode, const BatchCfg &c)
+// (fast: the batch takes the 32-bit modulo, which some kernels are built for)
+static const void *batch_kernel_mode(int mode, const BatchCfg &c, bool fast)
 {
-    if (mode == AGG_CHUNK) return batch_kernel<AGG_CHUNK>(c);
-    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c);
-    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c);
-    return batch_kernel<AGG_DIRECT>(c);
+    if (mode == AGG_CHUNK) return batch_kernel<AGG_CHUNK>(c, fast);
+    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c, fast);
+    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c, fast);
+    return batch_kernel<AGG_DIRECT>(c, fast);
 }
 
 // fact rows per tile of a configuration
@@ -1225,14 +1243,14 @@ static uint64_t batch_tile_rows(const BatchCfg &c)
 
 // grid of a configuration: persistent, limited by the least occupant of the
 // kernels it runs (COUNT and SCATTER of a staged run share one grid)
-static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, int *grid)
+static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, bool fast, int *grid)
 {
     const BatchCfg c = kBatchCfgs[cfg];
     const int modes[2] = {c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_COUNT : AGG_DIRECT,
                           c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_SCATTER : AGG_DIRECT};
     int best = 1 << 30;
     for (int m : modes) {
-        const void *k = batch_kernel_mode(m, c);
+        const void *k = batch_kernel_mode(m, c, fast);
         if (dyn > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
             if (e != hipSuccess) return e;
@@ -1257,7 +1275,7 @@ static hipError_t batch_launch(int cfg, int mode, BatchArgs &a, uint32_t dyn, in
     a.ntiles = (a.rows + tile - 1) / tile;
     if (a.ntiles == 0) return hipSuccess;
     void *args[] = {&a};
-    return hipLaunchKernel(batch_kernel_mode(mode, c), dim3(grid), dim3(c.bs), args, dyn, s);
+    return hipLaunchKernel(batch_kernel_mode(mode, c, a.fastmod != 0u), dim3(grid), dim3(c.bs), args, dyn, s);
 }
 
 }  // namespace olap
@@ -1624,7 +1642,7 @@ olap_status plan_config(olap_batch b, int c)
     a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);
-    e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, &b->grid[c]);
+    e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, a.fastmod != 0u, &b->grid[c]);
     if (e != hipSuccess) return bhip(ctx, e, "batch launch plan");
     b->planned[c] = true;
     return OLAP_OK;
