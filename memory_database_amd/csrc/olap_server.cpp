@@ -56,7 +56,8 @@ struct Request {
 constexpr double kStagedMatches = 2.0e6;
 constexpr double kStagedPerRow = 0.015;
 constexpr int kCfgAtomics = 0;   // 512 threads, 48 KiB of LDS tables
-constexpr int kCfgOnePass = 30;  // 1024 threads, small dimension tables in LDS, one-pass staging
+constexpr int kCfgOnePass = 33;  // 1024 threads, small dimension tables in LDS, one-pass staging, event segments
+constexpr int kCfgOnePassEv = 30;  // the same with counted event lists (when segments do not fit)
 constexpr int kCfgTwoPass = 9;   // 1024 threads, 48 KiB of LDS tables, two-pass staging
 
 }  // namespace
@@ -296,7 +297,8 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
         const double matches = olap_batch_expected_events(b) * (double)star.rows;
         cfg = matches > std::max(kStagedMatches, kStagedPerRow * (double)star.rows) ? kCfgOnePass : kCfgAtomics;
         st = olap_batch_set_config(b, cfg);
-        if (st == OLAP_E_CAPACITY && cfg == kCfgOnePass) st = olap_batch_set_config(b, cfg = kCfgTwoPass);
+        if (st == OLAP_E_CAPACITY && cfg == kCfgOnePass) st = olap_batch_set_config(b, cfg = kCfgOnePassEv);
+        if (st == OLAP_E_CAPACITY && cfg == kCfgOnePassEv) st = olap_batch_set_config(b, cfg = kCfgTwoPass);
     }
     const size_t total = (size_t)off.back();
     if (st == OLAP_OK && !w.out.fit(std::max<size_t>(total, 1))) st = OLAP_E_NOMEM;
