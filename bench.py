@@ -345,7 +345,7 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
     log(f"batched: {nq} queries {batch_ms:.2f} ms (config {cfg}: {threads} threads, {rpt} rows/thread, "
         f"{kib} KiB LDS tables, staged={staged}); plain {plain_ms:.2f} ms; x{plain_ms / batch_ms:.2f}; "
         f"all {nq} queries equal their plain scans")
-    traffic, src, current = profiled_traffic("k_batch_ev", f"{nq}q_{ds.cfg.name}",
+    traffic, src, current = profiled_traffic(oc.config_kernel(cfg), f"{nq}q_{ds.cfg.name}",
                                              key="batch HBM traffic")
     return {
         "verified": f"every slot of the {nq} queries equals its plain row-wise scan",
@@ -354,8 +354,8 @@ def bench_batched(ctx, ds, stream, nq=64, seed=7):
         "qps_batched": nq / (batch_ms / 1e3), "qps_plain": nq / (plain_ms / 1e3),
         "speedup": plain_ms / batch_ms,
         "matches_per_row": events,
-        "config": {"id": cfg, "threads": threads, "rows_per_thread": rpt, "lds_table_kib": kib,
-                   "aggregation": ["atomics", "two-pass staged", "one-pass staged"][staged]},
+        "config": {"id": cfg, "kernel": oc.config_kernel(cfg), "threads": threads, "rows_per_thread": rpt,
+                   "lds_table_kib": kib, "aggregation": ["atomics", "two-pass staged", "one-pass staged"][staged]},
         "union_bytes": union_bytes, "union_GBps": union_bytes / (batch_ms / 1e3) / 1e9,
         "effective_GBps": per_query_bytes / (batch_ms / 1e3) / 1e9,
         "timing": "hipGraph replay, HIP events, mean of 5 after 1 warmup; plain = sum of the "

@@ -374,6 +374,10 @@ olap_status olap_batch_run(olap_batch b, int64_t *sum, int64_t *count, void *str
 int32_t olap_batch_num_configs(void);
 olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
                                    int32_t *lds_table_kib, int32_t *staged);
+/* The scan kernel a configuration runs: 0 per lane (every lane walks its
+ * rows' matching queries), 1 event-grouped with counted event lists, 2
+ * event-grouped with fixed per-query event segments; -1 for an invalid cfg. */
+int32_t olap_batch_config_form(int32_t cfg);
 olap_status olap_batch_set_config(olap_batch b, int32_t cfg);
 olap_status olap_batch_autotune(olap_batch b, int64_t *sum, int64_t *count, void *stream,
                                 int32_t *cfg_out, double *ms_out);

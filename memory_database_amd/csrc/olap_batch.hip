@@ -1921,6 +1921,12 @@ double olap_batch_expected_events(olap_batch b) { return b ? b->events : -1.0; }
 
 int32_t olap_batch_num_configs(void) { return kNumBatchCfgs; }
 
+int32_t olap_batch_config_form(int32_t cfg)
+{
+    if (cfg < 0 || cfg >= kNumBatchCfgs) return -1;
+    return kBatchCfgs[cfg].seg ? 2 : kBatchCfgs[cfg].grouped ? 1 : 0;
+}
+
 olap_status olap_batch_config_info(int32_t cfg, int32_t *threads, int32_t *rows_per_thread,
                                    int32_t *lds_table_kib, int32_t *staged)
 {

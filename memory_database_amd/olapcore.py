@@ -252,6 +252,7 @@ SIGNATURES = {
     "olap_batch_offsets": (C.c_int, [_P, _P]),
     "olap_batch_run": (C.c_int, [_P, _P, _P, _P]),
     "olap_batch_num_configs": (C.c_int32, []),
+    "olap_batch_config_form": (C.c_int32, [C.c_int32]),
     "olap_batch_config_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "olap_batch_set_config": (C.c_int, [_P, C.c_int32]),
@@ -369,6 +370,15 @@ def config_info(cfg: int):
     th, rp, kib, st = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     load().olap_batch_config_info(cfg, C.byref(th), C.byref(rp), C.byref(kib), C.byref(st))
     return th.value, rp.value, kib.value, st.value
+
+
+def config_kernel(cfg: int) -> str:
+    """Name of the scan kernel a batch config runs (k_batch, k_batch_ev or
+    k_batch_seg)."""
+    form = int(load().olap_batch_config_form(cfg))
+    if form < 0:
+        raise ValueError(f"no batch config {cfg}")
+    return ("k_batch", "k_batch_ev", "k_batch_seg")[form]
 
 
 def dict_encode(strings):
