@@ -1293,9 +1293,15 @@ struct BatchWorkspace {
     int32_t *minmax = nullptr;          // [2] device scratch of measure_range
     int8_t *progvec = nullptr;          // pass vectors of the steps' general predicates
     size_t progvec_cap = 0;
-    const int32_t *range_col = nullptr; // the measure column whose range is kept
-    int64_t range_rows = -1;
-    int32_t range[2] = {0, 0};          // its min and max
+    // the ranges of the last kRanges measure columns seen (a server worker
+    // alternates between datasets): column, rows, min and max
+    static constexpr int kRanges = 8;
+    struct Range {
+        const int32_t *col = nullptr;
+        int64_t rows = -1;
+        int32_t lo = 0, hi = 0;
+    } ranges[kRanges];
+    int next_range = 0;
 
     hipError_t fit_table(int t, size_t rows)
     {
@@ -1857,17 +1863,22 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
     // are 4 bytes (the range is kept per workspace and column; a measure
     // outside it later, the columns having changed, is added directly)
     if (!star->m2 && star->rows > 0) {
-        if (ws->range_col != star->m1 || ws->range_rows != star->rows) {
-            e = measure_range(star->m1, (uint64_t)star->rows, ws->minmax, ws->range, strm);
+        BatchWorkspace::Range *hit = nullptr;
+        for (BatchWorkspace::Range &r : ws->ranges)
+            if (r.col == star->m1 && r.rows == star->rows) hit = &r;
+        if (!hit) {
+            int32_t mm[2] = {0, 0};
+            e = measure_range(star->m1, (uint64_t)star->rows, ws->minmax, mm, strm);
             if (e != hipSuccess) {
                 delete b;
                 return bhip(ctx, e, "batch measure range");
             }
-            ws->range_col = star->m1;
-            ws->range_rows = star->rows;
+            hit = &ws->ranges[ws->next_range];
+            ws->next_range = (ws->next_range + 1) % BatchWorkspace::kRanges;
+            *hit = {star->m1, star->rows, mm[0], mm[1]};
         }
-        b->compact = (int64_t)ws->range[1] - (int64_t)ws->range[0] < (1ll << kCompactMeasureBits);
-        b->mbase = ws->range[0];
+        b->compact = (int64_t)hit->hi - (int64_t)hit->lo < (1ll << kCompactMeasureBits);
+        b->mbase = hit->lo;
     }
     // the 32-bit modulo of the group-by terms: every attribute of the probed
     // tables non-negative and every G below 2^24

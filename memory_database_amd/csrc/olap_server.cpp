@@ -204,7 +204,7 @@ struct SliceJob {
     std::vector<int64_t> off;
     int mirror = 0;
     int worker = 0, dataset = 0, cfg = -1;
-    double setup_ms = 0.0, scan_ms = 0.0;
+    double create_ms = 0.0, setup_ms = 0.0, scan_ms = 0.0;
     std::chrono::steady_clock::time_point t_launched;
 };
 
@@ -292,6 +292,7 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
     // the previous batch's kernels, which read the workspace this setup may
     // grow, have finished: the worker synchronised its stream before admission
     int st = olap::batch_create_in(w.ctx, &star, qs.data(), (int32_t)qs.size(), w.ws, w.stream, &b);
+    const Clock::time_point tc = Clock::now();
     int cfg = -1;
     if (st == OLAP_OK) {
         const double matches = olap_batch_expected_events(b) * (double)star.rows;
@@ -335,6 +336,7 @@ int run_gpu(olap_server s, Worker &w, const std::vector<Request *> &batch,
     job.worker = w.id;
     job.dataset = batch[0]->dataset;
     job.cfg = cfg;
+    job.create_ms = ms(t0, tc);
     job.setup_ms = ms(t0, t1);
     job.scan_ms = ms(t1, t2);
     job.t_launched = Clock::now();
@@ -370,9 +372,10 @@ void Slicer::run()
                 return std::chrono::duration<double, std::milli>(b - a).count();
             };
             std::fprintf(stderr,
-                         "olap_server: worker %d dataset %d queries %zu config %d slots %zu: setup %.2f ms, "
-                         "scan %.2f ms, results %.2f ms, slices %.2f ms, status %d\n",
-                         job.worker, job.dataset, job.batch.size(), job.cfg, total, job.setup_ms, job.scan_ms,
+                         "olap_server: worker %d dataset %d queries %zu config %d slots %zu: setup %.2f ms "
+                         "(create %.2f), scan %.2f ms, results %.2f ms, slices %.2f ms, status %d\n",
+                         job.worker, job.dataset, job.batch.size(), job.cfg, total, job.setup_ms, job.create_ms,
+                         job.scan_ms,
                          ms(job.t_launched, t3), ms(t3, Clock::now()), st);
         }
         finish(s_, job.batch, st);
