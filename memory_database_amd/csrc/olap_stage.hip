@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_chunk_scatter(const unsigned long long 
 
 // work items: up to cpi chunks of one partition, aggregated in an LDS table
 template <uint32_t FMT>
-__global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
+__device__ __forceinline__ void chunk_reduce(
     const uint32_t *part_chunks, const uint32_t *item_begin, uint32_t nparts, uint32_t cpi,
     const unsigned long long *order, const void *pool, uint32_t ch_bits, int32_t mbase,
     const unsigned long long *part_out, const uint32_t *part_slots, uint32_t *work,
@@ -423,6 +423,101 @@ __global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(
         }
         __syncthreads();
         reduce_flush(lsum, lcnt, slots, part_out[p], gsum, gcnt);
+        __syncthreads();
+    }
+}
+
+#define CHUNK_REDUCE_PARAMS                                                                              \
+    const uint32_t *part_chunks, const uint32_t *item_begin, uint32_t nparts, uint32_t cpi,             \
+        const unsigned long long *order, const void *pool, uint32_t ch_bits, int32_t mbase,             \
+        const unsigned long long *part_out, const uint32_t *part_slots, uint32_t *work,                 \
+        unsigned long long *gsum, unsigned long long *gcnt
+#define CHUNK_REDUCE_ARGS \
+    part_chunks, item_begin, nparts, cpi, order, pool, ch_bits, mbase, part_out, part_slots, work, gsum, gcnt
+
+template <uint32_t FMT>
+__global__ __launch_bounds__(kReduceThreads) void k_chunk_reduce(CHUNK_REDUCE_PARAMS)
+{
+    chunk_reduce<FMT>(CHUNK_REDUCE_ARGS);
+}
+
+// 4-byte records (slot << 20 | measure - mbase): one 64-bit LDS atomic a
+// record adds (measure offset << 22) | 1, so a slot's word holds its count
+// in the low 22 bits and its sum of offsets above them. A work item holds
+// at most kReduceChunk = 2^21 records, so neither field overflows (count
+// <= 2^21 < 2^22, sum < 2^21 * 2^20 = 2^41 < 2^42); the flush adds
+// count * mbase back. A table of 2^12 slots (32 KiB) and at most 64 VGPRs
+// let two 1024-thread workgroups share a CU.
+constexpr uint32_t kPackCountBits = 22;
+static_assert(kReduceChunk < (1u << kPackCountBits), "packed count field");
+static_assert(kCompactMeasureBits + 21 + kPackCountBits <= 64, "packed sum field");
+
+__global__ __launch_bounds__(kReduceThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_chunk_reduce_compact(
+    CHUNK_REDUCE_PARAMS)
+{
+    constexpr uint32_t kSlots = 1u << kCompactPartBits;
+    __shared__ unsigned long long lpack[kSlots];
+    __shared__ unsigned long long lord[kReduceItemChunks];  // the item's (fill << 32 | chunk id)
+    __shared__ uint32_t s_item, s_part;
+    typedef __attribute__((address_space(3))) unsigned long long l_u64;
+    const uint4 *v4 = (const uint4 *)pool;
+    const uint32_t ubits = ch_bits - 2u;  // 16-byte units of a chunk: four records
+    for (;;) {
+        reduce_claim(work, item_begin, nparts, s_item, s_part);
+        __syncthreads();
+        const uint32_t item = s_item;
+        const uint32_t p = s_part;
+        if (item >= item_begin[nparts]) break;
+        const uint32_t slots = part_slots[p];
+        const uint32_t cb = part_chunks[p] + (item - item_begin[p]) * cpi;
+        const uint32_t n = min(cb + cpi, part_chunks[p + 1]) - cb;  // at most cpi <= kReduceItemChunks
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lord[i] = order[cb + i];
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) lpack[s] = 0ull;
+        __syncthreads();
+        const uint32_t units = n << ubits;
+        for (uint32_t i0 = threadIdx.x; i0 < units; i0 += (uint32_t)kReduceThreads * kReduceLoads) {
+            uint4 v[kReduceLoads];
+            uint32_t r[kReduceLoads], fill[kReduceLoads];
+#pragma unroll
+            for (int u = 0; u < kReduceLoads; u++) {
+                const uint32_t i = i0 + (uint32_t)u * kReduceThreads;
+                v[u] = make_uint4(0u, 0u, 0u, 0u);
+                r[u] = 0u;
+                fill[u] = 0u;
+                if (i < units) {
+                    const unsigned long long o = lord[i >> ubits];
+                    const uint32_t w = i & ((1u << ubits) - 1u);
+                    r[u] = 4u * w;
+                    fill[u] = (uint32_t)(o >> 32);
+                    if (r[u] < fill[u]) v[u] = v4[(((unsigned long long)(uint32_t)o << ch_bits) >> 2) + w];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kReduceLoads; u++) {
+                const uint32_t x[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t g = x[k] >> kCompactMeasureBits;
+                    if (r[u] + (uint32_t)k < fill[u] && g < slots) {
+                        const unsigned long long add =
+                            ((unsigned long long)(x[k] & ((1u << kCompactMeasureBits) - 1u)) << kPackCountBits) | 1ull;
+                        __hip_atomic_fetch_add((l_u64 *)&lpack[g], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned long long out = part_out[p];
+        for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) {
+            const unsigned long long w = lpack[s];
+            const unsigned long long c = w & ((1ull << kPackCountBits) - 1ull);
+            if (c) {
+                const unsigned long long sum = (w >> kPackCountBits) + c * (unsigned long long)(long long)mbase;
+                if (gcnt)
+                    __hip_atomic_fetch_add((g_u64 *)&gcnt[out + s], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add((g_u64 *)&gsum[out + s], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         __syncthreads();
     }
 }
@@ -635,11 +730,13 @@ hipError_t Stager::chunk_reduce(unsigned long long *gsum, unsigned long long *gc
     if (e == hipSuccess) e = launch_zero_u32(work, 1, s);
     if (e != hipSuccess) return e;
     auto launch = [&](auto kernel) {
-        kernel<<<num_cus, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order, pool,
-                                                  pool_ch_bits, pool_mbase, part_out, part_slots, work, gsum, gcnt);
+        const int per_cu = pool_fmt == REC_COMPACT ? 2 : 1;
+        kernel<<<num_cus * per_cu, kReduceThreads, 0, s>>>(part_chunks, citem_begin, nparts, cpi, chunk_order,
+                                                           pool, pool_ch_bits, pool_mbase, part_out, part_slots,
+                                                           work, gsum, gcnt);
     };
     if (pool_fmt == REC_WIDE) launch(k_chunk_reduce<REC_WIDE>);
-    else if (pool_fmt == REC_COMPACT) launch(k_chunk_reduce<REC_COMPACT>);
+    else if (pool_fmt == REC_COMPACT) launch(k_chunk_reduce_compact);
     else launch(k_chunk_reduce<REC_NARROW>);
     return hipGetLastError();
 }
