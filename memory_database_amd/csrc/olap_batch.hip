@@ -127,12 +127,21 @@ __device__ __forceinline__ void b_atomic_lds(uint32_t *tab, uint32_t G, uint32_t
 // ---------------------------------------------------------------- masks
 
 // (neg: set when an attribute is negative, i.e. 2^31 or more as the kernels
-// read it)
-__global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32_t D,
-                              const MaskPred *preds, int np, unsigned long long base,
-                              unsigned long long *mask, uint4 *rec, uint32_t *pass, uint32_t *neg)
+// read it). The pass counts are summed per workgroup in LDS (one wave
+// popcount per predicate and wave, no contention between workgroups) and
+// added to `pass` once per workgroup; the grid is a few workgroups per CU
+// striding over the rows.
+constexpr int kMaskThreads = 256;
+__global__ __launch_bounds__(kMaskThreads) void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32_t D,
+                                                              const MaskPred *preds, int np, unsigned long long base,
+                                                              unsigned long long *mask, uint4 *rec, uint32_t *pass,
+                                                              uint32_t *neg)
 {
-    for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < D; r0 += gridDim.x * blockDim.x) {
+    __shared__ uint32_t cnt[4 * OLAP_BATCH_MAX];
+    for (int i = threadIdx.x; i < np; i += kMaskThreads) cnt[i] = 0u;
+    __syncthreads();
+    bool any_neg = false;
+    for (uint32_t r0 = blockIdx.x * kMaskThreads; r0 < D; r0 += gridDim.x * kMaskThreads) {
         const uint32_t r = r0 + threadIdx.x;
         const bool in = r < D;
         const int32_t f = in ? filter[r] : 0;
@@ -143,15 +152,19 @@ __global__ void k_batch_masks(const int32_t *filter, const int32_t *attr, uint32
                             (!pv || pv[r] != (int8_t)OLAP_DIM_NULL_I8);
             if (ok) m |= 1ull << preds[i].q;
             const unsigned long long b = __ballot(ok);
-            if ((threadIdx.x & 63) == 0 && b) atomicAdd(&pass[i], (uint32_t)__popcll(b));
+            if ((threadIdx.x & 63) == 0 && b) atomicAdd(&cnt[i], (uint32_t)__popcll(b));
         }
         if (in) {
             const int32_t at = attr ? attr[r] : 0;
             mask[r] = m;
             rec[r] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), (uint32_t)at, 0u);
-            if (at < 0) atomicOr(neg, 1u);
+            any_neg |= at < 0;
         }
     }
+    if (__ballot(any_neg) && (threadIdx.x & 63) == 0) atomicOr(neg, 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += kMaskThreads)
+        if (cnt[i]) atomicAdd(&pass[i], cnt[i]);
 }
 
 // min and max of a measure column: a grid-stride pass with 16-byte loads
@@ -1841,8 +1854,8 @@ olap_status olap::batch_create_in(olap_ctx ctx, const olap_star *star, const ola
     for (int t = 0; t < 4 && e == hipSuccess; t++) {
         const uint32_t D = (uint32_t)star->dim_rows[t];
         if (!((b->used >> t) & 1u) || !D) continue;
-        const int grid = (int)std::min<uint32_t>((D + 255) / 256, 2048);
-        k_batch_masks<<<grid, 256, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds + pbeg[t],
+        const int grid = (int)std::min<uint32_t>((D + kMaskThreads - 1) / kMaskThreads, 2u * (uint32_t)ctx->num_cus);
+        k_batch_masks<<<grid, kMaskThreads, 0, strm>>>(star->filter[t], star->attr[t], D, ws->preds + pbeg[t],
                                               (int)(pbeg[t + 1] - pbeg[t]), b->nouse[t], ws->mask[t], ws->rec[t],
                                               ws->pass + pbeg[t], ws->pass + kPassWords - 1);
         e = hipGetLastError();
