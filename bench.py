@@ -634,7 +634,8 @@ def main():
 
     # the autotuner times every launch shape once (untimed) and keeps the best
     shape = q.autotune(args.strategy, osum, ocnt, stream=s_handle)
-    log(f"autotune: shape {shape[0]} = {shape[1]} threads x {shape[2]} rows/thread "
+    th, rp, agg = q.launch_info()
+    log(f"autotune: shape {shape[0]} = {th} threads x {rp} rows/thread, {agg} "
         f"({shape[3]:.3f} ms)")
 
     def step():
