@@ -1013,10 +1013,8 @@ olap_status olap_query_run(olap_query q, olap_strategy strategy, int64_t *sum, i
     // a sum pointer is required by the kernels; counting-only runs reuse count
     int64_t *sum_p = sum ? sum : count;
     olap_status st = OLAP_OK;
-    if (sum) st = zero_outputs(q, sum, count, s);
-    else HIPC(ctx, launch_zero_u64((unsigned long long *)count, (size_t)q->plan.group_nums, s));
-    if (st != OLAP_OK) return st;
-    HIPC(ctx, launch_zero_u32(ctx->d_err, 1, s));
+    HIPC(ctx, launch_zero_run((unsigned long long *)sum, (unsigned long long *)count, (size_t)q->plan.group_nums,
+                              ctx->d_err, s));
     const bool vec = columns_vec_ok(q);
     switch (strategy) {
     case OLAP_ROWWISE:

@@ -188,6 +188,8 @@ hipError_t noalloc_scratch_init(int device, void **ptr);
 // launchers, olap_kernels.hip
 hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_zero_run(unsigned long long *sum, unsigned long long *count, size_t n, uint32_t *err,
+                           hipStream_t s);
 hipError_t launch_gen_uniform(int32_t *out, int64_t n, int64_t first, uint64_t key, int32_t lo,
                               int32_t hi, hipStream_t s);
 hipError_t launch_gen_zipf(int32_t *out, int64_t n, int64_t first, uint64_t key,

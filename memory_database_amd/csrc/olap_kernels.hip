@@ -65,6 +65,26 @@ __global__ void k_zero_u32(uint32_t *p, size_t n)
         p[i] = 0u;
 }
 
+// The outputs of a run and its error counter in one launch (a run's set-up
+// is otherwise three dependent launches of a few microseconds each); sum may
+// be null.
+__global__ void k_zero_run(unsigned long long *sum, unsigned long long *count, size_t n, uint32_t *err)
+{
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i0 == 0 && err) *err = 0u;
+    for (size_t i = i0; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (sum) sum[i] = 0ull;
+        count[i] = 0ull;
+    }
+}
+
+hipError_t launch_zero_run(unsigned long long *sum, unsigned long long *count, size_t n, uint32_t *err,
+                           hipStream_t s)
+{
+    k_zero_run<<<grid_for((int64_t)std::max<size_t>(n, 1), 256 * 4, 4096), 256, 0, s>>>(sum, count, n, err);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_u64(unsigned long long *p, size_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;

@@ -333,6 +333,40 @@ __global__ __launch_bounds__(256) void k_chunk_scatter(const unsigned long long 
     }
 }
 
+// A work item's chunks laid end to end by their filled 16-byte units (kper
+// records a unit): ustart[c] = units before chunk c, ustart[n] the total.
+// Wave 0 scans; the caller synchronises before use. A partly filled chunk
+// (a workgroup's last chunk of a partition) then costs only its records.
+__device__ __forceinline__ void unit_starts(const unsigned long long *lord, uint32_t n, uint32_t kper,
+                                            uint32_t *ustart)
+{
+    if (threadIdx.x >= 64u) return;
+    const uint32_t lane = threadIdx.x;
+    uint32_t carry = 0u;
+    for (uint32_t b = 0; b < n; b += 64u) {
+        const uint32_t c = b + lane;
+        uint32_t v = c < n ? ((uint32_t)(lord[c] >> 32) + kper - 1u) / kper : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, (unsigned)d);
+            if (lane >= (uint32_t)d) v += t;
+        }
+        if (c < n) ustart[c + 1u] = carry + v;
+        carry += __shfl(v, 63);
+    }
+    if (lane == 0u) ustart[0] = 0u;
+}
+
+// the chunk of the item holding filled unit i (ustart[c] <= i < ustart[c+1]),
+// by a branch-free binary search over ustart[0..n]
+__device__ __forceinline__ uint32_t unit_chunk(const uint32_t *ustart, uint32_t n, uint32_t i)
+{
+    uint32_t c = 0u;
+    for (uint32_t step = 1u << (31 - __clz(n)); step; step >>= 1)
+        if (c + step <= n && ustart[c + step] <= i) c += step;
+    return c;
+}
+
 // work items: up to cpi chunks of one partition, aggregated in an LDS table
 template <uint32_t FMT>
 __device__ __forceinline__ void chunk_reduce(
@@ -344,12 +378,12 @@ __device__ __forceinline__ void chunk_reduce(
     __shared__ unsigned long long lsum[kPartSlots];
     __shared__ uint32_t lcnt[kPartSlots];
     __shared__ unsigned long long lord[kReduceItemChunks];  // the item's (fill << 32 | chunk id)
+    __shared__ uint32_t ustart[kReduceItemChunks + 1];
     __shared__ uint32_t s_item, s_part;
     const uint4 *v4 = (const uint4 *)pool;
     // 16-byte units of a chunk: one WIDE record; four COMPACT records; or
     // four NARROW records, as 16 bytes of measures and 8 bytes of slots
     constexpr uint32_t kPer = FMT == REC_WIDE ? 1u : 4u;
-    const uint32_t ubits = FMT == REC_WIDE ? ch_bits : ch_bits - 2u;
     const unsigned long long ch = 1ull << ch_bits;
     for (;;) {
         reduce_claim(work, item_begin, nparts, s_item, s_part);
@@ -366,7 +400,13 @@ __device__ __forceinline__ void chunk_reduce(
             lcnt[s] = 0u;
         }
         __syncthreads();
-        const uint32_t units = n << ubits;
+        unit_starts(lord, n, kPer, ustart);
+        __syncthreads();
+        // items of mostly full chunks walk every unit (a bounds check skips
+        // the rest of the last ones); sparse items only the filled units
+        const uint32_t ubits = FMT == REC_WIDE ? ch_bits : ch_bits - 2u;
+        const bool filled_only = 2u * ustart[n] < (n << ubits);
+        const uint32_t units = filled_only ? ustart[n] : n << ubits;
         for (uint32_t i0 = threadIdx.x; i0 < units; i0 += (uint32_t)kReduceThreads * kReduceLoads) {
             uint4 v[kReduceLoads];
             uint2 gs[kReduceLoads];
@@ -379,8 +419,9 @@ __device__ __forceinline__ void chunk_reduce(
                 r[u] = 0u;
                 fill[u] = 0u;
                 if (i < units) {
-                    const unsigned long long o = lord[i >> ubits];
-                    const uint32_t w = i & ((1u << ubits) - 1u);
+                    const uint32_t c = filled_only ? unit_chunk(ustart, n, i) : i >> ubits;
+                    const unsigned long long o = lord[c];
+                    const uint32_t w = filled_only ? i - ustart[c] : i & ((1u << ubits) - 1u);
                     const unsigned long long cbase = (unsigned long long)(uint32_t)o << ch_bits;  // first record
                     r[u] = kPer * w;  // first record index of the unit within the chunk
                     fill[u] = (uint32_t)(o >> 32);
@@ -461,7 +502,6 @@ __global__ __launch_bounds__(kReduceThreads) __attribute__((amdgpu_waves_per_eu(
     __shared__ uint32_t s_item, s_part;
     typedef __attribute__((address_space(3))) unsigned long long l_u64;
     const uint4 *v4 = (const uint4 *)pool;
-    const uint32_t ubits = ch_bits - 2u;  // 16-byte units of a chunk: four records
     for (;;) {
         reduce_claim(work, item_begin, nparts, s_item, s_part);
         __syncthreads();
@@ -474,6 +514,10 @@ __global__ __launch_bounds__(kReduceThreads) __attribute__((amdgpu_waves_per_eu(
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lord[i] = order[cb + i];
         for (uint32_t s = threadIdx.x; s < slots; s += blockDim.x) lpack[s] = 0ull;
         __syncthreads();
+        // (every unit of the item's chunks: the batched scan's chunks are
+        // mostly full, and chunk_reduce's walk over filled units only would
+        // spill registers here)
+        const uint32_t ubits = ch_bits - 2u;  // 16-byte units of a chunk: four records
         const uint32_t units = n << ubits;
         for (uint32_t i0 = threadIdx.x; i0 < units; i0 += (uint32_t)kReduceThreads * kReduceLoads) {
             uint4 v[kReduceLoads];

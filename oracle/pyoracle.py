@@ -230,7 +230,12 @@ class Star:
 
 
 def run(strategy: str, p: OrcPlan, star: Star, nthreads: int = 1):
-    """Returns (sum int64[G], count int64[G], bad_rows)."""
+    """Returns (sum int64[G], count int64[G], bad_rows). The strategies read
+    M1 for every matching row, as the reference's workers do: a star without
+    a measure column is refused (count-only results are checked against a
+    run over a zero measure)."""
+    if star.m1 is None and star.cols.rows > 0:
+        raise ValueError("the oracle's strategies need a measure column")
     G = int(p.group_nums)
     s = np.zeros(G, dtype=np.int64)
     c = np.zeros(G, dtype=np.int64)

@@ -211,6 +211,63 @@ def test_staged_aggregation_matches_oracle(ctx, ncpu, G, sel, rows, two, skew):
     q.close()
 
 
+@pytest.mark.parametrize("case", ["streamed_second", "one_streamed", "count_only", "pool_overflow",
+                                  "tail_only", "one_dim"])
+def test_onepass_matches_oracle(ctx, ncpu, case):
+    """One-pass staged aggregation (layout 3): with the first step passing
+    most rows and almost none, counting only, a
+    pool sized from a pass estimate far below the real matches (records past
+    the pool are added directly), fewer rows than one tile (tail loop only),
+    and a single dimension; every slot equals the oracle's."""
+    import torch
+
+    rng = np.random.default_rng(sum(map(ord, case)))
+    G0 = 100_000  # groups of the first table; the second multiplies them by 3
+    rows = {"tail_only": 5_001, "pool_overflow": 1_500_000}.get(case, 2_000_003)
+    D = 50_000
+    thr = {"one_streamed": 1, "pool_overflow": 1}.get(case, 70)
+    f = rng.integers(0, 100, D, dtype=np.int32)
+    g = rng.integers(0, G0, D, dtype=np.int32)
+    dv = po.build_dimvec(f, g, 0, thr, 1, 4)
+    if case == "pool_overflow":  # every fact row finds a passing dimension row
+        passing = np.flatnonzero(f < thr).astype(np.int32)
+        fk0 = passing[rng.integers(0, passing.size, rows)]
+    else:
+        fk0 = rng.integers(0, D, rows, dtype=np.int32)
+    if case == "one_dim":
+        dvs, fks, orders, factor, G = [dv, None, None, None], [fk0, None, None, None], [0], [1], G0
+    else:
+        D2 = 2000
+        dv2 = po.build_dimvec(rng.integers(0, 100, D2, dtype=np.int32),
+                              rng.integers(0, 3, D2, dtype=np.int32), 0, 80, 1, 4)
+        dvs = [dv, None, None, dv2]
+        fks = [fk0, None, None, rng.integers(0, D2, rows, dtype=np.int32)]
+        orders, factor, G = [0, 3], [1, G0], 3 * G0
+    m1 = None if case == "count_only" else rng.integers(-2 ** 31, 2 ** 31 - 1, rows, dtype=np.int32)
+    m2 = rng.integers(0, 1000, rows, dtype=np.int32) if case == "streamed_second" else None
+    _say(f"one-pass staged {case}: rows={rows} G={G}")
+    p = po.plan(orders, factor, G)
+    m1_oracle = np.zeros(rows, np.int32) if m1 is None else m1  # the oracle always reads a measure
+    es, ec, bad = po.run("rowwise", p, po.Star(dvs, fks, m1_oracle, m2), nthreads=ncpu)
+    assert bad == 0
+    plan = oc.plan_from(orders, factor, G)
+    cols = oc.Context.columns([_dev(d, torch) for d in dvs], [_dev(x, torch) for x in fks],
+                              _dev(m1, torch), _dev(m2, torch), rows=rows)
+    q = ctx.query(plan, cols)
+    osum = torch.full((G,), -7, dtype=torch.int64, device="cuda")
+    ocnt = torch.full((G,), -7, dtype=torch.int64, device="cuda")
+    q.set_aggregation(3)
+    for rep in range(2):
+        q.run("rowwise", osum if m1 is not None else None, ocnt)
+        torch.cuda.synchronize()
+        th, rp, agg = q.launch_info()
+        assert agg == "one-pass staged"
+        np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"{case} counts, run {rep}")
+        if m1 is not None:
+            np.testing.assert_array_equal(osum.cpu().numpy(), es, err_msg=f"{case} sums, run {rep}")
+    q.close()
+
+
 def test_staged_forms_refuse_partitions_beyond_lds(ctx, ncpu):
     """2^27 groups make 16,384 partitions, whose per-partition state does not
     fit LDS: both staged forms are refused with E_CAPACITY, and the automatic
