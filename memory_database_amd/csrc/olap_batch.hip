@@ -437,7 +437,7 @@ __device__ __forceinline__ uint32_t fast_term(uint32_t attr, uint32_t m, uint32_
 // global atomics, or a staged record (the COUNT/SCATTER passes or a one-pass
 // chunk). `valid` false is a padding lane: it takes part in the wave's
 // ballots only.
-template <int MODE>
+template <int MODE, bool CPT = false>
 __device__ __forceinline__ void event_aggregate(const BatchArgs &a, bool valid, uint32_t gid, uint32_t G,
                                                 uint32_t lds_tab, uint32_t pbase, uint32_t out_off, uint32_t m1v,
                                                 uint32_t m2v, uint32_t *lds, stage_l_u32 *pc_cur,
@@ -447,8 +447,8 @@ __device__ __forceinline__ void event_aggregate(const BatchArgs &a, bool valid, 
     const bool ok = valid && gid < G;
     if (valid && gid >= G)
         __hip_atomic_fetch_add(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const long long mv = (long long)(int32_t)m1v + (long long)(int32_t)m2v;
-    const bool compact = MODE == AGG_CHUNK && a.chunk.fmt == REC_COMPACT;
+    const long long mv = (long long)(int32_t)m1v + (CPT ? 0ll : (long long)(int32_t)m2v);
+    const bool compact = MODE == AGG_CHUNK && (CPT || a.chunk.fmt == REC_COMPACT);
     const uint32_t pbits = compact ? kCompactPartBits : kPartBits;
     const uint32_t part = pbase + (gid >> pbits);
     const uint32_t glow = gid & ((1u << pbits) - 1u);
@@ -516,6 +516,8 @@ __device__ __forceinline__ void event_aggregate(const BatchArgs &a, bool valid, 
                 __hip_atomic_fetch_add(a.chunk.overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else if (compact) {
                 ((uint32_t *)a.chunk.pool)[base + off] = (glow << kCompactMeasureBits) | moff;
+            } else if (CPT) {
+                // (not reached: CPT kernels run COMPACT pools only)
             } else if (a.m2) {
                 ((uint4 *)a.chunk.pool)[base + off] = make_uint4(glow, m1v, m2v, 0u);
             } else {
@@ -544,7 +546,7 @@ __device__ __forceinline__ void event_aggregate(const BatchArgs &a, bool valid, 
 // aggregation. FAST: 1 the 32-bit modulo, 0 the 64-bit one, -1 as a.fastmod
 // says. The descriptor words every path needs are read before any branch,
 // so that their scalar loads are issued, and waited for, together.
-template <int MODE, int FAST = -1>
+template <int MODE, int FAST = -1, bool CPT = false>
 __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d, bool valid, uint32_t row,
                                             uint32_t *lds, const l_u32 *rattr, const l_u32 *rm, uint32_t R,
                                             stage_l_u32 *pc_cur, stage_l_u32 *pc_spare, stage_l_u32 *pc_fill,
@@ -560,7 +562,7 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
     const v4u a4 = ((l_cu4 *)rattr)[row];
     const uint32_t av[4] = {a4.x, a4.y, a4.z, a4.w};
     const uint32_t m1v = rm[row];
-    const uint32_t m2v = a.m2 ? rm[R + row] : 0u;
+    const uint32_t m2v = !CPT && a.m2 ? rm[R + row] : 0u;
     const uint32_t G = d.G;
     const uint32_t lds_tab = d.lds_tab;
     const uint32_t pbase = MODE == AGG_CHUNK ? d.pbase1 : d.pbase;
@@ -573,7 +575,7 @@ __device__ __forceinline__ void batch_event(const BatchArgs &a, const c_desc &d,
         for (int t = 0; t < 4; t++)
             gid += (uint32_t)__umul64hi(d.M[t] * (uint64_t)av[t], (uint64_t)d.g[t]) * d.fac[t];
     }
-    event_aggregate<MODE>(a, valid, gid, G, lds_tab, pbase, d.out_off, m1v, m2v, lds, pc_cur, pc_spare, pc_fill,
+    event_aggregate<MODE, CPT>(a, valid, gid, G, lds_tab, pbase, d.out_off, m1v, m2v, lds, pc_cur, pc_spare, pc_fill,
                           scur, sstart, CH, lane);
 }
 
@@ -931,7 +933,7 @@ __device__ __forceinline__ void batch_event_direct(const BatchArgs &a, uint32_t 
     b_atomic_dev(&a.gcnt[d.out_off + gid], 1ull);
 }
 
-template <int BS, int FAST>
+template <int BS, int FAST, bool CPT>
 __global__ __launch_bounds__(BS) void k_batch_seg(BatchArgs a)
 {
     constexpr int MODE = AGG_CHUNK;
@@ -1162,7 +1164,7 @@ __global__ __launch_bounds__(BS) void k_batch_seg(BatchArgs a)
             const bool valid = lane < n;
             const uint32_t row = valid ? e : 0u;
             const c_desc &d = ((const c_desc *)a.desc)[q];
-            batch_event<MODE, FAST>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, nullptr,
+            batch_event<MODE, FAST, CPT>(a, d, valid, row, lds, rattr, rm, R, pc_cur, pc_spare, pc_fill, nullptr,
                                     nullptr, CH, lane);
         }
         gather_second();
@@ -1224,13 +1226,18 @@ constexpr uint32_t kSegEvents = 32768;
 // the mask kernels' pass counts (one per query and table) and a flag
 constexpr uint32_t kPassWords = 4u * OLAP_BATCH_MAX + 1u;
 
+// fast: the batch takes the 32-bit modulo; cpt: its pool holds 4-byte
+// records (one measure), which some kernels are built for
 template <int MODE>
-static const void *batch_kernel(const BatchCfg &c, bool fast)
+static const void *batch_kernel(const BatchCfg &c, bool fast, bool cpt)
 {
     if (c.seg) {
         if (MODE != AGG_CHUNK) return nullptr;
-        if (fast) return c.bs == 512 ? (const void *)k_batch_seg<512, 1> : (const void *)k_batch_seg<1024, 1>;
-        return c.bs == 512 ? (const void *)k_batch_seg<512, 0> : (const void *)k_batch_seg<1024, 0>;
+#define OLAP_SEG(BSX, F) \
+    (cpt ? (const void *)k_batch_seg<BSX, F, true> : (const void *)k_batch_seg<BSX, F, false>)
+        if (fast) return c.bs == 512 ? OLAP_SEG(512, 1) : OLAP_SEG(1024, 1);
+        return c.bs == 512 ? OLAP_SEG(512, 0) : OLAP_SEG(1024, 0);
+#undef OLAP_SEG
     }
     if (c.grouped) return c.bs == 512 ? (const void *)k_batch_ev<512, MODE> : (const void *)k_batch_ev<1024, MODE>;
     if (MODE == AGG_CHUNK) return nullptr;  // one-pass staging exists in the grouped kernel only
@@ -1239,13 +1246,12 @@ static const void *batch_kernel(const BatchCfg &c, bool fast)
     return (const void *)k_batch<1024, 1, MODE>;
 }
 
-// (fast: the batch takes the 32-bit modulo, which some kernels are built for)
-static const void *batch_kernel_mode(int mode, const BatchCfg &c, bool fast)
+static const void *batch_kernel_mode(int mode, const BatchCfg &c, bool fast, bool cpt)
 {
-    if (mode == AGG_CHUNK) return batch_kernel<AGG_CHUNK>(c, fast);
-    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c, fast);
-    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c, fast);
-    return batch_kernel<AGG_DIRECT>(c, fast);
+    if (mode == AGG_CHUNK) return batch_kernel<AGG_CHUNK>(c, fast, cpt);
+    if (mode == AGG_COUNT) return batch_kernel<AGG_COUNT>(c, fast, cpt);
+    if (mode == AGG_SCATTER) return batch_kernel<AGG_SCATTER>(c, fast, cpt);
+    return batch_kernel<AGG_DIRECT>(c, fast, cpt);
 }
 
 // fact rows per tile of a configuration
@@ -1256,14 +1262,14 @@ static uint64_t batch_tile_rows(const BatchCfg &c)
 
 // grid of a configuration: persistent, limited by the least occupant of the
 // kernels it runs (COUNT and SCATTER of a staged run share one grid)
-static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, bool fast, int *grid)
+static hipError_t batch_grid(int cfg, uint32_t dyn, int num_cus, uint64_t rows, bool fast, bool cpt, int *grid)
 {
     const BatchCfg c = kBatchCfgs[cfg];
     const int modes[2] = {c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_COUNT : AGG_DIRECT,
                           c.staged == 2 ? AGG_CHUNK : c.staged ? AGG_SCATTER : AGG_DIRECT};
     int best = 1 << 30;
     for (int m : modes) {
-        const void *k = batch_kernel_mode(m, c, fast);
+        const void *k = batch_kernel_mode(m, c, fast, cpt);
         if (dyn > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
             if (e != hipSuccess) return e;
@@ -1288,7 +1294,8 @@ static hipError_t batch_launch(int cfg, int mode, BatchArgs &a, uint32_t dyn, in
     a.ntiles = (a.rows + tile - 1) / tile;
     if (a.ntiles == 0) return hipSuccess;
     void *args[] = {&a};
-    return hipLaunchKernel(batch_kernel_mode(mode, c, a.fastmod != 0u), dim3(grid), dim3(c.bs), args, dyn, s);
+    return hipLaunchKernel(batch_kernel_mode(mode, c, a.fastmod != 0u, a.chunk.fmt == REC_COMPACT), dim3(grid),
+                           dim3(c.bs), args, dyn, s);
 }
 
 }  // namespace olap
@@ -1661,7 +1668,7 @@ olap_status plan_config(olap_batch b, int c)
     a.desc = b->ws->desc[c];
     a.err = ctx->d_err;
     b->dyn[c] = std::max<uint32_t>(cur, 16u);
-    e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, a.fastmod != 0u, &b->grid[c]);
+    e = batch_grid(c, b->dyn[c], ctx->num_cus, a.rows, a.fastmod != 0u, !b->star.m2 && b->compact, &b->grid[c]);
     if (e != hipSuccess) return bhip(ctx, e, "batch launch plan");
     b->planned[c] = true;
     return OLAP_OK;
