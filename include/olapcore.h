@@ -207,7 +207,8 @@ olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *thre
  * 2 = the same, never streaming the second step's keys with the first's
  * (1 streams them when most of their cache lines hold a live row); 3 / 4 =
  * survivor queue: the streamed keys two tiles ahead, the rows that pass them
- * queued per wave and finished 64 / 256 at a time by gathers. */
+ * queued per wave and finished 64 / 256 at a time by gathers; 5 / 6 = the
+ * survivor queue never streaming the second step's keys. */
 int32_t olap_shape_pipeline(olap_strategy strategy, int32_t shape);
 olap_status olap_query_set_shape(olap_query q, int32_t shape);
 /* Aggregation layout of row-wise runs whose group table exceeds LDS:

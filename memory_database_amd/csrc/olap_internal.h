@@ -47,7 +47,8 @@ struct ScanShape {
     int unroll;  // 128-bit chunks per thread per tile
     int pipe;    // row-wise: 0 one tile at a time, 1 / 2 dependent fetches one tile
                  // ahead (2: second step never streamed), 3 / 4 survivor queue
-                 // drained 1 / 4 rows per lane at a time
+                 // drained 1 / 4 rows per lane at a time, 5 / 6 the same with
+                 // the second step never streamed
 };
 
 // One probe step as the kernels see it. Codes of the step's dimension are

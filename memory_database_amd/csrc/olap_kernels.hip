@@ -1451,11 +1451,13 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
 // Row-wise launch shapes: (threads per workgroup, 128-bit chunks per thread
 // per tile). The autotuner times them on a live query.
 // Shapes 6 and 7 run the pipelined kernel (every dependent fetch one tile
-// ahead), 8-11 the survivor-queue kernel (1024 threads, streamed keys 2
-// tiles ahead, drains of 1 or 4 rows per lane).
-const ScanShape kRowShapes[] = {{512, 4, 0}, {512, 2, 0}, {1024, 2, 0}, {1024, 4, 0},
-                                {256, 8, 0}, {512, 8, 0}, {1024, 2, 1}, {512, 2, 1},
-                                {1024, 2, 3}, {1024, 2, 4}, {1024, 4, 3}, {1024, 4, 4}};
+// ahead), 8-15 the survivor-queue kernel (1024 threads, streamed keys 2
+// tiles ahead, drains of 1 or 4 rows per lane); 12-15 never stream the
+// second step's keys with the first's, whatever share of their lines hold
+// a survivor (the queue's gathers then fetch only those lines).
+const ScanShape kRowShapes[] = {{512, 4, 0},  {512, 2, 0},  {1024, 2, 0}, {1024, 4, 0}, {256, 8, 0},  {512, 8, 0},
+                                {1024, 2, 1}, {512, 2, 1},  {1024, 2, 3}, {1024, 2, 4}, {1024, 4, 3}, {1024, 4, 4},
+                                {1024, 2, 5}, {1024, 2, 6}, {1024, 4, 5}, {1024, 4, 6}};
 const int kNumRowShapes = (int)(sizeof(kRowShapes) / sizeof(kRowShapes[0]));
 
 template <int NDIM, bool VEC, bool GLDS>
@@ -1470,11 +1472,11 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
         // with more rows than 32-bit row numbers, or queues that do not fit,
         // take the pipelined kernel of the same size
         const uint32_t qoff = (dyn_lds + 15u) & ~15u;
-        const int B = sh.pipe == 3 ? 1 : 4;
+        const int B = sh.pipe == 3 || sh.pipe == 5 ? 1 : 4;
         const uint32_t dyn = qoff + queue_lds_bytes(1024, B);
         if (a.rows <= 0xFFFFFFFFull && dyn <= kLdsBytes) {
             a.q_off = qoff / 4u;
-            const bool eager = NDIM > 1 && a.eager1;
+            const bool eager = NDIM > 1 && a.eager1 && sh.pipe <= 4;
             a.launched = OLAP_PIPE_QUEUE | (eager ? OLAP_PIPE_STREAM2 : 0);
 #define OLAP_QUEUE(UX, BX)                                                                                        \
     return eager ? launch_persistent(k_rowwise_queue<NDIM, GLDS, 1024, UX, true, 2, BX>, a, num_cus, dyn, a.ntiles, s, 1024) \
@@ -1489,7 +1491,7 @@ static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int 
         }
     }
     if (VEC && sh.pipe) {
-        const bool eager = NDIM > 1 && a.eager1 && sh.pipe != 2;
+        const bool eager = NDIM > 1 && a.eager1 && (sh.pipe == 1 || sh.pipe == 3 || sh.pipe == 4);
         a.launched = OLAP_PIPE_AHEAD | (eager ? OLAP_PIPE_STREAM2 : 0);
 #define OLAP_PIPE(BSX, UX)                                                                                  \
     return eager ? launch_persistent(k_rowwise_pipe<NDIM, GLDS, BSX, UX, true>, a, num_cus, dyn_lds, a.ntiles, s, BSX) \

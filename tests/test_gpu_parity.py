@@ -632,11 +632,11 @@ def test_row_shapes_every_pipeline_depth(ctx, ncpu, ndims, case):
             torch.cuda.synchronize()
             pipe = ctx.lib.olap_shape_pipeline(oc.ROWWISE, shape)
             ran = q.pipeline()
-            if pipe in (3, 4):  # queues that do not fit the LDS the query leaves take the pipelined form
+            if pipe in (3, 4, 5, 6):  # queues that do not fit the LDS the query leaves take the pipelined form
                 assert ran["survivor_queue"] or (layout == 2 and ran["fetch_ahead"])
             else:
                 assert ran["fetch_ahead"] == (pipe in (1, 2)) and not ran["survivor_queue"]
-            if pipe == 2:
+            if pipe in (2, 5, 6):
                 assert not ran["second_step_streamed"]
             np.testing.assert_array_equal(ocnt.cpu().numpy(), ec, err_msg=f"shape {shape} counts")
             if not count_only:
