@@ -1,5 +1,8 @@
 """Parity at the benchmarked sizes.
 
+- The timed plain query over the 600M-row fact table: every launch shape and
+  strategy agree slot by slot, and the totals equal the star-join sub-step's
+  passing rows and their measure sum.
 - The 64-query batch over the 600M-row fact table, autotuned, captured and
   replayed as a hipGraph exactly as it is timed: every query's slots equal
   the same query run as a plain row-wise scan, and every query's total count
@@ -33,6 +36,55 @@ def fact600(ctx):
     yield ds
     del ds
     torch.cuda.empty_cache()
+
+
+def test_c3_every_shape_and_strategy_agree_at_600M(ctx, fact600):
+    """The timed plain query (the 600M-row 4-dimension star, G = 875): every
+    row-wise launch shape, the autotuned one included, and every other
+    strategy give the same slots as the one-tile-at-a-time scan; the counts
+    sum to the rows the star-join sub-step (a different kernel) passes, and
+    the sums to the measure summed over exactly those rows."""
+    import torch
+
+    ds = fact600
+    G = int(ds.plan.group_nums)
+    q = ctx.query(ds.plan, ds.cols)
+    rs = torch.empty(G, dtype=torch.int64, device="cuda")
+    rc = torch.empty_like(rs)
+    q.set_shape(0)
+    q.run("rowwise", rs, rc)
+    torch.cuda.synchronize()
+    osum = torch.empty_like(rs)
+    ocnt = torch.empty_like(rs)
+    for shape in range(ctx.lib.olap_num_shapes(oc.ROWWISE)):
+        q.set_shape(shape)
+        ocnt.fill_(-1)
+        q.run("rowwise", osum, ocnt)
+        torch.cuda.synchronize()
+        ran = q.pipeline()
+        _say(f"C3 shape {shape}: {q.launch_info()} {ran}")
+        assert torch.equal(ocnt, rc) and torch.equal(osum, rs), f"shape {shape}"
+    q.set_shape(-1)
+    shape, threads, rpt, ms = q.autotune("rowwise", osum, ocnt)
+    torch.cuda.synchronize()
+    _say(f"C3 autotune: shape {shape} ({ms:.3f} ms)")
+    assert torch.equal(ocnt, rc) and torch.equal(osum, rs)
+    for strat in ("vwm_dv", "vwm_sv", "cwm_dv", "cwm_sv"):
+        ocnt.fill_(-1)
+        q.run(strat, osum, ocnt)
+        torch.cuda.synchronize()
+        assert torch.equal(ocnt, rc) and torch.equal(osum, rs), strat
+    q.close()
+    res = torch.empty(ds.cfg.rows, dtype=torch.int32, device="cuda")
+    n = ctx.starjoin("rowwise", ds.dimvecs, ds.fks, result=res)
+    passing = res != oc.STARJOIN_NULL
+    assert n == int(passing.sum().item()) == int(rc.sum().item())
+    msum = ds.m1.to(torch.int64)
+    if ds.m2 is not None:
+        msum = msum + ds.m2.to(torch.int64)
+    assert int(msum[passing].sum().item()) == int(rs.sum().item())
+    _say(f"C3: {n} passing rows, sum {int(rs.sum().item())}: equal")
+    del res, passing, msum
 
 
 def test_c4_batch_graph_equals_plain_scans_at_600M(ctx, fact600):
