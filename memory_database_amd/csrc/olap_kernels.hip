@@ -1131,6 +1131,16 @@ __global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
     constexpr int kU = kTileV / (kBlock * 4);
+    // software pipeline: the next tile's first key column is in flight while
+    // the current tile's lists are built
+    v4i nf[kU];
+    uint32_t nalive[kU];
+    if (blockIdx.x < a.ntiles) {
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+            nf[u] = load4<VEC>(a.st[0].fk, (uint64_t)blockIdx.x * kTileV + (uint64_t)(u * kBlock + tid) * 4u, rows,
+                               nalive[u]);
+    }
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile * (uint64_t)kTileV;
         v4i f[kU];
@@ -1138,8 +1148,14 @@ __global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
         uint32_t gid[kU][4];
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
-            f[u] = load4<VEC>(a.st[0].fk, r, rows, alive[u]);
+            f[u] = nf[u];
+            alive[u] = nalive[u];
+        }
+        const uint64_t next = tile + gridDim.x;
+        if (next < a.ntiles) {
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                nf[u] = load4<VEC>(a.st[0].fk, next * kTileV + (uint64_t)(u * kBlock + tid) * 4u, rows, nalive[u]);
         }
 #pragma unroll
         for (int u = 0; u < kU; u++) {
