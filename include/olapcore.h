@@ -210,6 +210,10 @@ olap_status olap_shape_info(olap_strategy strategy, int32_t shape, int32_t *thre
  * queued per wave and finished 64 / 256 at a time by gathers; 5 / 6 = the
  * survivor queue never streaming the second step's keys. */
 int32_t olap_shape_pipeline(olap_strategy strategy, int32_t shape);
+/* Fix a row-wise query's launch shape (0..olap_num_shapes-1), or -1: the
+ * shape the autotuner cached for the query's class, else a survivor-queue
+ * shape (11 or 15, the latter when step 1's keys are better gathered than
+ * streamed). */
 olap_status olap_query_set_shape(olap_query q, int32_t shape);
 /* Aggregation layout of row-wise runs whose group table exceeds LDS:
  * -1 automatic (from the expected matches: many matches take the one-pass

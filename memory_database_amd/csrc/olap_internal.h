@@ -162,6 +162,7 @@ struct olap_query_s {
     int agg_layout = -1;
     double est_events = 0.0;           // expected matches (uniform keys assumed)
     uint32_t eager1 = 0;               // row-wise: stream step 1's keys with step 0's
+    int32_t default_shape = 0;         // row-wise shape without a set or tuned one
     olap::Stager *stager = nullptr;
     uint32_t stage_grid = 0;
     // launch of the last run: threads per workgroup, rows per thread and the
@@ -214,6 +215,10 @@ hipError_t run_rowwise_chunk(ScanArgs &a, int ndims, uint32_t dyn_lds, uint32_t 
 hipError_t run_rowwise(ScanArgs &a, int ndims, int num_cus, uint32_t dyn_lds, bool vec,
                        int shape, hipStream_t s);
 int num_row_shapes();
+// row-wise shapes taken when none is set or tuned (survivor queue, 1024
+// threads x 16 rows, 256-row drains; with / without step 1 streamed)
+constexpr int kDefaultShapeStreamed = 11;
+constexpr int kDefaultShapeUnstreamed = 15;
 ScanShape row_shape(int i);
 // launch shapes of the one-pass and the two-pass staged row-wise scans
 ScanShape chunk_shape();

@@ -1471,10 +1471,16 @@ static hipError_t launch_persistent(K kernel, ScanArgs &a, int num_cus, uint32_t
 // tiles ahead, drains of 1 or 4 rows per lane); 12-15 never stream the
 // second step's keys with the first's, whatever share of their lines hold
 // a survivor (the queue's gathers then fetch only those lines).
-const ScanShape kRowShapes[] = {{512, 4, 0},  {512, 2, 0},  {1024, 2, 0}, {1024, 4, 0}, {256, 8, 0},  {512, 8, 0},
+constexpr ScanShape kRowShapes[] = {{512, 4, 0},  {512, 2, 0},  {1024, 2, 0}, {1024, 4, 0}, {256, 8, 0},  {512, 8, 0},
                                 {1024, 2, 1}, {512, 2, 1},  {1024, 2, 3}, {1024, 2, 4}, {1024, 4, 3}, {1024, 4, 4},
                                 {1024, 2, 5}, {1024, 2, 6}, {1024, 4, 5}, {1024, 4, 6}};
 const int kNumRowShapes = (int)(sizeof(kRowShapes) / sizeof(kRowShapes[0]));
+static_assert(kRowShapes[kDefaultShapeStreamed].bs == 1024 && kRowShapes[kDefaultShapeStreamed].unroll == 4 &&
+                  kRowShapes[kDefaultShapeStreamed].pipe == 4,
+              "default streamed shape");
+static_assert(kRowShapes[kDefaultShapeUnstreamed].bs == 1024 && kRowShapes[kDefaultShapeUnstreamed].unroll == 4 &&
+                  kRowShapes[kDefaultShapeUnstreamed].pipe == 6,
+              "default unstreamed shape");
 
 template <int NDIM, bool VEC, bool GLDS>
 static hipError_t rowwise_shape(ScanArgs &a, int num_cus, uint32_t dyn_lds, int shape,
