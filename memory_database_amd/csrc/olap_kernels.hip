@@ -1121,31 +1121,39 @@ __global__ __launch_bounds__(kBlock) void k_vwm_sv(ScanArgs a)
 template <int NDIM, bool VEC, bool GLDS>
 __global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
 {
+    // Each wave owns a block of kTileV / kWaves rows of the tile and its own
+    // slice of the LDS list: step 0 compacts the block's passing rows (row
+    // offset, partial gid) with ballot / mbcnt, every later step re-filters
+    // the list in place, and the survivors are aggregated. A wave's list is
+    // its own, so the tile loop has no workgroup barrier.
+    constexpr uint32_t kWaves = kBlock / 64u;
+    constexpr uint32_t kWaveRows = (uint32_t)kTileV / kWaves;  // rows, and list entries, of a wave
+    constexpr int kU = (int)(kWaveRows / 256u);                 // 128-bit chunks per lane
     extern __shared__ __align__(16) uint32_t lds[];
     lds_init<NDIM, GLDS>(a, lds);
-    uint32_t *lgid = lds + a.list_off;
-    uint16_t *loff = (uint16_t *)(lgid + kTileV);
-    uint32_t *counter = lgid + kTileV + kTileV / 2;
-    if (threadIdx.x == 0) *counter = 0u;
     __syncthreads();
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
-    constexpr int kU = kTileV / (kBlock * 4);
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    typedef __attribute__((address_space(3))) uint32_t l32;
+    typedef __attribute__((address_space(3))) uint16_t l16;
+    l32 *lgid = (l32 *)(lds + a.list_off) + wave * kWaveRows;
+    l16 *loff = (l16 *)((l32 *)(lds + a.list_off) + kTileV) + wave * kWaveRows;
+    auto row_of = [&](int u, int k) { return wave * kWaveRows + (uint32_t)(u * 64 + (int)lane) * 4u + (uint32_t)k; };
     // software pipeline: the next tile's first key column is in flight while
-    // the current tile's lists are built
+    // the current tile's list is built
     v4i nf[kU];
     uint32_t nalive[kU];
     if (blockIdx.x < a.ntiles) {
 #pragma unroll
         for (int u = 0; u < kU; u++)
-            nf[u] = load4<VEC>(a.st[0].fk, (uint64_t)blockIdx.x * kTileV + (uint64_t)(u * kBlock + tid) * 4u, rows,
-                               nalive[u]);
+            nf[u] = load4<VEC>(a.st[0].fk, (uint64_t)blockIdx.x * kTileV + row_of(u, 0), rows, nalive[u]);
     }
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile * (uint64_t)kTileV;
         v4i f[kU];
         uint32_t alive[kU];
-        uint32_t gid[kU][4];
 #pragma unroll
         for (int u = 0; u < kU; u++) {
             f[u] = nf[u];
@@ -1154,70 +1162,67 @@ __global__ __launch_bounds__(kBlock) void k_vwm_dv(ScanArgs a)
         const uint64_t next = tile + gridDim.x;
         if (next < a.ntiles) {
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                nf[u] = load4<VEC>(a.st[0].fk, next * kTileV + (uint64_t)(u * kBlock + tid) * 4u, rows, nalive[u]);
+            for (int u = 0; u < kU; u++) nf[u] = load4<VEC>(a.st[0].fk, next * kTileV + row_of(u, 0), rows, nalive[u]);
         }
+        // step 0: compact the passing rows into the wave's list
+        uint32_t n = 0u;
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            probe4(a.st[0], lds, f[u], gid[u], alive[u], true);
+            uint32_t gid[4];
+            probe4(a.st[0], lds, f[u], gid, alive[u], true);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool want = (alive[u] >> k) & 1u;
-                const uint32_t pos = wave_alloc(counter, want);
+                const unsigned long long m = __ballot(want);
+                const uint32_t pos = n + (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (want) {
-                    loff[pos] = (uint16_t)((u * kBlock + tid) * 4u + k);
-                    lgid[pos] = gid[u][k];
+                    loff[pos] = (uint16_t)row_of(u, k);
+                    lgid[pos] = gid[k];
                 }
+                n += (uint32_t)__popcll(m);
             }
         }
-        __syncthreads();
-        uint32_t n = *counter;
+        // later steps: re-filter the list in place, 64 entries at a time (a
+        // round reads its entries before it writes, and writes only at or
+        // below them)
 #pragma unroll
         for (int s = 1; s < NDIM; s++) {
-            uint32_t eoff[kPerThreadV], egid[kPerThreadV];
-            uint32_t ok = 0;
-#pragma unroll
-            for (int j = 0; j < kPerThreadV; j++) {
-                const uint32_t e = tid + j * kBlock;
-                if (e < n) {
-                    eoff[j] = loff[e];
-                    egid[j] = lgid[e];
-                    uint32_t code;
-                    const int32_t key = a.st[s].fk[base + eoff[j]];
-                    if (probe1_any(a.st[s], lds, key, code)) {
-                        egid[j] += code * a.st[s].factor;
-                        ok |= 1u << j;
-                    }
+            uint32_t kept = 0u;
+            for (uint32_t e0 = 0; e0 < n; e0 += 64u) {
+                const uint32_t e = e0 + lane;
+                const bool in = e < n;
+                const uint32_t off = in ? (uint32_t)loff[e] : 0u;
+                uint32_t g = in ? lgid[e] : 0u;
+                const int32_t key = in ? a.st[s].fk[base + off] : 0;
+                uint32_t code = 0u;
+                const bool pass = in && probe1_any(a.st[s], lds, key, code);
+                g += code * a.st[s].factor;
+                const unsigned long long m = __ballot(pass);
+                const uint32_t pos = kept + (uint32_t)__builtin_amdgcn_mbcnt_hi(
+                                                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (pass) {
+                    loff[pos] = (uint16_t)off;
+                    lgid[pos] = g;
                 }
+                kept += (uint32_t)__popcll(m);
             }
-            __syncthreads();
-            if (tid == 0) *counter = 0u;
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kPerThreadV; j++) {
-                const bool want = (ok >> j) & 1u;
-                const uint32_t pos = wave_alloc(counter, want);
-                if (want) {
-                    loff[pos] = (uint16_t)eoff[j];
-                    lgid[pos] = egid[j];
+            n = kept;
+        }
+        // the survivors' measures, then their aggregation
+        for (uint32_t e0 = 0; e0 < n; e0 += 64u) {
+            const uint32_t e = e0 + lane;
+            if (e < n) {
+                const uint64_t row = base + loff[e];
+                unsigned long long m = 0ull;
+                if (!a.count_only) {
+                    long long v = (long long)a.m1[row];
+                    if (a.m2) v += (long long)a.m2[row];
+                    m = (unsigned long long)v;
                 }
+                aggregate<GLDS>(a, lds, lgid[e], m);
             }
-            __syncthreads();
-            n = *counter;
         }
-        for (uint32_t e = tid; e < n; e += kBlock) {
-            const uint64_t row = base + loff[e];
-            unsigned long long m = 0ull;
-            if (!a.count_only) {
-                long long v = (long long)a.m1[row];
-                if (a.m2) v += (long long)a.m2[row];
-                m = (unsigned long long)v;
-            }
-            aggregate<GLDS>(a, lds, lgid[e], m);
-        }
-        __syncthreads();
-        if (tid == 0) *counter = 0u;
-        __syncthreads();
     }
     flush<GLDS>(a, lds);
 }
