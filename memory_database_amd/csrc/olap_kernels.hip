@@ -1258,6 +1258,7 @@ __global__ __launch_bounds__(kBlock) void k_cwm_sv_join(ScanArgs a)
             for (int k = 0; k < 4; k++) alive |= (g[k] != OLAP_GID_NULL ? 1u : 0u) << k;
             alive &= present;
         }
+        const uint32_t was_alive = alive;
         if (alive) {
             v4i f = load4<VEC>(st.fk, r, rows, dummy);
             uint32_t gg[4] = {(uint32_t)g.x, (uint32_t)g.y, (uint32_t)g.z, (uint32_t)g.w};
@@ -1265,6 +1266,10 @@ __global__ __launch_bounds__(kBlock) void k_cwm_sv_join(ScanArgs a)
             g = v4i{(int)gg[0], (int)gg[1], (int)gg[2], (int)gg[3]};
         }
         const int32_t nul = last ? a.gvec_null : OLAP_GID_NULL;
+        // a chunk whose rows an earlier step had all nulled keeps its words
+        // (most chunks after a selective first step), unless the last step
+        // writes a different null
+        if (!first && !was_alive && nul == OLAP_GID_NULL) continue;
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (!((alive >> k) & 1u)) g[k] = nul;
