@@ -1323,40 +1323,51 @@ __global__ __launch_bounds__(kBlock) void k_cwm_dv_join0(ScanArgs a)
     const StepDesc &st = a.st[0];
     if (st.in_lds)
         for (uint32_t i = threadIdx.x; i < st.nwords; i += kBlock) lds[st.lds_off + i] = st.words[i];
+    // two counters used by alternate tiles: a tile's count is published, and
+    // its counter cleared, after the next tile's barrier, so one barrier a
+    // tile suffices
     uint32_t *counter = lds + a.list_off;
-    if (threadIdx.x == 0) *counter = 0u;
+    if (threadIdx.x < 2u) counter[threadIdx.x] = 0u;
     __syncthreads();
     const uint64_t rows = a.rows;
     const uint32_t tid = threadIdx.x;
     constexpr int kU = kTileV / (kBlock * 4);
+    uint32_t par = 0u;
+    uint64_t prev = ~0ull;
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile * (uint64_t)kTileV;
         uint32_t *soid = a.list_oid + base;
         int32_t *sgid = a.list_gid + base;
+        v4i f[kU];
+        uint32_t alive[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+            f[u] = load4<VEC>(st.fk, base + (uint64_t)(u * kBlock + tid) * 4u, rows, alive[u]);
+        __syncthreads();  // the previous tile's list is complete
+        if (tid == 0 && prev != ~0ull) {
+            a.list_cnt[prev] = counter[par ^ 1u];
+            counter[par ^ 1u] = 0u;
+        }
 #pragma unroll
         for (int u = 0; u < kU; u++) {
             const uint64_t r = base + (uint64_t)(u * kBlock + tid) * 4u;
-            uint32_t alive;
-            v4i f = load4<VEC>(st.fk, r, rows, alive);
             uint32_t gid[4];
-            probe4(st, lds, f, gid, alive, true);
+            probe4(st, lds, f[u], gid, alive[u], true);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const bool want = (alive >> k) & 1u;
-                const uint32_t pos = wave_alloc(counter, want);
+                const bool want = (alive[u] >> k) & 1u;
+                const uint32_t pos = wave_alloc(counter + par, want);
                 if (want) {
                     soid[pos] = (uint32_t)(r + k);
                     sgid[pos] = (int32_t)gid[k];
                 }
             }
         }
-        __syncthreads();
-        if (tid == 0) {
-            a.list_cnt[tile] = *counter;
-            *counter = 0u;
-        }
-        __syncthreads();
+        prev = tile;
+        par ^= 1u;
     }
+    __syncthreads();
+    if (tid == 0 && prev != ~0ull) a.list_cnt[prev] = counter[par ^ 1u];
 }
 
 __global__ __launch_bounds__(kBlock) void k_cwm_dv_join(ScanArgs a)
@@ -1365,10 +1376,14 @@ __global__ __launch_bounds__(kBlock) void k_cwm_dv_join(ScanArgs a)
     const StepDesc &st = a.st[a.step];
     if (st.in_lds)
         for (uint32_t i = threadIdx.x; i < st.nwords; i += kBlock) lds[st.lds_off + i] = st.words[i];
+    // two counters used by alternate tiles (as in k_cwm_dv_join0): one
+    // barrier a tile, between reading a segment and compacting it in place
     uint32_t *counter = lds + a.list_off;
-    if (threadIdx.x == 0) *counter = 0u;
+    if (threadIdx.x < 2u) counter[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t tid = threadIdx.x;
+    uint32_t par = 0u;
+    uint64_t prev = ~0ull;
     for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
         const uint64_t base = tile * (uint64_t)kTileV;
         uint32_t *soid = a.list_oid + base;
@@ -1389,23 +1404,25 @@ __global__ __launch_bounds__(kBlock) void k_cwm_dv_join(ScanArgs a)
                 }
             }
         }
-        __syncthreads();  // every entry of the segment has been read
+        __syncthreads();  // every entry of the segment has been read; the previous tile's list is complete
+        if (tid == 0 && prev != ~0ull) {
+            a.list_cnt[prev] = counter[par ^ 1u];
+            counter[par ^ 1u] = 0u;
+        }
 #pragma unroll
         for (int j = 0; j < kPerThreadV; j++) {
             const bool want = (ok >> j) & 1u;
-            const uint32_t pos = wave_alloc(counter, want);
+            const uint32_t pos = wave_alloc(counter + par, want);
             if (want) {
                 soid[pos] = eoid[j];
                 sgid[pos] = (int32_t)egid[j];
             }
         }
-        __syncthreads();
-        if (tid == 0) {
-            a.list_cnt[tile] = *counter;
-            *counter = 0u;
-        }
-        __syncthreads();
+        prev = tile;
+        par ^= 1u;
     }
+    __syncthreads();
+    if (tid == 0 && prev != ~0ull) a.list_cnt[prev] = counter[par ^ 1u];
 }
 
 template <bool GLDS>
